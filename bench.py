@@ -1,0 +1,139 @@
+"""Benchmark: Gibbs sweeps/s of the BSGLMM probit sweep on one MI355X.
+
+Workload (the configuration the metric is quoted on): grid 91x109x91 with an
+ellipsoidal-shell mask of V = 230,906 voxels, N = 250 subjects, P = 4
+covariates, one chain, FP32 state, FP32-FMA X'Z, synthetic data from the
+model (bsglmm_amd.synth). A step is one full sweep: latent draws, X'Z,
+checkerboard coefficient updates (two colours), FP64 pair sums and the device
+Wishart draw. All inputs are resident in HBM before timing starts.
+
+Prints one JSON line. Also reports the dominant kernel's roofline (latent
+draws, algorithmic bytes N*V*(4 + 1/8) + 4*P*V per launch against 8 TB/s)
+and the reference CPU path timed on this host for a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gibbs sweeps/s + latent draws/s (N=250,V≈230k,P=4); ESS/s batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--cpu-sweeps", type=int, default=2, help="timed reference CPU sweeps")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "latent_c2_traffic.json"))
+    return ap.parse_args()
+
+
+def cpu_baseline(d, nsweeps):
+    """Reference CPU path (oracle/_ref: bsglmm/covar.cpp, randgen.cpp, cholesky.cpp
+    compiled unchanged) timed for a bounded number of sweeps on one pinned core."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle
+    if not oracle.ref_available():
+        return None
+    nx, ny, nz = d["grid"]
+    lay = oracle.layout(d["mask"], nx, ny, nz)
+    covar = d["covar"]
+    P, N = covar.shape
+    V, VP = lay["totvox"], lay["totvoxp"]
+    oracle.RefCPU(lay, covar, oracle.xtx_reference(covar))
+    R = oracle.ref()
+    z = np.zeros((N, V), np.float32)
+    data = np.ascontiguousarray(d["data"])
+    wm = np.ones(V, np.float32)
+    sc = np.zeros((P, VP), np.float32)
+    prec = np.eye(P, dtype=np.float32)
+    seed = oracle.seed_array([5118772087888418775, 289433066200759127, 1964621368])
+    f, u8 = C.c_float, C.c_uint8
+    args = (z.ctypes.data_as(C.POINTER(f)), data.ctypes.data_as(C.POINTER(u8)), wm.ctypes.data_as(C.POINTER(f)),
+            sc.ctypes.data_as(C.POINTER(f)), covar.ctypes.data_as(C.POINTER(f)),
+            prec.ctypes.data_as(C.POINTER(f)), seed)
+    old = os.sched_getaffinity(0)
+    try:
+        os.sched_setaffinity(0, {min(old)})
+        R.ref_sweep(*args)  # warm-up sweep
+        t0 = time.perf_counter()
+        for _ in range(nsweeps):
+            R.ref_sweep(*args)
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, old)
+    return {"value": nsweeps / dt, "unit": "sweeps/s", "cores": 1, "kind": "reference",
+            "sample": f"{nsweeps} full sweeps of the same workload after 1 warm-up sweep, "
+                      f"reference updateZ/updateSpatCoef/updateSpatCoefPrec on 1 pinned core "
+                      f"of {os.cpu_count()}; {dt:.1f} s",
+            "latent_draws_per_s": N * V * nsweeps / dt}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(the sweep shards over voxel slabs by design but this build measures one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    from bsglmm_amd import Layout, Sampler, synth
+
+    d = synth.generate(a.config)
+    nx, ny, nz = d["grid"]
+    L = Layout(d["mask"], nx, ny, nz)
+    P, N = d["covar"].shape
+    V = L.totvox
+    S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15)
+    S.sweep(a.warmup)
+    S.synchronize()
+    t0 = time.perf_counter()
+    kms = S.sweep_profiled(a.steps)
+    S.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = sum(kms.values())
+    ms_per_step = 1e3 * wall / a.steps
+    value = a.steps / wall
+
+    lat_bytes = N * V * (4 + 1 / 8) + 4 * P * V
+    lat_s = kms["latent"] / a.steps / 1e3
+    achieved = lat_bytes / lat_s / 1e9
+    traffic = None
+    if os.path.exists(a.traffic):
+        with open(a.traffic) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    sweep_bytes = N * V * (4 + 4 + 1 / 8) + V * (24 * P + 5)
+    out = {
+        "metric": METRIC, "value": value, "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{a.config}: grid {nx}x{ny}x{nz} shell mask, plain single chain, "
+                               f"FP32 state, FP32-FMA X'Z", "V": V, "N": N, "P": P, "chains": 1},
+        "latent_draws_per_s": value * N * V,
+        "kernel_ms_per_sweep": {k: v / a.steps for k, v in kms.items()},
+        "sweep_gpu_ms": gpu_ms / a.steps,
+        "sweep_hbm_frac": sweep_bytes / (gpu_ms / a.steps / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"kernel": "k_latent", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": lat_bytes},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(d, a.cpu_sweeps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,264 @@
+"""bsglmm_amd -- MI355X-native Gibbs sweep of BSGLMM behind a C ABI.
+
+The compute path is libbsglmm.so (hand-written HIP kernels for gfx950, see
+include/bsglmm.h). This module is a thin ctypes binding over that ABI, shaped
+like the reference's update functions (bsglmm/covarGPU.cu):
+
+    Layout(mask, nrow, ncol, ndepth)          main.cu:189-394 layout
+    Sampler.update_latent()                   updateZGPU          covarGPU.cu:1625
+    Sampler.compute_xtz() / update_coef(c)    updateSpatCoefGPU   covarGPU.cu:535
+    Sampler.update_car_prec()                 updateSpatCoefPrecGPU covarGPU.cu:1063
+    Sampler.sweep(n)                          mcmc.cpp:311-394 loop body
+
+There is no CPU fallback: if the library is missing, importing the binding
+raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbsglmm.so")
+
+_lib = None
+
+
+class BsgError(RuntimeError):
+    pass
+
+
+class _Opts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("nchains", C.c_int), ("xtz_mode", C.c_int),
+                ("df_offset", C.c_int), ("trunc_eps", C.c_float), ("pivot_min", C.c_float)]
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def lib():
+    """Load libbsglmm.so (built in-tree by bsglmm_amd.build). Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BsgError("libbsglmm.so is not built: run `python -m bsglmm_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    pi, pf, pd, pu8 = C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_uint8)
+    sig = {
+        "bsg_last_error": (C.c_char_p, []),
+        "bsg_layout_create": (C.c_int, [pu8, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+        "bsg_layout_destroy": (None, [vp]),
+        "bsg_layout_counts": (C.c_int, [vp, pi]),
+        "bsg_layout_export": (C.c_int, [vp, pi, pi, pi, pi, pu8, pu8, pi]),
+        "bsg_opts_default": (None, [C.POINTER(_Opts)]),
+        "bsg_create": (C.c_int, [vp, C.c_int, C.c_int, pf, pu8, pf, C.POINTER(_Opts), C.POINTER(vp)]),
+        "bsg_destroy": (C.c_int, [vp]),
+        "bsg_sweep": (C.c_int, [vp, C.c_int]),
+        "bsg_update_latent": (C.c_int, [vp]),
+        "bsg_compute_xtz": (C.c_int, [vp]),
+        "bsg_update_coef": (C.c_int, [vp, C.c_int]),
+        "bsg_update_car_prec": (C.c_int, [vp]),
+        "bsg_synchronize": (C.c_int, [vp]),
+        "bsg_sweep_profiled": (C.c_int, [vp, C.c_int, pd]),
+        "bsg_get_spat_coef": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_set_spat_coef": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_get_z": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_get_xtz": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_set_xtz": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_get_prec": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_set_prec": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_get_pair_sums": (C.c_int, [vp, C.c_int, pd]),
+        "bsg_get_sweep_index": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint32)]),
+        "bsg_set_sweep_index": (C.c_int, [vp, C.c_uint32]),
+        "bsg_get_fallback_count": (C.c_int, [vp, C.POINTER(C.c_uint32)]),
+        "bsg_kiss": (C.c_double, [C.POINTER(C.c_ulong)]),
+        "bsg_devseed": (C.c_uint64, [C.POINTER(C.c_ulong)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise BsgError(lib().bsg_last_error().decode(errors="replace") + f" (status {rc})")
+
+
+class Layout:
+    """The reference's mask layout (hostIdx, hostIdxSC, red/black INDX lists)."""
+
+    def __init__(self, mask, nrow, ncol, ndepth):
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).reshape(-1))
+        if m.size != (nrow + 2) * (ncol + 2) * (ndepth + 2):
+            raise ValueError("mask must be the padded (nrow+2)(ncol+2)(ndepth+2) grid")
+        self._h = C.c_void_p()
+        self.nrow, self.ncol, self.ndepth = nrow, ncol, ndepth
+        _check(lib().bsg_layout_create(_ptr(m, C.c_uint8), nrow, ncol, ndepth, C.byref(self._h)))
+        cnt = (C.c_int * 4)()
+        _check(lib().bsg_layout_counts(self._h, cnt))
+        self.totvox, self.totvoxp, self.n0, self.n1 = list(cnt)
+
+    def export(self):
+        nsize = (self.nrow + 2) * (self.ncol + 2) * (self.ndepth + 2)
+        out = dict(host_idx=np.empty(nsize, np.int32), host_idx_sc=np.empty(nsize, np.int32),
+                   vox0=np.empty(self.n0, np.int32), vox1=np.empty(self.n1, np.int32),
+                   nbrs0=np.empty(self.n0, np.uint8), nbrs1=np.empty(self.n1, np.uint8),
+                   nbr6=np.empty(self.totvox * 6, np.int32))
+        _check(lib().bsg_layout_export(
+            self._h, _ptr(out["host_idx"], C.c_int), _ptr(out["host_idx_sc"], C.c_int),
+            _ptr(out["vox0"], C.c_int), _ptr(out["vox1"], C.c_int), _ptr(out["nbrs0"], C.c_uint8),
+            _ptr(out["nbrs1"], C.c_uint8), _ptr(out["nbr6"], C.c_int)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().bsg_layout_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_opts():
+    o = _Opts()
+    lib().bsg_opts_default(C.byref(o))
+    return o
+
+
+class Sampler:
+    """One sampler context: device state for nchains chains over one data set."""
+
+    KERNELS = ("latent", "xtz", "coef0", "coef1", "pairs", "wishart")
+
+    def __init__(self, layout, covar, data, wm=None, seed=0, nchains=1, xtz_mode=0,
+                 df_offset=19, trunc_eps=1e-6, pivot_min=1e-6):
+        covar = np.ascontiguousarray(covar, dtype=np.float32)
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self.P, self.N = covar.shape
+        self.layout = layout
+        self.V, self.VP = layout.totvox, layout.totvoxp
+        if data.shape != (self.N, self.V):
+            raise ValueError(f"data must be uint8[N={self.N}][TOTVOX={self.V}]")
+        o = default_opts()
+        o.seed, o.nchains, o.xtz_mode = int(seed), int(nchains), int(xtz_mode)
+        o.df_offset, o.trunc_eps, o.pivot_min = int(df_offset), float(trunc_eps), float(pivot_min)
+        self.nchains = int(nchains)
+        wmp = None
+        if wm is not None:
+            self._wm = np.ascontiguousarray(wm, dtype=np.float32)
+            wmp = _ptr(self._wm, C.c_float)
+        self._h = C.c_void_p()
+        _check(lib().bsg_create(layout._h, self.N, self.P, _ptr(covar, C.c_float), _ptr(data, C.c_uint8),
+                                wmp, C.byref(o), C.byref(self._h)))
+
+    # ---- sweep steps ----
+    def sweep(self, n=1):
+        _check(lib().bsg_sweep(self._h, int(n)))
+
+    def sweep_profiled(self, n):
+        ms = np.zeros(6, dtype=np.float64)
+        _check(lib().bsg_sweep_profiled(self._h, int(n), _ptr(ms, C.c_double)))
+        return dict(zip(self.KERNELS, ms))
+
+    def update_latent(self):
+        _check(lib().bsg_update_latent(self._h))
+
+    def compute_xtz(self):
+        _check(lib().bsg_compute_xtz(self._h))
+
+    def update_coef(self, colour):
+        _check(lib().bsg_update_coef(self._h, int(colour)))
+
+    def update_car_prec(self):
+        _check(lib().bsg_update_car_prec(self._h))
+
+    def synchronize(self):
+        _check(lib().bsg_synchronize(self._h))
+
+    # ---- state, reference layouts ----
+    def get_spat_coef(self, chain=0):
+        out = np.empty((self.P, self.VP), np.float32)
+        _check(lib().bsg_get_spat_coef(self._h, chain, _ptr(out, C.c_float)))
+        return out
+
+    def set_spat_coef(self, sc, chain=0):
+        sc = np.ascontiguousarray(sc, dtype=np.float32)
+        assert sc.shape == (self.P, self.VP)
+        _check(lib().bsg_set_spat_coef(self._h, chain, _ptr(sc, C.c_float)))
+
+    def get_z(self, chain=0):
+        out = np.empty((self.N, self.V), np.float32)
+        _check(lib().bsg_get_z(self._h, chain, _ptr(out, C.c_float)))
+        return out
+
+    def get_xtz(self, chain=0):
+        out = np.empty((self.P, self.V), np.float32)
+        _check(lib().bsg_get_xtz(self._h, chain, _ptr(out, C.c_float)))
+        return out
+
+    def set_xtz(self, m, chain=0):
+        m = np.ascontiguousarray(m, dtype=np.float32)
+        _check(lib().bsg_set_xtz(self._h, chain, _ptr(m, C.c_float)))
+
+    def get_prec(self, chain=0):
+        out = np.empty((self.P, self.P), np.float32)
+        _check(lib().bsg_get_prec(self._h, chain, _ptr(out, C.c_float)))
+        return out
+
+    def set_prec(self, prec, chain=0):
+        prec = np.ascontiguousarray(prec, dtype=np.float32)
+        _check(lib().bsg_set_prec(self._h, chain, _ptr(prec, C.c_float)))
+
+    def get_pair_sums(self, chain=0):
+        out = np.empty((self.P, self.P), np.float64)
+        _check(lib().bsg_get_pair_sums(self._h, chain, _ptr(out, C.c_double)))
+        return out
+
+    def sweep_index(self, chain=0):
+        v = C.c_uint32()
+        _check(lib().bsg_get_sweep_index(self._h, chain, C.byref(v)))
+        return v.value
+
+    def set_sweep_index(self, sweep):
+        _check(lib().bsg_set_sweep_index(self._h, C.c_uint32(sweep)))
+
+    def fallback_count(self):
+        v = C.c_uint32()
+        _check(lib().bsg_get_fallback_count(self._h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._h:
+            lib().bsg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kiss(seed):
+    """One KISS draw (randgen.cpp:9-23); seed is a length-3 uint64 array, advanced in place."""
+    s = (C.c_ulong * 3)(*[int(x) for x in seed])
+    u = lib().bsg_kiss(s)
+    seed[:] = list(s)
+    return u
+
+
+def devseed(seed):
+    """Philox key from a seed.dat triple, as main.cu:413 derives devseed."""
+    s = (C.c_ulong * 3)(*[int(x) for x in seed])
+    v = lib().bsg_devseed(s)
+    seed[:] = list(s)
+    return v

@@ -1,0 +1,573 @@
+// bsg_ctx.cpp -- C ABI of libbsglmm: context, data upload, sweep, state exchange.
+#include <math.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "../../include/bsglmm.h"
+#include "bsg_internal.h"
+
+namespace bsg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+}  // namespace bsg
+
+using bsg::set_error;
+
+struct bsg_layout {
+  bsg::Layout L;
+};
+
+struct bsg_ctx {
+  bsg::Layout L;
+  bsg_opts opts{};
+  int N = 0, P = 0, NS = 0, NQ = 0, NW = 0, C = 1;
+  hipStream_t stream = nullptr;
+  bsg::DevProblem pb{};
+  std::vector<void *> allocs;
+  hipEvent_t ev[7] = {};
+
+  void release()
+  {
+    if (stream) hipStreamSynchronize(stream);
+    for (void *p : allocs) hipFree(p);
+    allocs.clear();
+    for (auto &e : ev) {
+      if (e) hipEventDestroy(e);
+      e = nullptr;
+    }
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+  ~bsg_ctx() { release(); }
+};
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+      return BSG_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+template <typename T>
+static int dev_alloc(bsg_ctx *ctx, T **p, size_t count)
+{
+  void *ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return BSG_ERR_NOMEM;
+  }
+  ctx->allocs.push_back(ptr);
+  *p = static_cast<T *>(ptr);
+  return BSG_OK;
+}
+
+extern "C" {
+
+const char *bsg_last_error(void) { return bsg::g_last_error.c_str(); }
+
+int bsg_layout_create(const unsigned char *mask, int nrow, int ncol, int ndepth, bsg_layout **out)
+{
+  if (!out) {
+    set_error("bsg_layout_create: out is NULL");
+    return BSG_ERR_ARG;
+  }
+  *out = nullptr;
+  auto *lay = new (std::nothrow) bsg_layout;
+  if (!lay) {
+    set_error("bsg_layout_create: out of host memory");
+    return BSG_ERR_NOMEM;
+  }
+  if (bsg::build_layout(mask, nrow, ncol, ndepth, lay->L) != 0) {
+    delete lay;
+    return BSG_ERR_ARG;
+  }
+  *out = lay;
+  return BSG_OK;
+}
+
+void bsg_layout_destroy(bsg_layout *layout) { delete layout; }
+
+int bsg_layout_counts(const bsg_layout *layout, int counts[4])
+{
+  if (!layout || !counts) {
+    set_error("bsg_layout_counts: NULL argument");
+    return BSG_ERR_ARG;
+  }
+  counts[0] = layout->L.totvox;
+  counts[1] = layout->L.totvoxp;
+  counts[2] = layout->L.n[0];
+  counts[3] = layout->L.n[1];
+  return BSG_OK;
+}
+
+int bsg_layout_export(const bsg_layout *layout, int *host_idx, int *host_idx_sc, int *vox0,
+                      int *vox1, unsigned char *nbrs0, unsigned char *nbrs1, int *nbr6)
+{
+  if (!layout) {
+    set_error("bsg_layout_export: NULL layout");
+    return BSG_ERR_ARG;
+  }
+  const bsg::Layout &L = layout->L;
+  if (host_idx) memcpy(host_idx, L.host_idx.data(), sizeof(int) * L.nsize);
+  if (host_idx_sc) memcpy(host_idx_sc, L.host_idx_sc.data(), sizeof(int) * L.nsize);
+  if (vox0) memcpy(vox0, L.vox[0].data(), sizeof(int) * L.n[0]);
+  if (vox1) memcpy(vox1, L.vox[1].data(), sizeof(int) * L.n[1]);
+  if (nbrs0) memcpy(nbrs0, L.nbrs[0].data(), L.n[0]);
+  if (nbrs1) memcpy(nbrs1, L.nbrs[1].data(), L.n[1]);
+  if (nbr6) memcpy(nbr6, L.nbr6.data(), sizeof(int) * L.nbr6.size());
+  return BSG_OK;
+}
+
+void bsg_opts_default(bsg_opts *o)
+{
+  if (!o) return;
+  o->seed = 0;
+  o->nchains = 1;
+  o->xtz_mode = 0;
+  o->df_offset = 19;
+  o->trunc_eps = 1e-6f;
+  o->pivot_min = 1e-6f;
+}
+
+int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *covar,
+               const unsigned char *data, const float *wm, const bsg_opts *opts, bsg_ctx **out)
+{
+  if (!out) {
+    set_error("bsg_create: out is NULL");
+    return BSG_ERR_ARG;
+  }
+  *out = nullptr;
+  if (!layout || !covar || !data) {
+    set_error("bsg_create: NULL layout, covar or data");
+    return BSG_ERR_ARG;
+  }
+  if (ncovar < 1 || ncovar > BSG_MAX_COVAR) {
+    set_error("bsg_create: number of covariates must be in 1..8");
+    return BSG_ERR_ARG;
+  }
+  if (nsubs < 1) {
+    set_error("bsg_create: no subjects");
+    return BSG_ERR_ARG;
+  }
+  bsg_opts o;
+  bsg_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.nchains < 1 || o.nchains > 4096) {
+    set_error("bsg_create: nchains must be in 1..4096");
+    return BSG_ERR_ARG;
+  }
+  if (o.xtz_mode != 0 && o.xtz_mode != 1) {
+    set_error("bsg_create: xtz_mode must be 0 or 1");
+    return BSG_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("bsg_create: no HIP device");
+    return BSG_ERR_HIP;
+  }
+
+  std::unique_ptr<bsg_ctx> ctx(new (std::nothrow) bsg_ctx);
+  if (!ctx) {
+    set_error("bsg_create: out of host memory");
+    return BSG_ERR_NOMEM;
+  }
+  ctx->L = layout->L;
+  ctx->opts = o;
+  const bsg::Layout &L = ctx->L;
+  const int V = L.totvox, N = nsubs, P = ncovar, C = o.nchains;
+  ctx->N = N;
+  ctx->P = P;
+  ctx->C = C;
+  ctx->NS = (N + 3) & ~3;
+  ctx->NQ = (N + 3) / 4;
+  ctx->NW = (N + 31) / 32;
+  const int NS = ctx->NS, NW = ctx->NW;
+  const long long df = (long long)V + o.df_offset;
+  if (df < P || df > (1LL << 30)) {
+    set_error("bsg_create: Wishart degrees of freedom out of range");
+    return BSG_ERR_ARG;
+  }
+  if ((long long)V * ctx->NQ >= (1LL << 32)) {
+    set_error("bsg_create: TOTVOX * ceil(N/4) must be below 2^32 (Philox counter)");
+    return BSG_ERR_ARG;
+  }
+
+  // ---- host-side preparation in sorted order ----
+  std::vector<uint32_t> ybits((size_t)V * NW, 0u);
+  for (int s = 0; s < V; s++) {
+    const int rank = L.rank_of_sorted[s];
+    for (int i = 0; i < N; i++)
+      if (data[(size_t)i * V + rank] == 1) ybits[(size_t)s * NW + (i >> 5)] |= 1u << (i & 31);
+  }
+  std::vector<float> X((size_t)P * NS, 0.0f), xsum(P, 0.0f), xtx((size_t)P * P, 0.0f);
+  for (int c = 0; c < P; c++)
+    for (int i = 0; i < N; i++) X[(size_t)c * NS + i] = covar[(size_t)c * N + i];
+  // X'X accumulated in float, subject-outer, as read_data.cpp:321-327
+  for (int i = 0; i < N; i++)
+    for (int a = 0; a < P; a++)
+      for (int b = 0; b < P; b++) xtx[b + a * P] += covar[(size_t)a * N + i] * covar[(size_t)b * N + i];
+  for (int c = 0; c < P; c++) {
+    float acc = 0.0f;
+    for (int i = 0; i < N; i++) acc += covar[(size_t)c * N + i];
+    xsum[c] = acc;
+  }
+  std::vector<float> wms(V, 1.0f);
+  if (wm)
+    for (int s = 0; s < V; s++) wms[s] = wm[L.rank_of_sorted[s]];
+
+  // ---- device state ----
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  bsg::DevProblem &pb = ctx->pb;
+  pb.V = V;
+  pb.N = N;
+  pb.NS = NS;
+  pb.NQ = ctx->NQ;
+  pb.NW = NW;
+  pb.P = P;
+  pb.nchains = C;
+  pb.n0 = L.n[0];
+  pb.key0 = (uint32_t)(o.seed & 0xffffffffu);
+  pb.key1 = (uint32_t)(o.seed >> 32);
+  pb.df = (int)df;
+  pb.trunc_eps = o.trunc_eps;
+  pb.pivot_min = o.pivot_min;
+  const int NE = P * (P + 1) / 2;
+  int rc;
+  int *nbr6;
+  uint32_t *yb;
+  float *Xd, *xtxd, *xsumd, *wmd;
+#define ALLOC(ptr, n)                                   \
+  if ((rc = dev_alloc(ctx.get(), &(ptr), (n))) != BSG_OK) return rc;
+  ALLOC(nbr6, (size_t)V * 6);
+  ALLOC(yb, (size_t)V * NW);
+  ALLOC(Xd, (size_t)P * NS);
+  ALLOC(xtxd, (size_t)P * P);
+  ALLOC(xsumd, (size_t)P);
+  ALLOC(wmd, (size_t)V);
+  ALLOC(pb.Z, (size_t)C * V * NS);
+  ALLOC(pb.beta, (size_t)C * V * P);
+  ALLOC(pb.M, (size_t)C * V * P);
+  ALLOC(pb.lam, (size_t)C * P * P);
+  ALLOC(pb.lam64, (size_t)C * P * P);
+  ALLOC(pb.tsum, (size_t)C * P * P);
+  ALLOC(pb.partials, (size_t)C * bsg::PAIR_BLOCKS * NE);
+  ALLOC(pb.bwm, (size_t)C);
+  ALLOC(pb.sweep, (size_t)C);
+  ALLOC(pb.flags, (size_t)4);
+#undef ALLOC
+  pb.nbr6 = nbr6;
+  pb.ybits = yb;
+  pb.X = Xd;
+  pb.xtx = xtxd;
+  pb.xsum = xsumd;
+  pb.wm = wmd;
+
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipMemcpyAsync(nbr6, L.nbr6.data(), sizeof(int) * L.nbr6.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(yb, ybits.data(), sizeof(uint32_t) * ybits.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(Xd, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(xtxd, xtx.data(), sizeof(float) * xtx.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(xsumd, xsum.data(), sizeof(float) * P, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(wmd, wms.data(), sizeof(float) * V, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(pb.Z, 0, sizeof(float) * (size_t)C * V * NS, st));
+  HIPCHK(hipMemsetAsync(pb.beta, 0, sizeof(float) * (size_t)C * V * P, st));
+  HIPCHK(hipMemsetAsync(pb.M, 0, sizeof(float) * (size_t)C * V * P, st));
+  HIPCHK(hipMemsetAsync(pb.tsum, 0, sizeof(double) * (size_t)C * P * P, st));
+  HIPCHK(hipMemsetAsync(pb.bwm, 0, sizeof(float) * C, st));
+  HIPCHK(hipMemsetAsync(pb.sweep, 0, sizeof(uint32_t) * C, st));
+  HIPCHK(hipMemsetAsync(pb.flags, 0, sizeof(uint32_t) * 4, st));
+  std::vector<float> lam((size_t)C * P * P, 0.0f);
+  std::vector<double> lam64((size_t)C * P * P, 0.0);
+  for (int c = 0; c < C; c++)
+    for (int a = 0; a < P; a++) {
+      lam[(size_t)c * P * P + a * P + a] = 1.0f;  // Lambda = I (mcmc.cpp:150-154)
+      lam64[(size_t)c * P * P + a * P + a] = 1.0;
+    }
+  HIPCHK(hipMemcpyAsync(pb.lam, lam.data(), sizeof(float) * lam.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(pb.lam64, lam64.data(), sizeof(double) * lam64.size(), hipMemcpyHostToDevice, st));
+  for (auto &e : ctx->ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipStreamSynchronize(st));
+  *out = ctx.release();
+  return BSG_OK;
+}
+
+int bsg_destroy(bsg_ctx *ctx)
+{
+  delete ctx;
+  return BSG_OK;
+}
+
+#define CTXCHK(ctx)                         \
+  if (!(ctx)) {                             \
+    set_error("NULL context");              \
+    return BSG_ERR_ARG;                     \
+  }
+
+int bsg_update_latent(bsg_ctx *ctx)
+{
+  CTXCHK(ctx);
+  HIPCHK(bsg::launch_latent(ctx->pb, ctx->stream));
+  return BSG_OK;
+}
+
+int bsg_compute_xtz(bsg_ctx *ctx)
+{
+  CTXCHK(ctx);
+  HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, ctx->stream));
+  return BSG_OK;
+}
+
+int bsg_update_coef(bsg_ctx *ctx, int colour)
+{
+  CTXCHK(ctx);
+  if (colour != 0 && colour != 1) {
+    set_error("bsg_update_coef: colour must be 0 or 1");
+    return BSG_ERR_ARG;
+  }
+  HIPCHK(bsg::launch_coef(ctx->pb, colour, ctx->stream));
+  return BSG_OK;
+}
+
+int bsg_update_car_prec(bsg_ctx *ctx)
+{
+  CTXCHK(ctx);
+  HIPCHK(bsg::launch_pairs(ctx->pb, ctx->stream));
+  HIPCHK(bsg::launch_wishart(ctx->pb, ctx->stream));
+  return BSG_OK;
+}
+
+static int sweep_once(bsg_ctx *ctx)
+{
+  hipStream_t st = ctx->stream;
+  HIPCHK(bsg::launch_latent(ctx->pb, st));
+  HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
+  HIPCHK(bsg::launch_coef(ctx->pb, 0, st));
+  HIPCHK(bsg::launch_coef(ctx->pb, 1, st));
+  HIPCHK(bsg::launch_pairs(ctx->pb, st));
+  HIPCHK(bsg::launch_wishart(ctx->pb, st));
+  return BSG_OK;
+}
+
+int bsg_sweep(bsg_ctx *ctx, int nsweeps)
+{
+  CTXCHK(ctx);
+  if (nsweeps < 0) {
+    set_error("bsg_sweep: negative sweep count");
+    return BSG_ERR_ARG;
+  }
+  for (int i = 0; i < nsweeps; i++) {
+    int rc = sweep_once(ctx);
+    if (rc != BSG_OK) return rc;
+  }
+  return BSG_OK;
+}
+
+int bsg_synchronize(bsg_ctx *ctx)
+{
+  CTXCHK(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  uint32_t flags[4];
+  HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
+  if (flags[1]) {
+    set_error("coefficient update: " + std::to_string(flags[1]) +
+              " voxel(s) had a non positive definite posterior precision");
+    return BSG_ERR_NUM;
+  }
+  return BSG_OK;
+}
+
+int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
+{
+  CTXCHK(ctx);
+  if (!kernel_ms || nsweeps < 0) {
+    set_error("bsg_sweep_profiled: bad argument");
+    return BSG_ERR_ARG;
+  }
+  for (int k = 0; k < 6; k++) kernel_ms[k] = 0.0;
+  hipStream_t st = ctx->stream;
+  for (int i = 0; i < nsweeps; i++) {
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(bsg::launch_latent(ctx->pb, st));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, st));
+    HIPCHK(hipEventRecord(ctx->ev[3], st));
+    HIPCHK(bsg::launch_coef(ctx->pb, 1, st));
+    HIPCHK(hipEventRecord(ctx->ev[4], st));
+    HIPCHK(bsg::launch_pairs(ctx->pb, st));
+    HIPCHK(hipEventRecord(ctx->ev[5], st));
+    HIPCHK(bsg::launch_wishart(ctx->pb, st));
+    HIPCHK(hipEventRecord(ctx->ev[6], st));
+    HIPCHK(hipEventSynchronize(ctx->ev[6]));
+    for (int k = 0; k < 6; k++) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+      kernel_ms[k] += ms;
+    }
+  }
+  return BSG_OK;
+}
+
+// ---- state exchange ----------------------------------------------------
+#define CHAINCHK(ctx, chain)                   \
+  CTXCHK(ctx);                                 \
+  if ((chain) < 0 || (chain) >= (ctx)->C) {    \
+    set_error("chain index out of range");     \
+    return BSG_ERR_ARG;                        \
+  }
+
+int bsg_get_spat_coef(bsg_ctx *ctx, int chain, float *spat_coef)
+{
+  CHAINCHK(ctx, chain);
+  const int V = ctx->L.totvox, VP = ctx->L.totvoxp, P = ctx->P;
+  std::vector<float> b((size_t)V * P);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(b.data(), ctx->pb.beta + (size_t)chain * V * P, sizeof(float) * b.size(),
+                   hipMemcpyDeviceToHost));
+  memset(spat_coef, 0, sizeof(float) * (size_t)P * VP);
+  for (int s = 0; s < V; s++)
+    for (int c = 0; c < P; c++) spat_coef[(size_t)c * VP + ctx->L.sc_of_sorted[s]] = b[(size_t)s * P + c];
+  return BSG_OK;
+}
+
+int bsg_set_spat_coef(bsg_ctx *ctx, int chain, const float *spat_coef)
+{
+  CHAINCHK(ctx, chain);
+  const int V = ctx->L.totvox, VP = ctx->L.totvoxp, P = ctx->P;
+  std::vector<float> b((size_t)V * P);
+  for (int s = 0; s < V; s++)
+    for (int c = 0; c < P; c++) b[(size_t)s * P + c] = spat_coef[(size_t)c * VP + ctx->L.sc_of_sorted[s]];
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(ctx->pb.beta + (size_t)chain * V * P, b.data(), sizeof(float) * b.size(),
+                   hipMemcpyHostToDevice));
+  return BSG_OK;
+}
+
+int bsg_get_z(bsg_ctx *ctx, int chain, float *z)
+{
+  CHAINCHK(ctx, chain);
+  const int V = ctx->L.totvox, N = ctx->N, NS = ctx->NS;
+  std::vector<float> zz((size_t)V * NS);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(zz.data(), ctx->pb.Z + (size_t)chain * V * NS, sizeof(float) * zz.size(),
+                   hipMemcpyDeviceToHost));
+  for (int s = 0; s < V; s++) {
+    const int rank = ctx->L.rank_of_sorted[s];
+    for (int i = 0; i < N; i++) z[(size_t)i * V + rank] = zz[(size_t)s * NS + i];
+  }
+  return BSG_OK;
+}
+
+int bsg_get_xtz(bsg_ctx *ctx, int chain, float *m)
+{
+  CHAINCHK(ctx, chain);
+  const int V = ctx->L.totvox, P = ctx->P;
+  std::vector<float> mm((size_t)V * P);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(mm.data(), ctx->pb.M + (size_t)chain * V * P, sizeof(float) * mm.size(),
+                   hipMemcpyDeviceToHost));
+  for (int s = 0; s < V; s++)
+    for (int c = 0; c < P; c++) m[(size_t)c * V + ctx->L.rank_of_sorted[s]] = mm[(size_t)s * P + c];
+  return BSG_OK;
+}
+
+int bsg_set_xtz(bsg_ctx *ctx, int chain, const float *m)
+{
+  CHAINCHK(ctx, chain);
+  const int V = ctx->L.totvox, P = ctx->P;
+  std::vector<float> mm((size_t)V * P);
+  for (int s = 0; s < V; s++)
+    for (int c = 0; c < P; c++) mm[(size_t)s * P + c] = m[(size_t)c * V + ctx->L.rank_of_sorted[s]];
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(ctx->pb.M + (size_t)chain * V * P, mm.data(), sizeof(float) * mm.size(),
+                   hipMemcpyHostToDevice));
+  return BSG_OK;
+}
+
+int bsg_get_prec(bsg_ctx *ctx, int chain, float *prec)
+{
+  CHAINCHK(ctx, chain);
+  const int P = ctx->P;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(prec, ctx->pb.lam + (size_t)chain * P * P, sizeof(float) * P * P, hipMemcpyDeviceToHost));
+  return BSG_OK;
+}
+
+int bsg_set_prec(bsg_ctx *ctx, int chain, const float *prec)
+{
+  CHAINCHK(ctx, chain);
+  const int P = ctx->P;
+  std::vector<double> d((size_t)P * P);
+  for (int i = 0; i < P * P; i++) d[i] = prec[i];
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(ctx->pb.lam + (size_t)chain * P * P, prec, sizeof(float) * P * P, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->pb.lam64 + (size_t)chain * P * P, d.data(), sizeof(double) * P * P,
+                   hipMemcpyHostToDevice));
+  return BSG_OK;
+}
+
+int bsg_get_pair_sums(bsg_ctx *ctx, int chain, double *t)
+{
+  CHAINCHK(ctx, chain);
+  const int P = ctx->P;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(t, ctx->pb.tsum + (size_t)chain * P * P, sizeof(double) * P * P, hipMemcpyDeviceToHost));
+  return BSG_OK;
+}
+
+int bsg_get_sweep_index(bsg_ctx *ctx, int chain, uint32_t *sweep)
+{
+  CHAINCHK(ctx, chain);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(sweep, ctx->pb.sweep + chain, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return BSG_OK;
+}
+
+int bsg_set_sweep_index(bsg_ctx *ctx, uint32_t sweep)
+{
+  CTXCHK(ctx);
+  std::vector<uint32_t> v(ctx->C, sweep);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(ctx->pb.sweep, v.data(), sizeof(uint32_t) * v.size(), hipMemcpyHostToDevice));
+  return BSG_OK;
+}
+
+int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count)
+{
+  CTXCHK(ctx);
+  uint32_t flags[4];
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
+  *count = flags[0];
+  return BSG_OK;
+}
+
+// ---- host KISS (randgen.cpp:9-23, 36-45; main.cu:54) -------------------
+double bsg_kiss(unsigned long seed[3])
+{
+  static const double M = exp(-64 * log(2.0));
+  seed[1] = seed[1] ^ (seed[1] << 17);
+  seed[2] = (seed[2] ^ (seed[2] << 18)) & 0x7FFFFFFF;
+  seed[0] = 69069 * seed[0] + 23606797;
+  seed[1] ^= (seed[1] >> 15);
+  seed[2] ^= (seed[2] >> 13);
+  return (seed[0] + seed[1] + seed[2]) * M;
+}
+
+uint64_t bsg_devseed(unsigned long seed[3])
+{
+  const unsigned long n = ~0UL;
+  unsigned long i = (unsigned long)floorl(n * bsg_kiss(seed));
+  return (uint64_t)(i == n ? i - 1 : i);
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// bsg_internal.h -- shared definitions of libbsglmm (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bsg {
+
+// Philox counter tags: counter = (index, sweep, tag | sub << 4, chain)
+constexpr uint32_t TAG_LATENT = 1;  // index = voxel * NQ + subject quad
+constexpr uint32_t TAG_COEF = 2;    // index = voxel, sub = group of 4 normals
+constexpr uint32_t TAG_GAMMA = 3;   // index = Bartlett row, sub = trial
+constexpr uint32_t TAG_WNORM = 4;   // index = row * P + column
+
+constexpr int PAIR_BLOCKS = 512;    // fixed partial count per chain: deterministic sums
+constexpr int MAX_P = 8;
+
+// Host copy of the reference layout (main.cu:189-394) plus the colour-sorted
+// order used on the device: sorted index s runs over INDX[0].hostVox, then
+// INDX[1].hostVox.
+struct Layout {
+  int nrow = 0, ncol = 0, ndepth = 0, nsize = 0;
+  int totvox = 0, totvoxp = 0;
+  int n[2] = {0, 0};
+  std::vector<int> host_idx, host_idx_sc;
+  std::vector<int> vox[2];
+  std::vector<unsigned char> nbrs[2];
+  std::vector<int> rank_of_sorted;  // sorted s -> hostIdx (mask rank)
+  std::vector<int> sc_of_sorted;    // sorted s -> hostIdxSC (dilated-mask rank)
+  std::vector<int> nbr6;            // [V][6] sorted neighbours, -1 outside mask
+};
+
+int build_layout(const unsigned char *mask, int nrow, int ncol, int ndepth, Layout &out);
+
+// Everything a kernel needs, passed by value.
+struct DevProblem {
+  int V, N, NS, NQ, NW, P, nchains, n0;
+  uint32_t key0, key1;
+  int df;
+  float trunc_eps, pivot_min;
+  const int *nbr6;         // [V][6]
+  const uint32_t *ybits;   // [V][NW], bit i of row s = (y_is == 1)
+  const float *X;          // [P][NS], zero past N
+  const float *xtx;        // [P][P]
+  const float *xsum;       // [P]  column sums of X
+  const float *wm;         // [V]
+  float *Z;                // [chain][V][NS]
+  float *beta;             // [chain][V][P]
+  float *M;                // [chain][V][P]  X'(Z - b WM)
+  float *lam;              // [chain][P][P]
+  double *lam64;           // [chain][P][P]
+  double *tsum;            // [chain][P][P]
+  double *partials;        // [chain][PAIR_BLOCKS][P(P+1)/2]
+  float *bwm;              // [chain]  WM coefficient b
+  uint32_t *sweep;         // [chain]  sweep index, advanced by the Wishart kernel
+  uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
+};
+
+// kernel launchers (bsg_kernels.hip)
+hipError_t launch_latent(const DevProblem &pb, hipStream_t st);
+hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st);
+hipError_t launch_coef(const DevProblem &pb, int colour, hipStream_t st);
+hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
+hipError_t launch_wishart(const DevProblem &pb, hipStream_t st);
+
+void set_error(const std::string &msg);
+
+}  // namespace bsg

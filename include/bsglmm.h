@@ -1,0 +1,121 @@
+/*
+ * bsglmm.h -- C ABI of the MI355X BSGLMM Gibbs sweep (libbsglmm.so).
+ *
+ * The reference has no plugin API: its sampler driver (bsglmm/mcmc.cpp)
+ * calls C-linkage update functions that share state through process globals
+ * (prototypes at mcmc.cpp:96-127). Each entry point below names the reference
+ * function it replaces. Rules: every call returns an int status (BSG_OK or a
+ * negative code; bsg_last_error() gives the message), nothing exits or
+ * asserts; the caller owns host arrays, the context owns device memory; a
+ * context issues all work on its own HIP stream; contexts are
+ * thread-compatible, not thread-safe.
+ *
+ * Host arrays use the reference's layouts:
+ *   mask        padded (NROW+2)(NCOL+2)(NDEPTH+2) bytes, raster index
+ *               i + (NROW+2) j + (NROW+2)(NCOL+2) k        (main.cu:56, read_data.cpp:227-240)
+ *   covar       float[P][N], covariate-major               (read_data.cpp:299-302)
+ *   data        uint8[N][TOTVOX], y of subject i at mask rank  (read_data.cpp:334-395)
+ *   WM          float[TOTVOX], NULL means 1.0 everywhere   (read_data.cpp:531-534)
+ *   spatCoef    float[P][TOTVOXp], dilated-mask rank       (mcmc.cpp:227, covarGPU.cu:389-390)
+ *   Z           float[N][TOTVOX]                            (mcmc.cpp:173-178)
+ *   SpatCoefPrec float[P][P]                                (mcmc.cpp:150-154)
+ */
+#ifndef BSGLMM_H
+#define BSGLMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSG_OK 0
+#define BSG_ERR_ARG (-1)   /* bad argument or unsupported size */
+#define BSG_ERR_HIP (-2)   /* HIP runtime error (no device, launch failure) */
+#define BSG_ERR_NOMEM (-3) /* host or device allocation failed */
+#define BSG_ERR_NUM (-4)   /* numerical failure reported by a kernel */
+
+#define BSG_MAX_COVAR 8    /* P <= 8 (the reference allows 20: covarGPU.cu:65) */
+
+typedef struct bsg_layout bsg_layout;
+typedef struct bsg_ctx bsg_ctx;
+
+/* Message of the last failing call on this thread. */
+const char *bsg_last_error(void);
+
+/* ---- layout: main.cu:189-394 (dilation, hostIdx, hostIdxSC, red/black
+ * lists INDX[c].hostVox, neighbour counts INDX[c].hostNBRS via cnt_nbrs
+ * main.cu:471-502; INDEX struct binCAR.h:12-19). Bit-exact with the
+ * reference. ------------------------------------------------------------ */
+int bsg_layout_create(const unsigned char *mask, int nrow, int ncol, int ndepth,
+                      bsg_layout **out);
+void bsg_layout_destroy(bsg_layout *layout);
+/* counts = {TOTVOX, TOTVOXp, INDX[0].hostN, INDX[1].hostN} */
+int bsg_layout_counts(const bsg_layout *layout, int counts[4]);
+/* Copies the reference arrays out: hostIdx and hostIdxSC (NSIZE ints each),
+ * INDX[c].hostVox (ints), INDX[c].hostNBRS (bytes). Any pointer may be NULL.
+ * nbr6 (TOTVOX*6 ints): this library's colour-sorted neighbour table
+ * (-x,+x,-y,+y,-z,+z; -1 outside the mask). */
+int bsg_layout_export(const bsg_layout *layout, int *host_idx, int *host_idx_sc, int *vox0,
+                      int *vox1, unsigned char *nbrs0, unsigned char *nbrs1, int *nbr6);
+
+/* ---- sampler context ------------------------------------------------- */
+typedef struct {
+  uint64_t seed;        /* Philox key; the CLI derives it from seed.dat as main.cu:413 does */
+  int nchains;          /* independent chains over the same data (>= 1) */
+  int xtz_mode;         /* 0: FP32 FMA X'Z; 1: BF16 hi+lo split on MFMA, FP32 accumulate */
+  int df_offset;        /* Wishart df = TOTVOX + df_offset: 19 as covarGPU.cu:1104, P as covar.cpp:291 */
+  float trunc_eps;      /* latent truncation: 1e-6 as covarGPU.cu:1414-1418, 0 as covar.cpp:317,320 */
+  float pivot_min;      /* FP32 Cholesky pivot below this -> FP64 refactor of that voxel */
+} bsg_opts;
+
+void bsg_opts_default(bsg_opts *opts);
+
+/* Allocates device state and uploads the data (read_data.cpp:321-332,
+ * mcmc.cpp:136-253). Initial state as mcmc.cpp: beta = 0, Lambda = I, b = 0,
+ * Z = 0, sweep index 0. The layout may be destroyed afterwards. */
+int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *covar,
+               const unsigned char *data, const float *wm, const bsg_opts *opts, bsg_ctx **out);
+int bsg_destroy(bsg_ctx *ctx);
+
+/* ---- the sweep (mcmc.cpp:311-394, GPU branch, probit, no WM image) ---- */
+/* nsweeps full sweeps: latent, X'Z, coefficients colour 0 then 1, CAR precision */
+int bsg_sweep(bsg_ctx *ctx, int nsweeps);
+/* updateZGPU (covarGPU.cu:1625-1669) */
+int bsg_update_latent(bsg_ctx *ctx);
+/* X'(Z - b WM), the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352) */
+int bsg_compute_xtz(bsg_ctx *ctx);
+/* Spat_Coef_Probit for one colour (covarGPU.cu:547-560); reads X'Z from bsg_compute_xtz */
+int bsg_update_coef(bsg_ctx *ctx, int colour);
+/* updateSpatCoefPrecGPU (covarGPU.cu:1063-1112): FP64 pair reduction and a
+ * device Wishart draw; advances the sweep index */
+int bsg_update_car_prec(bsg_ctx *ctx);
+/* waits for the context's stream; reports kernel-side numerical failures */
+int bsg_synchronize(bsg_ctx *ctx);
+/* nsweeps sweeps launched kernel by kernel with HIP events around each one;
+ * kernel_ms[6] receives the summed durations of latent, xtz, coef0, coef1,
+ * pairs, wishart. Returns after the stream drained. */
+int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6]);
+
+/* ---- state exchange in the reference layouts (chain index first) ------ */
+int bsg_get_spat_coef(bsg_ctx *ctx, int chain, float *spat_coef);
+int bsg_set_spat_coef(bsg_ctx *ctx, int chain, const float *spat_coef);
+int bsg_get_z(bsg_ctx *ctx, int chain, float *z);
+int bsg_get_xtz(bsg_ctx *ctx, int chain, float *m);           /* float[P][TOTVOX] */
+int bsg_set_xtz(bsg_ctx *ctx, int chain, const float *m);
+int bsg_get_prec(bsg_ctx *ctx, int chain, float *prec);       /* float[P][P] */
+int bsg_set_prec(bsg_ctx *ctx, int chain, const float *prec);
+int bsg_get_pair_sums(bsg_ctx *ctx, int chain, double *t);    /* double[P][P], last update */
+int bsg_get_sweep_index(bsg_ctx *ctx, int chain, uint32_t *sweep);
+int bsg_set_sweep_index(bsg_ctx *ctx, uint32_t sweep);        /* all chains */
+int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count);     /* FP64 Cholesky refactors */
+
+/* ---- host KISS, for seed.dat (randgen.cpp:9-23, 36-45; main.cu:54, 413) */
+double bsg_kiss(unsigned long seed[3]);
+/* devseed = runiform_long_n(ULONG_MAX, seed), as main.cu:413 */
+uint64_t bsg_devseed(unsigned long seed[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

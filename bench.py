@@ -7,9 +7,12 @@ model (bsglmm_amd.synth). A step is one full sweep: latent draws, X'Z,
 checkerboard coefficient updates (two colours), FP64 pair sums and the device
 Wishart draw. All inputs are resident in HBM before timing starts.
 
-Prints one JSON line. Also reports the dominant kernel's roofline (latent
-draws, algorithmic bytes N*V*(4 + 1/8) + 4*P*V per launch against 8 TB/s)
-and the reference CPU path timed on this host for a bounded sample.
+The headline configuration forms X'Z inside the latent kernel (FP32 FMA);
+the same sweep with a separate X'Z kernel re-reading Z is timed beside it.
+Prints one JSON line with the dominant kernel's roofline (latent draws with
+fused X'Z: algorithmic bytes N*V*(4 + 1/8) + 8*P*V per launch against
+8 TB/s), a measured copy-kernel bandwidth, and the reference CPU path timed on
+this host for a bounded sample.
 """
 import argparse
 import json
@@ -80,6 +83,19 @@ def cpu_baseline(d, nsweeps):
             "latent_draws_per_s": N * V * nsweeps / dt}
 
 
+def time_mode(L, d, mode, steps, warmup):
+    from bsglmm_amd import Sampler
+    S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15, xtz_mode=mode)
+    S.sweep(warmup)
+    S.synchronize()
+    t0 = time.perf_counter()
+    kms = S.sweep_profiled(steps)
+    S.synchronize()
+    wall = time.perf_counter() - t0
+    S.close()
+    return wall, kms
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -87,47 +103,55 @@ def main():
               "(the sweep shards over voxel slabs by design but this build measures one GPU)",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
-
-    from bsglmm_amd import Layout, Sampler, synth
+    from bsglmm_amd import Layout, copy_bandwidth, synth
 
     d = synth.generate(a.config)
     nx, ny, nz = d["grid"]
     L = Layout(d["mask"], nx, ny, nz)
     P, N = d["covar"].shape
     V = L.totvox
-    S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15)
-    S.sweep(a.warmup)
-    S.synchronize()
-    t0 = time.perf_counter()
-    kms = S.sweep_profiled(a.steps)
-    S.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = sum(kms.values())
-    ms_per_step = 1e3 * wall / a.steps
-    value = a.steps / wall
+    copy_gbs, copy_ms = copy_bandwidth(1 << 30, 20)
 
-    lat_bytes = N * V * (4 + 1 / 8) + 4 * P * V
+    # plain single chain, X'Z fused into the latent kernel (FP32 FMA)
+    wall, kms = time_mode(L, d, 2, a.steps, a.warmup)
+    # plain single chain, separate FP32 X'Z kernel reading Z (the layout of the reference sweep)
+    wall_sep, kms_sep = time_mode(L, d, 0, a.steps, a.warmup)
+
+    value = a.steps / wall
+    gpu_ms = sum(kms.values()) / a.steps
+    lat_bytes = N * V * (4 + 1 / 8) + 8 * P * V
     lat_s = kms["latent"] / a.steps / 1e3
     achieved = lat_bytes / lat_s / 1e9
     traffic = None
     if os.path.exists(a.traffic):
         with open(a.traffic) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
-    sweep_bytes = N * V * (4 + 4 + 1 / 8) + V * (24 * P + 5)
+    sweep_bytes_fused = N * V * (4 + 1 / 8) + V * (24 * P + 5)
+    sweep_bytes_sep = N * V * (4 + 4 + 1 / 8) + V * (24 * P + 5)
+    gpu_ms_sep = sum(kms_sep.values()) / a.steps
     out = {
         "metric": METRIC, "value": value, "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{a.config}: grid {nx}x{ny}x{nz} shell mask, plain single chain, "
-                               f"FP32 state, FP32-FMA X'Z", "V": V, "N": N, "P": P, "chains": 1},
+        "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{a.config}: grid {nx}x{ny}x{nz} ellipsoidal-shell mask, plain single chain, "
+                               f"FP32 state, FP32-FMA X'Z fused into the latent kernel",
+                   "V": V, "N": N, "P": P, "chains": 1},
         "latent_draws_per_s": value * N * V,
         "kernel_ms_per_sweep": {k: v / a.steps for k, v in kms.items()},
-        "sweep_gpu_ms": gpu_ms / a.steps,
-        "sweep_hbm_frac": sweep_bytes / (gpu_ms / a.steps / 1e3) / 1e9 / HBM_PEAK_GBS,
-        "roofline": {"kernel": "k_latent", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": lat_bytes},
+        "sweep_gpu_ms": gpu_ms,
+        "sweep_hbm_frac": sweep_bytes_fused / (gpu_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "copy_kernel_gbs": copy_gbs,
+        "roofline": {"kernel": "k_latent (X'Z fused)", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes_per_launch": lat_bytes,
+                     "frac_of_copy": achieved / copy_gbs},
+        "plain_separate_xtz": {
+            "value": a.steps / wall_sep, "unit": "sweeps/s", "ms_per_step": 1e3 * wall_sep / a.steps,
+            "kernel_ms_per_sweep": {k: v / a.steps for k, v in kms_sep.items()},
+            "sweep_gpu_ms": gpu_ms_sep,
+            "sweep_hbm_frac": sweep_bytes_sep / (gpu_ms_sep / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "latent_gbs": (N * V * (4 + 1 / 8) + 4 * P * V) / (kms_sep["latent"] / a.steps / 1e3) / 1e9,
+            "xtz_gbs": (4 * N * V + 4 * P * V) / (kms_sep["xtz"] / a.steps / 1e3) / 1e9},
         "cpu_baseline": None,
     }
     if not a.no_cpu:

@@ -1,6 +1,7 @@
 // bsg_ctx.cpp -- C ABI of libbsglmm: context, data upload, sweep, state exchange.
 #include <math.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -164,8 +165,8 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     set_error("bsg_create: nchains must be in 1..4096");
     return BSG_ERR_ARG;
   }
-  if (o.xtz_mode != 0 && o.xtz_mode != 1) {
-    set_error("bsg_create: xtz_mode must be 0 or 1");
+  if (o.xtz_mode < 0 || o.xtz_mode > 2) {
+    set_error("bsg_create: xtz_mode must be 0, 1 or 2");
     return BSG_ERR_ARG;
   }
   int ndev = 0;
@@ -258,7 +259,9 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(pb.lam, (size_t)C * P * P);
   ALLOC(pb.lam64, (size_t)C * P * P);
   ALLOC(pb.tsum, (size_t)C * P * P);
-  ALLOC(pb.partials, (size_t)C * bsg::PAIR_BLOCKS * NE);
+  pb.max_parts = std::max(bsg::PAIR_BLOCKS, (std::max(L.n[0], L.n[1]) + 255) / 256);
+  ALLOC(pb.partials, (size_t)C * pb.max_parts * NE);
+  ALLOC(pb.counter, (size_t)C);
   ALLOC(pb.bwm, (size_t)C);
   ALLOC(pb.sweep, (size_t)C);
   ALLOC(pb.flags, (size_t)4);
@@ -284,6 +287,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemsetAsync(pb.bwm, 0, sizeof(float) * C, st));
   HIPCHK(hipMemsetAsync(pb.sweep, 0, sizeof(uint32_t) * C, st));
   HIPCHK(hipMemsetAsync(pb.flags, 0, sizeof(uint32_t) * 4, st));
+  HIPCHK(hipMemsetAsync(pb.counter, 0, sizeof(uint32_t) * C, st));
   std::vector<float> lam((size_t)C * P * P, 0.0f);
   std::vector<double> lam64((size_t)C * P * P, 0.0);
   for (int c = 0; c < C; c++)
@@ -314,7 +318,7 @@ int bsg_destroy(bsg_ctx *ctx)
 int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  HIPCHK(bsg::launch_latent(ctx->pb, ctx->stream));
+  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->stream));
   return BSG_OK;
 }
 
@@ -332,7 +336,7 @@ int bsg_update_coef(bsg_ctx *ctx, int colour)
     set_error("bsg_update_coef: colour must be 0 or 1");
     return BSG_ERR_ARG;
   }
-  HIPCHK(bsg::launch_coef(ctx->pb, colour, ctx->stream));
+  HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->stream));
   return BSG_OK;
 }
 
@@ -347,12 +351,16 @@ int bsg_update_car_prec(bsg_ctx *ctx)
 static int sweep_once(bsg_ctx *ctx)
 {
   hipStream_t st = ctx->stream;
-  HIPCHK(bsg::launch_latent(ctx->pb, st));
-  HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
-  HIPCHK(bsg::launch_coef(ctx->pb, 0, st));
-  HIPCHK(bsg::launch_coef(ctx->pb, 1, st));
-  HIPCHK(bsg::launch_pairs(ctx->pb, st));
-  HIPCHK(bsg::launch_wishart(ctx->pb, st));
+  const bool fused = ctx->opts.xtz_mode == 2;
+  HIPCHK(bsg::launch_latent(ctx->pb, fused, st));
+  if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
+  HIPCHK(bsg::launch_coef(ctx->pb, 0, false, st));
+  if (ctx->L.n[1] > 0) {
+    HIPCHK(bsg::launch_coef(ctx->pb, 1, true, st));  // colour 1 + pair sums + Wishart
+  } else {
+    HIPCHK(bsg::launch_pairs(ctx->pb, st));
+    HIPCHK(bsg::launch_wishart(ctx->pb, st));
+  }
   return BSG_OK;
 }
 
@@ -394,18 +402,20 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
   for (int k = 0; k < 6; k++) kernel_ms[k] = 0.0;
   hipStream_t st = ctx->stream;
   for (int i = 0; i < nsweeps; i++) {
+    const bool fused = ctx->opts.xtz_mode == 2;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(bsg::launch_latent(ctx->pb, st));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, st));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
+    if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
-    HIPCHK(bsg::launch_coef(ctx->pb, 0, st));
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, st));
     HIPCHK(hipEventRecord(ctx->ev[3], st));
-    HIPCHK(bsg::launch_coef(ctx->pb, 1, st));
+    const bool one_pass = ctx->L.n[1] > 0;
+    if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, st));
     HIPCHK(hipEventRecord(ctx->ev[4], st));
-    HIPCHK(bsg::launch_pairs(ctx->pb, st));
+    if (!one_pass) HIPCHK(bsg::launch_pairs(ctx->pb, st));
     HIPCHK(hipEventRecord(ctx->ev[5], st));
-    HIPCHK(bsg::launch_wishart(ctx->pb, st));
+    if (!one_pass) HIPCHK(bsg::launch_wishart(ctx->pb, st));
     HIPCHK(hipEventRecord(ctx->ev[6], st));
     HIPCHK(hipEventSynchronize(ctx->ev[6]));
     for (int k = 0; k < 6; k++) {
@@ -549,6 +559,49 @@ int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count)
   HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
   *count = flags[0];
   return BSG_OK;
+}
+
+int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_per_copy)
+{
+  if (!gbps || bytes < 16 || reps < 1) {
+    set_error("bsg_measure_copy_bandwidth: bad argument");
+    return BSG_ERR_ARG;
+  }
+  bytes &= ~(size_t)15;
+  void *src = nullptr, *dst = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = BSG_OK;
+  float ms = 0.0f;
+  if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) {
+    set_error("bsg_measure_copy_bandwidth: hipMalloc failed");
+    rc = BSG_ERR_NOMEM;
+  } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+             hipMemsetAsync(src, 0, bytes, st) != hipSuccess ||
+             bsg::launch_copy(src, dst, bytes, st) != hipSuccess ||
+             hipEventRecord(e0, st) != hipSuccess) {
+    set_error("bsg_measure_copy_bandwidth: HIP failure");
+    rc = BSG_ERR_HIP;
+  } else {
+    for (int i = 0; i < reps && rc == BSG_OK; i++)
+      if (bsg::launch_copy(src, dst, bytes, st) != hipSuccess) rc = BSG_ERR_HIP;
+    if (rc == BSG_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = BSG_ERR_HIP;
+    if (rc != BSG_OK) set_error("bsg_measure_copy_bandwidth: HIP failure");
+  }
+  if (rc == BSG_OK) {
+    *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+  }
+  if (st) hipStreamSynchronize(st);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (src) hipFree(src);
+  if (dst) hipFree(dst);
+  return rc;
 }
 
 // ---- host KISS (randgen.cpp:9-23, 36-45; main.cu:54) -------------------

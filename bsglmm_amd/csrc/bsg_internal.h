@@ -54,18 +54,21 @@ struct DevProblem {
   float *lam;              // [chain][P][P]
   double *lam64;           // [chain][P][P]
   double *tsum;            // [chain][P][P]
-  double *partials;        // [chain][PAIR_BLOCKS][P(P+1)/2]
+  double *partials;        // [chain][max_parts][P(P+1)/2]
+  int max_parts;           // partial slots per chain
+  uint32_t *counter;       // [chain] completion counter of the fused colour-1 kernel
   float *bwm;              // [chain]  WM coefficient b
   uint32_t *sweep;         // [chain]  sweep index, advanced by the Wishart kernel
   uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
 };
 
 // kernel launchers (bsg_kernels.hip)
-hipError_t launch_latent(const DevProblem &pb, hipStream_t st);
+hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st);
-hipError_t launch_coef(const DevProblem &pb, int colour, hipStream_t st);
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, hipStream_t st);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, hipStream_t st);
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
 void set_error(const std::string &msg);
 

@@ -21,21 +21,20 @@
 namespace bsg {
 
 // ---------------------------------------------------------------------------
-// Philox4x32-10 and uniforms
+// Philox4x32-10 and uniforms. The 32x32->64 products are written as 64-bit
+// multiplies so each becomes one v_mad_u64_u32 (high and low halves at once).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                          uint32_t k0, uint32_t k1, uint32_t r[4])
 {
 #pragma unroll
   for (int i = 0; i < 10; i++) {
-    uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-    uint32_t lo0 = 0xD2511F53u * c0;
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
-    uint32_t lo1 = 0xCD9E8D57u * c2;
-    c0 = hi1 ^ c1 ^ k0;
-    c1 = lo1;
-    c2 = hi0 ^ c3 ^ k1;
-    c3 = lo0;
+    const uint64_t p0 = (uint64_t)c0 * 0xD2511F53u;
+    const uint64_t p1 = (uint64_t)c2 * 0xCD9E8D57u;
+    c0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    c1 = (uint32_t)p1;
+    c2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c3 = (uint32_t)p0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
@@ -56,10 +55,54 @@ __device__ __forceinline__ double u01d(uint32_t x)
 }
 
 // ---------------------------------------------------------------------------
+// Normal tail function and its inverse in FP32, branch free. Coefficients are
+// this project's own fits (bsglmm_amd/csrc/fit_normal_tail.py):
+//   Q(t)    = r exp(-t^2/2 + g(r)), r = 1/(1 + t/(2 sqrt 2)), g of degree 9
+//   Qinv(p) = n(s)/d(s), s = sqrt(-2 ln p), n, d of degree 4, p in (1e-38, 1/2]
+// FP32 errors: Q 3e-6 relative (t <= 8), Qinv 5e-7 relative (floor 1).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float q_tail(float t)
+{
+  const float r = __builtin_amdgcn_rcpf(fmaf(0.35355339059327373f, t, 1.0f));
+  float g = 0.17109377682209015f;
+  g = fmaf(g, r, -0.8232021331787109f);
+  g = fmaf(g, r, 1.4906160831451416f);
+  g = fmaf(g, r, -1.1375014781951904f);
+  g = fmaf(g, r, 0.2803618311882019f);
+  g = fmaf(g, r, -0.1868750900030136f);
+  g = fmaf(g, r, 0.0969197228550911f);
+  g = fmaf(g, r, 0.3740749955177307f);
+  g = fmaf(g, r, 1.000024676322937f);
+  g = fmaf(g, r, -1.9586594104766846f);
+  const float e = fmaf(-0.5f * t, t, g);
+  return r * __builtin_amdgcn_exp2f(e * 1.4426950408889634f);
+}
+
+__device__ __forceinline__ float q_inv(float p)
+{
+  p = fminf(p, 0.5f);
+  const float s = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(p));
+  float n = 0.5416675806045532f;
+  n = fmaf(n, s, 4.245328426361084f);
+  n = fmaf(n, s, 3.2832629680633545f);
+  n = fmaf(n, s, -7.911550045013428f);
+  n = fmaf(n, s, -3.2067947387695312f);
+  float d = 7.4306449278083164e-06f;
+  d = fmaf(d, s, 0.5411052107810974f);
+  d = fmaf(d, s, 4.2728986740112305f);
+  d = fmaf(d, s, 5.463263034820557f);
+  d = fmaf(d, s, 1.0f);
+  return n * __builtin_amdgcn_rcpf(d);
+}
+
+// ---------------------------------------------------------------------------
 // Truncated-normal latent draw by inversion (one uniform, no rejection loop).
 // W ~ N(0,1) truncated below at a; Z = mu + s W with s = +1 (y=1), -1 (y=0)
-// and a = eps - s mu. The upper limit 500 (covarGPU.cu:1415,1418) has
-// Q(500 - s mu) = 0 in FP32 for |mu| < 480 and is not evaluated.
+// and a = eps - s mu (covarGPU.cu:1414-1422). W = Qinv(u Q(a)); for a < 0,
+// Q(a) = 1 - Q(|a|) and, when u Q(a) > 1/2, W = -Qinv(1 - u Q(a)) with
+// 1 - u Q(a) = (1-u) + u Q(|a|) formed without cancellation. The upper limit
+// 500 (covarGPU.cu:1415,1418) has Q(500 - s mu) = 0 in FP32 for |mu| < 480 and
+// is not evaluated. a >= 8 takes a log-space Newton path (rare).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float logq_asym(float x)  // log Q(x) for x >= 8
 {
@@ -83,129 +126,460 @@ __device__ __noinline__ float tail_w(float a, float u)
 
 __device__ __forceinline__ float latent_draw(float mu, uint32_t y, float u, float eps)
 {
-  float s = y ? 1.0f : -1.0f;
-  float a = eps - s * mu;
-  float w;
-  if (a >= 8.0f) {
-    w = tail_w(a, u);
-  } else {
-    float qt = 0.5f * erfcf(fabsf(a) * 0.70710678118654752f);  // Q(|a|) <= 1/2
-    float arg, sg = 1.0f;
-    if (a >= 0.0f) {
-      arg = u * qt;
-    } else {
-      float p = u * (1.0f - qt);
-      if (p <= 0.5f) {
-        arg = p;
-      } else {
-        arg = (1.0f - u) + u * qt;  // 1 - p without cancellation
-        sg = -1.0f;
-      }
-    }
-    w = sg * 1.41421356237309505f * erfcinvf(2.0f * arg);
+  const float s = y ? 1.0f : -1.0f;
+  const float a = fmaf(-s, mu, eps);
+  if (a >= 8.0f) return fmaf(s, fmaxf(tail_w(a, u), a), mu);
+  const float qt = q_tail(fabsf(a));  // Q(|a|) <= 1/2
+  const float p = (a >= 0.0f) ? u * qt : fmaf(-u, qt, u);
+  const bool flip = p > 0.5f;
+  const float arg = flip ? fmaf(u, qt, 1.0f - u) : p;
+  float w = q_inv(arg);
+  w = flip ? -w : w;
+  return fmaf(s, fmaxf(w, a), mu);
+}
+
+// ---------------------------------------------------------------------------
+// Cross-lane sum of P per-lane values over a full wave64 (reduce-scatter):
+// v_permlane32_swap pairs values (c, c + PP/2) across the wave halves,
+// v_permlane16_swap pairs them across 16-lane rows, DPP finishes inside a row
+// (or half row for PP = 8). The total of value c ends in lane red_lane<P>(c).
+// PP = P padded to 4 or 8 with zeros.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void swap32(float &x, float &y)
+{
+  auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  x = __uint_as_float(r[0]);
+  y = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap16(float &x, float &y)
+{
+  auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  x = __uint_as_float(r[0]);
+  y = __uint_as_float(r[1]);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v)
+{
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+template <int P>
+constexpr int red_pad() { return P <= 4 ? 4 : 8; }
+
+template <int P>
+__device__ __forceinline__ int red_lane(int c)
+{
+  if (red_pad<P>() == 4) return 16 * c;
+  // rows hold {a0,a2},{a1,a3},{a4,a6},{a5,a7} as (low half, high half)
+  const int row = (c & 1) + 2 * (c >> 2);
+  const int half = (c >> 1) & 1;
+  return 16 * row + 8 * half;
+}
+
+template <int P>
+__device__ __forceinline__ float wave_reduce_scatter(const float (&acc)[P], int lane)
+{
+  constexpr int PP = red_pad<P>();
+  float a[PP];
+#pragma unroll
+  for (int c = 0; c < PP; c++) a[c] = c < P ? acc[c] : 0.0f;
+  float b[PP / 2];
+#pragma unroll
+  for (int k = 0; k < PP / 2; k++) {
+    float x = a[k], y = a[k + PP / 2];
+    swap32(x, y);
+    b[k] = x + y;  // lanes 0-31: a_k, lanes 32-63: a_{k+PP/2}
   }
-  w = fmaxf(w, a);
-  return mu + s * w;
+  if (PP == 4) {
+    float x = b[0], y = b[1];
+    swap16(x, y);
+    float c = x + y;  // row r holds a_r
+    c = dpp_add<0x128>(c);  // row_ror:8
+    c = dpp_add<0x124>(c);  // row_ror:4
+    c = dpp_add<0x122>(c);  // row_ror:2
+    c = dpp_add<0x121>(c);  // row_ror:1
+    return c;
+  } else {
+    float x0 = b[0], y0 = b[1], x1 = b[2], y1 = b[3];
+    swap16(x0, y0);
+    swap16(x1, y1);
+    float c0 = dpp_add<0x128>(x0 + y0);  // rows: a0, a1, a4, a5
+    float c1 = dpp_add<0x128>(x1 + y1);  // rows: a2, a3, a6, a7
+    float e = (lane & 8) ? c1 : c0;
+    e = dpp_add<0x141>(e);  // row_half_mirror
+    e = dpp_add<0xB1>(e);   // quad_perm [1,0,3,2]
+    e = dpp_add<0x4E>(e);   // quad_perm [2,3,0,1]
+    return e;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // k_latent. Lanes of a wave are split into groups of L = 2^lshift lanes; a
 // group owns one voxel and lane ql of the group owns the subject quads
-// q = ql, ql + L, ... (4 subjects per Philox block, one float4 store).
+// q = ql, ql + L, ... (4 subjects per Philox block, one float4 store). Each
+// wave walks `viter` voxel groups. XREG keeps the lane's covariates in
+// registers (one quad per lane, N <= 4 L). FUSED also forms
+// m_j = X'(Z_j - b WM_j) from the fresh draws (no re-read of Z).
 // ---------------------------------------------------------------------------
-template <int P>
-__global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps)
+template <int P, bool XREG, bool FUSED>
+__global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps, int viter)
 {
   const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int s = (wave << (6 - lshift)) + (lane >> lshift);
-  if (s >= pb.V) return;
+  const int vpw = 64 >> lshift;
+  const int sub = lane >> lshift;
   const int ql = lane & ((1 << lshift) - 1);
   const uint32_t sweep = pb.sweep[chain];
   const float b = pb.bwm[chain];
-  const float *bet = pb.beta + ((size_t)chain * pb.V + s) * P;
-  float bs[P];
+  const float eps = pb.trunc_eps;
+  float4 xr[P];
+  if (XREG) {
+    const int i0 = 4 * min(ql, pb.NQ - 1);
 #pragma unroll
-  for (int c = 0; c < P; c++) bs[c] = bet[c];
-  const float base = b * pb.wm[s];
-  float *zrow = pb.Z + ((size_t)chain * pb.V + s) * pb.NS;
-  const uint32_t *yrow = pb.ybits + (size_t)s * pb.NW;
-  for (int r = 0; r < reps; r++) {
-    const int q = ql + (r << lshift);
-    if (q >= pb.NQ) break;
-    const int i0 = 4 * q;
-    float mu[4] = {base, base, base, base};
+    for (int c = 0; c < P; c++) xr[c] = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + i0);
+  }
+  for (int it = 0; it < viter; it++) {
+    const int grp = wave * viter + it;
+    if (grp * vpw >= pb.V) break;  // wave-uniform
+    const int s = grp * vpw + sub;
+    const bool valid = s < pb.V;
+    const int sv = valid ? s : pb.V - 1;
+    const float *bet = pb.beta + ((size_t)chain * pb.V + sv) * P;
+    float bs[P];
 #pragma unroll
-    for (int c = 0; c < P; c++) {
-      float4 x = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + i0);
-      mu[0] = fmaf(bs[c], x.x, mu[0]);
-      mu[1] = fmaf(bs[c], x.y, mu[1]);
-      mu[2] = fmaf(bs[c], x.z, mu[2]);
-      mu[3] = fmaf(bs[c], x.w, mu[3]);
+    for (int c = 0; c < P; c++) bs[c] = bet[c];
+    const float bw = b * pb.wm[sv];
+    float *zrow = pb.Z + ((size_t)chain * pb.V + sv) * pb.NS;
+    const uint32_t *yrow = pb.ybits + (size_t)sv * pb.NW;
+    float acc[P];
+#pragma unroll
+    for (int c = 0; c < P; c++) acc[c] = 0.0f;
+    for (int r = 0; r < reps; r++) {
+      const int q = ql + (r << lshift);
+      if (valid && q < pb.NQ) {
+        const int i0 = 4 * q;
+        float4 x[P];
+#pragma unroll
+        for (int c = 0; c < P; c++)
+          x[c] = XREG ? xr[c] : *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + i0);
+        float mu[4] = {bw, bw, bw, bw};
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+          mu[0] = fmaf(bs[c], x[c].x, mu[0]);
+          mu[1] = fmaf(bs[c], x[c].y, mu[1]);
+          mu[2] = fmaf(bs[c], x[c].z, mu[2]);
+          mu[3] = fmaf(bs[c], x[c].w, mu[3]);
+        }
+        const uint32_t yw = yrow[q >> 3] >> ((q & 7) * 4);
+        uint32_t rr[4];
+        philox10((uint32_t)s * (uint32_t)pb.NQ + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain,
+                 pb.key0, pb.key1, rr);
+        float z[4];
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+          z[l] = (i0 + l < pb.N) ? latent_draw(mu[l], (yw >> l) & 1u, u01f(rr[l]), eps) : 0.0f;
+        *reinterpret_cast<float4 *>(zrow + i0) = make_float4(z[0], z[1], z[2], z[3]);
+        if (FUSED) {
+#pragma unroll
+          for (int c = 0; c < P; c++) {
+            acc[c] = fmaf(x[c].x, z[0], acc[c]);
+            acc[c] = fmaf(x[c].y, z[1], acc[c]);
+            acc[c] = fmaf(x[c].z, z[2], acc[c]);
+            acc[c] = fmaf(x[c].w, z[3], acc[c]);
+          }
+        }
+      }
     }
-    const uint32_t yw = yrow[q >> 3] >> ((q & 7) * 4);
-    uint32_t rr[4];
-    philox10((uint32_t)s * (uint32_t)pb.NQ + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain, pb.key0,
-             pb.key1, rr);
-    float z[4];
+    if (FUSED) {
+      if (lshift == 6) {
+        const float tot = wave_reduce_scatter<P>(acc, lane);
+        float *m = pb.M + ((size_t)chain * pb.V + s) * P;
 #pragma unroll
-    for (int l = 0; l < 4; l++)
-      z[l] = (i0 + l < pb.N) ? latent_draw(mu[l], (yw >> l) & 1u, u01f(rr[l]), pb.trunc_eps) : 0.0f;
-    *reinterpret_cast<float4 *>(zrow + i0) = make_float4(z[0], z[1], z[2], z[3]);
+        for (int c = 0; c < P; c++)
+          if (lane == red_lane<P>(c)) m[c] = fmaf(-bw, pb.xsum[c], tot);
+      } else {
+        for (int off = (1 << lshift) >> 1; off > 0; off >>= 1) {
+#pragma unroll
+          for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
+        }
+        if (valid && ql == 0) {
+          float *m = pb.M + ((size_t)chain * pb.V + s) * P;
+#pragma unroll
+          for (int c = 0; c < P; c++) m[c] = fmaf(-bw, pb.xsum[c], acc[c]);
+        }
+      }
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// k_xtz: m_j = X'Z_j - b WM_j X'1, FP32 FMA, same lane grouping as k_latent.
+// k_xtz_f32: m_j = X'Z_j - b WM_j X'1 in FP32 FMA, reading Z once. Same lane
+// grouping and voxel walk as k_latent (covarGPU.cu:327-352 contraction).
 // ---------------------------------------------------------------------------
-template <int P>
-__global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int reps)
+template <int P, bool XREG>
+__global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int reps, int viter)
 {
   const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int s = (wave << (6 - lshift)) + (lane >> lshift);
-  const bool active = s < pb.V;
+  const int vpw = 64 >> lshift;
+  const int sub = lane >> lshift;
   const int ql = lane & ((1 << lshift) - 1);
-  float acc[P];
+  const float b = pb.bwm[chain];
+  float4 xr[P];
+  if (XREG) {
+    const int i0 = 4 * min(ql, pb.NQ - 1);
 #pragma unroll
-  for (int c = 0; c < P; c++) acc[c] = 0.0f;
-  if (active) {
-    const float *zrow = pb.Z + ((size_t)chain * pb.V + s) * pb.NS;
-    for (int r = 0; r < reps; r++) {
-      const int q = ql + (r << lshift);
-      if (q >= pb.NQ) break;
-      const int i0 = 4 * q;
-      float4 z = *reinterpret_cast<const float4 *>(zrow + i0);
+    for (int c = 0; c < P; c++) xr[c] = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + i0);
+  }
+  for (int it = 0; it < viter; it++) {
+    const int grp = wave * viter + it;
+    if (grp * vpw >= pb.V) break;  // wave-uniform
+    const int s = grp * vpw + sub;
+    const bool valid = s < pb.V;
+    float acc[P];
 #pragma unroll
-      for (int c = 0; c < P; c++) {
-        float4 x = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + i0);
-        acc[c] = fmaf(x.x, z.x, acc[c]);
-        acc[c] = fmaf(x.y, z.y, acc[c]);
-        acc[c] = fmaf(x.z, z.z, acc[c]);
-        acc[c] = fmaf(x.w, z.w, acc[c]);
+    for (int c = 0; c < P; c++) acc[c] = 0.0f;
+    if (valid) {
+      const float *zrow = pb.Z + ((size_t)chain * pb.V + s) * pb.NS;
+      for (int r = 0; r < reps; r++) {
+        const int q = ql + (r << lshift);
+        if (q >= pb.NQ) break;
+        const int i0 = 4 * q;
+        const float4 z = *reinterpret_cast<const float4 *>(zrow + i0);
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+          const float4 x = XREG ? xr[c] : *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + i0);
+          acc[c] = fmaf(x.x, z.x, acc[c]);
+          acc[c] = fmaf(x.y, z.y, acc[c]);
+          acc[c] = fmaf(x.z, z.z, acc[c]);
+          acc[c] = fmaf(x.w, z.w, acc[c]);
+        }
       }
     }
-  }
-  for (int off = (1 << lshift) >> 1; off > 0; off >>= 1) {
+    for (int off = (1 << lshift) >> 1; off > 0; off >>= 1) {
 #pragma unroll
-    for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
-  }
-  if (active && ql == 0) {
-    const float bw = pb.bwm[chain] * pb.wm[s];
-    float *m = pb.M + ((size_t)chain * pb.V + s) * P;
+      for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
+    }
+    if (valid && ql == 0) {
+      const float bw = b * pb.wm[s];
+      float *m = pb.M + ((size_t)chain * pb.V + s) * P;
 #pragma unroll
-    for (int c = 0; c < P; c++) m[c] = acc[c] - bw * pb.xsum[c];
+      for (int c = 0; c < P; c++) m[c] = fmaf(-bw, pb.xsum[c], acc[c]);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Coefficient update. With Q = X'X + n_j Lambda and r = m + Lambda sumV, the
-// reference draws b = Q^-1 r + chol_lower(Q^-1) z (covarGPU.cu:374-386).
-// Factor Q = U U' with U upper (Cholesky of the index-reversed Q); then
-// chol_lower(Q^-1) = U^-T and b = U^-T (U^-1 r + z): one factorisation and two
-// triangular solves, no explicit inverse.
+// Copy probe: a plain float4 stream copy, used to calibrate bandwidth figures.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n)
+{
+  const size_t stride = (size_t)gridDim.x * 256;
+  size_t i = blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, static_cast<const float4 *>(src),
+                     static_cast<float4 *>(dst), n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k_wishart / wishart_block: sums the per-block FP64 partials of T in block
+// order, forms S = (T+I)^-1 and L = chol(S) in FP64 (covarGPU.cu:1086-1101),
+// draws the Bartlett factor A (A_ii^2 ~ chi2(df-i) by Marsaglia-Tsang gamma,
+// A_ij ~ N(0,1)) and writes Lambda = (L A)(L A)' (cholesky.cpp:259-281).
+// Advances the chain's sweep index. Runs on the first wave of a block.
+// ---------------------------------------------------------------------------
+__device__ double gamma_mt(double a, uint32_t row, uint32_t sweep, uint32_t chain, uint32_t k0,
+                           uint32_t k1)
+{
+  const double d = a - 1.0 / 3.0;
+  const double c = 1.0 / sqrt(9.0 * d);
+  for (uint32_t t = 0; t < 64; t++) {
+    uint32_t rr[4];
+    philox10(row, sweep, TAG_GAMMA | (t << 4), chain, k0, k1, rr);
+    double x = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
+    double v = 1.0 + c * x;
+    if (v <= 0.0) continue;
+    v = v * v * v;
+    if (log(u01d(rr[2])) < 0.5 * x * x + d - d * v + d * log(v)) return d * v;
+  }
+  return d;
+}
+
+template <int P>
+__device__ void chol_lower(double (&a)[P][P])
+{
+  for (int j = 0; j < P; j++) {
+    double d = a[j][j];
+    for (int k = 0; k < j; k++) d -= a[j][k] * a[j][k];
+    d = sqrt(d);
+    a[j][j] = d;
+    for (int i = j + 1; i < P; i++) {
+      double s = a[i][j];
+      for (int k = 0; k < j; k++) s -= a[i][k] * a[j][k];
+      a[i][j] = s / d;
+    }
+    for (int i = 0; i < j; i++) a[i][j] = 0.0;
+  }
+}
+
+template <int P>
+__device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part)
+{
+  constexpr int NE = P * (P + 1) / 2;
+  __shared__ double T[NE];
+  __shared__ double A[P][P];
+  const int lane = threadIdx.x;
+  if (lane >= 64) return;
+  const uint32_t sweep = pb.sweep[chain];
+  double acc[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) acc[e] = 0.0;
+  for (int b = lane; b < nparts; b += 64) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
+  }
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    double v = acc[e];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) T[e] = v;
+  }
+  for (int idx = lane; idx < P * P; idx += 64) {
+    const int i = idx / P, j = idx % P;
+    double val = 0.0;
+    if (i == j) {
+      val = sqrt(2.0 * gamma_mt(0.5 * (double)(pb.df - i), (uint32_t)i, sweep, (uint32_t)chain,
+                                pb.key0, pb.key1));
+    } else if (j < i) {
+      uint32_t rr[4];
+      philox10((uint32_t)idx, sweep, TAG_WNORM, (uint32_t)chain, pb.key0, pb.key1, rr);
+      val = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
+    }
+    A[i][j] = val;
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  if (lane != 0) return;
+  double S[P][P];
+  for (int a = 0, e = 0; a < P; a++)
+    for (int c = 0; c <= a; c++, e++) {
+      S[a][c] = T[e];
+      S[c][a] = T[e];
+      pb.tsum[(size_t)chain * P * P + a * P + c] = T[e];
+      pb.tsum[(size_t)chain * P * P + c * P + a] = T[e];
+    }
+  for (int a = 0; a < P; a++) S[a][a] += 1.0;
+  chol_lower<P>(S);
+  double inv[P][P];
+  for (int col = 0; col < P; col++) {
+    double y[P];
+    for (int i = 0; i < P; i++) {
+      double v = (i == col) ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) v -= S[i][k] * y[k];
+      y[i] = v / S[i][i];
+    }
+    for (int i = P - 1; i >= 0; i--) {
+      double v = y[i];
+      for (int k = i + 1; k < P; k++) v -= S[k][i] * inv[k][col];
+      inv[i][col] = v / S[i][i];
+    }
+  }
+  chol_lower<P>(inv);
+  double B[P][P];
+  for (int i = 0; i < P; i++)
+    for (int j = 0; j < P; j++) {
+      double v = 0.0;
+      if (j <= i)
+        for (int k = j; k <= i; k++) v += inv[i][k] * A[k][j];
+      B[i][j] = v;
+    }
+  float *lam = pb.lam + (size_t)chain * P * P;
+  double *lam64 = pb.lam64 + (size_t)chain * P * P;
+  for (int i = 0; i < P; i++)
+    for (int j = 0; j < P; j++) {
+      const int K = i < j ? i : j;
+      double v = 0.0;
+      for (int k = 0; k <= K; k++) v += B[i][k] * B[j][k];
+      lam64[i * P + j] = v;
+      lam[i * P + j] = (float)v;
+    }
+  pb.sweep[chain] = sweep + 1;
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_wishart(DevProblem pb)
+{
+  constexpr int NE = P * (P + 1) / 2;
+  const int chain = blockIdx.x;
+  wishart_block<P>(pb, chain, PAIR_BLOCKS, pb.partials + (size_t)chain * pb.max_parts * NE);
+}
+
+// Adds the pair terms of voxel j (new coefficients bj) with neighbour k
+// (coefficients bk). The reference tests b^a != 0 at the +x/+y/+z member of
+// the pair (covarGPU.cu:973-988): that is k when k lies in the + direction,
+// j otherwise.
+template <int P>
+__device__ __forceinline__ void add_pair(const float (&bj)[P], const float (&bk)[P], bool k_is_plus,
+                                         double (&acc)[P * (P + 1) / 2])
+{
+  float df[P];
+#pragma unroll
+  for (int c = 0; c < P; c++) df[c] = bj[c] - bk[c];
+  int e = 0;
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    const bool ok = (k_is_plus ? bk[a] : bj[a]) != 0.0f;
+    const double da = ok ? (double)df[a] : 0.0;
+#pragma unroll
+    for (int c = 0; c <= a; c++, e++) acc[e] = fma(da, (double)df[c], acc[e]);
+  }
+}
+
+// Block sum of NE doubles in fixed order; thread e < NE of the block gets entry e.
+template <int NE>
+__device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst)
+{
+  __shared__ double red[4][NE];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    double v = acc[e];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wv][e] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NE) {
+    const int e = threadIdx.x;
+    dst[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient update of one colour. With Q = X'X + n_j Lambda and
+// r = m + Lambda sumV, the reference draws b = Q^-1 r + chol_lower(Q^-1) z
+// (covarGPU.cu:354-392). Factor Q = U U' with U upper (the Cholesky factor of
+// the index-reversed Q); then chol_lower(Q^-1) = U^-T and
+// b = U^-T (U^-1 r + z): one factorisation and two triangular solves, no
+// explicit inverse. PREC (colour 1 of a full sweep) also accumulates the pair
+// terms of each voxel with its six colour-0 neighbours -- every neighbour pair
+// has exactly one colour-1 member -- and the last block to finish draws the
+// CAR precision (no spinning: a completion counter elects it).
 // ---------------------------------------------------------------------------
 template <int P, typename T>
 __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], const T (&z)[P],
@@ -250,14 +624,15 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
   return true;
 }
 
-template <int P>
+template <int P, bool PREC>
 __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour)
 {
+  constexpr int NE = P * (P + 1) / 2;
   const int chain = blockIdx.y;
   const int q = blockIdx.x * 256 + threadIdx.x;
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
-  if (q >= nc) return;
-  const int s = q + (colour ? pb.n0 : 0);
+  const bool active = q < nc;
+  const int s = (active ? q : nc - 1) + (colour ? pb.n0 : 0);
   const uint32_t sweep = pb.sweep[chain];
   const float *betc = pb.beta + (size_t)chain * pb.V * P;
   const float *lam = pb.lam + (size_t)chain * P * P;
@@ -274,13 +649,20 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour)
     nb[2 * d] = v.x;
     nb[2 * d + 1] = v.y;
   }
+  float bk[6][P];
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     if (nb[d] >= 0) {
       nn++;
-      const float *bk = betc + (size_t)nb[d] * P;
+      const float *src = betc + (size_t)nb[d] * P;
 #pragma unroll
-      for (int c = 0; c < P; c++) sv[c] += bk[c];
+      for (int c = 0; c < P; c++) {
+        bk[d][c] = src[c];
+        sv[c] += bk[d][c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < P; c++) bk[d][c] = 0.0f;
     }
   }
   const float *m = pb.M + ((size_t)chain * pb.V + s) * P;
@@ -327,23 +709,49 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour)
       rd[i] = acc;
       zd[i] = (double)z[i];
     }
-    atomicAdd(&pb.flags[0], 1u);
-    if (!ul_draw<P, double>(Qd, rd, zd, od, 0.0)) {
-      atomicAdd(&pb.flags[1], 1u);
-      return;  // not positive definite: keep the previous coefficients
+    if (active) atomicAdd(&pb.flags[0], 1u);
+    if (ul_draw<P, double>(Qd, rd, zd, od, 0.0)) {
+#pragma unroll
+      for (int c = 0; c < P; c++) out[c] = (float)od[c];
+    } else {
+      // not positive definite: keep the previous coefficients
+      if (active) atomicAdd(&pb.flags[1], 1u);
+#pragma unroll
+      for (int c = 0; c < P; c++) out[c] = dst[c];
     }
-#pragma unroll
-    for (int c = 0; c < P; c++) out[c] = (float)od[c];
   }
+  if (active) {
 #pragma unroll
-  for (int c = 0; c < P; c++) dst[c] = out[c];
+    for (int c = 0; c < P; c++) dst[c] = out[c];
+  }
+  if (!PREC) return;
+
+  double acc[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) acc[e] = 0.0;
+  if (active) {
+#pragma unroll
+    for (int d = 0; d < 6; d++)
+      if (nb[d] >= 0) add_pair<P>(out, bk[d], (d & 1) != 0, acc);
+  }
+  double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
+  block_sum_store<NE>(acc, parts + (size_t)blockIdx.x * NE);
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(&pb.counter[chain], 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) pb.counter[chain] = 0;
+  wishart_block<P>(pb, chain, (int)gridDim.x, parts);
 }
 
 // ---------------------------------------------------------------------------
-// k_pairs: T_ab = sum over mask voxels j and their +x,+y,+z neighbours k with
-// b^a_k != 0 (covarGPU.cu:975) of (b^a_j - b^a_k)(b^b_j - b^b_k). Float
-// differences as the reference, exact FP64 products, FP64 sums; one partial
-// per block, fixed block count, fixed reduction order (bitwise reproducible).
+// k_pairs (standalone CAR-precision update): T_ab = sum over mask voxels j and
+// their +x,+y,+z neighbours k with b^a_k != 0 (covarGPU.cu:975) of
+// (b^a_j - b^a_k)(b^b_j - b^b_k); float differences as the reference, exact
+// FP64 products, FP64 sums, fixed block count and order.
 // ---------------------------------------------------------------------------
 template <int P>
 __global__ __launch_bounds__(256) void k_pairs(DevProblem pb)
@@ -365,154 +773,13 @@ __global__ __launch_bounds__(256) void k_pairs(DevProblem pb)
     for (int d = 1; d < 6; d += 2) {
       const int k = pb.nbr6[(size_t)s * 6 + d];
       if (k < 0) continue;
-      float bk[P], df[P];
+      float bk[P];
 #pragma unroll
-      for (int c = 0; c < P; c++) {
-        bk[c] = betc[(size_t)k * P + c];
-        df[c] = bj[c] - bk[c];
-      }
-      int e = 0;
-#pragma unroll
-      for (int a = 0; a < P; a++) {
-        const bool ok = bk[a] != 0.0f;
-        const double da = ok ? (double)df[a] : 0.0;
-#pragma unroll
-        for (int c = 0; c <= a; c++, e++) acc[e] = fma(da, (double)df[c], acc[e]);
-      }
+      for (int c = 0; c < P; c++) bk[c] = betc[(size_t)k * P + c];
+      add_pair<P>(bj, bk, true, acc);
     }
   }
-  __shared__ double red[4][NE];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int e = 0; e < NE; e++) {
-    double v = acc[e];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wv][e] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < NE) {
-    const int e = threadIdx.x;
-    double v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
-    pb.partials[((size_t)chain * PAIR_BLOCKS + blockIdx.x) * NE + e] = v;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// k_wishart: one wave per chain. Sums the partials, forms S = (T+I)^-1 and
-// L = chol(S) in FP64 (covarGPU.cu:1086-1101), draws the Bartlett factor A
-// (A_ii^2 ~ chi2(df-i) by Marsaglia-Tsang gamma, A_ij ~ N(0,1)) and writes
-// Lambda = (L A)(L A)' (cholesky.cpp:259-281). Advances the chain's sweep.
-// ---------------------------------------------------------------------------
-__device__ double gamma_mt(double a, uint32_t row, uint32_t sweep, uint32_t chain, uint32_t k0,
-                           uint32_t k1)
-{
-  const double d = a - 1.0 / 3.0;
-  const double c = 1.0 / sqrt(9.0 * d);
-  for (uint32_t t = 0; t < 64; t++) {
-    uint32_t rr[4];
-    philox10(row, sweep, TAG_GAMMA | (t << 4), chain, k0, k1, rr);
-    double x = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
-    double v = 1.0 + c * x;
-    if (v <= 0.0) continue;
-    v = v * v * v;
-    if (log(u01d(rr[2])) < 0.5 * x * x + d - d * v + d * log(v)) return d * v;
-  }
-  return d;
-}
-
-template <int P>
-__device__ void chol_lower(double (&a)[P][P])
-{
-  for (int j = 0; j < P; j++) {
-    double d = a[j][j];
-    for (int k = 0; k < j; k++) d -= a[j][k] * a[j][k];
-    d = sqrt(d);
-    a[j][j] = d;
-    for (int i = j + 1; i < P; i++) {
-      double s = a[i][j];
-      for (int k = 0; k < j; k++) s -= a[i][k] * a[j][k];
-      a[i][j] = s / d;
-    }
-    for (int i = 0; i < j; i++) a[i][j] = 0.0;
-  }
-}
-
-template <int P>
-__global__ __launch_bounds__(64) void k_wishart(DevProblem pb)
-{
-  constexpr int NE = P * (P + 1) / 2;
-  const int chain = blockIdx.x;
-  const int lane = threadIdx.x;
-  __shared__ double T[NE];
-  __shared__ double A[P][P];
-  const uint32_t sweep = pb.sweep[chain];
-  const double *part = pb.partials + (size_t)chain * PAIR_BLOCKS * NE;
-  for (int e = 0; e < NE; e++) {
-    double v = 0.0;
-    for (int b = lane; b < PAIR_BLOCKS; b += 64) v += part[(size_t)b * NE + e];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) T[e] = v;
-  }
-  for (int idx = lane; idx < P * P; idx += 64) {
-    const int i = idx / P, j = idx % P;
-    double val = 0.0;
-    if (i == j) {
-      val = sqrt(2.0 * gamma_mt(0.5 * (double)(pb.df - i), (uint32_t)i, sweep, (uint32_t)chain,
-                                pb.key0, pb.key1));
-    } else if (j < i) {
-      uint32_t rr[4];
-      philox10((uint32_t)idx, sweep, TAG_WNORM, (uint32_t)chain, pb.key0, pb.key1, rr);
-      val = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
-    }
-    A[i][j] = val;
-  }
-  __syncthreads();
-  if (lane != 0) return;
-  double S[P][P];
-  for (int a = 0, e = 0; a < P; a++)
-    for (int c = 0; c <= a; c++, e++) {
-      S[a][c] = T[e];
-      S[c][a] = T[e];
-      pb.tsum[(size_t)chain * P * P + a * P + c] = T[e];
-      pb.tsum[(size_t)chain * P * P + c * P + a] = T[e];
-    }
-  for (int a = 0; a < P; a++) S[a][a] += 1.0;
-  chol_lower<P>(S);
-  // S <- (L L')^-1, column by column
-  double inv[P][P];
-  for (int col = 0; col < P; col++) {
-    double y[P];
-    for (int i = 0; i < P; i++) {
-      double v = (i == col) ? 1.0 : 0.0;
-      for (int k = 0; k < i; k++) v -= S[i][k] * y[k];
-      y[i] = v / S[i][i];
-    }
-    for (int i = P - 1; i >= 0; i--) {
-      double v = y[i];
-      for (int k = i + 1; k < P; k++) v -= S[k][i] * inv[k][col];
-      inv[i][col] = v / S[i][i];
-    }
-  }
-  chol_lower<P>(inv);
-  double B[P][P];
-  for (int i = 0; i < P; i++)
-    for (int j = 0; j < P; j++) {
-      double v = 0.0;
-      if (j <= i)
-        for (int k = j; k <= i; k++) v += inv[i][k] * A[k][j];
-      B[i][j] = v;
-    }
-  float *lam = pb.lam + (size_t)chain * P * P;
-  double *lam64 = pb.lam64 + (size_t)chain * P * P;
-  for (int i = 0; i < P; i++)
-    for (int j = 0; j < P; j++) {
-      const int K = i < j ? i : j;
-      double v = 0.0;
-      for (int k = 0; k <= K; k++) v += B[i][k] * B[j][k];
-      lam64[i * P + j] = v;
-      lam[i * P + j] = (float)v;
-    }
-  pb.sweep[chain] = sweep + 1;
+  block_sum_store<NE>(acc, pb.partials + ((size_t)chain * pb.max_parts + blockIdx.x) * NE);
 }
 
 // ---------------------------------------------------------------------------
@@ -543,13 +810,27 @@ static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
     default: return hipErrorInvalidValue; \
   }
 
-hipError_t launch_latent(const DevProblem &pb, hipStream_t st)
+hipError_t launch_latent(const DevProblem &pb, bool fused, hipStream_t st)
 {
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
-  const long long waves = ((long long)pb.V + (64 >> lshift) - 1) / (64 >> lshift);
+  const int vpw = 64 >> lshift;
+  const long long groups = ((long long)pb.V + vpw - 1) / vpw;
+  const int viter = 8;
+  const long long waves = (groups + viter - 1) / viter;
   dim3 grid((unsigned)((waves + 3) / 4), (unsigned)pb.nchains);
-#define L_(PP) hipLaunchKernelGGL(k_latent<PP>, grid, dim3(256), 0, st, pb, lshift, reps)
+  const bool xreg = reps == 1;
+#define L_(PP)                                                                                   \
+  do {                                                                                           \
+    if (xreg && fused)                                                                           \
+      hipLaunchKernelGGL((k_latent<PP, true, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter);   \
+    else if (xreg)                                                                               \
+      hipLaunchKernelGGL((k_latent<PP, true, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter);  \
+    else if (fused)                                                                              \
+      hipLaunchKernelGGL((k_latent<PP, false, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter);  \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_latent<PP, false, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter); \
+  } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
   return hipGetLastError();
@@ -560,20 +841,35 @@ hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st)
   (void)mode;
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
-  const long long waves = ((long long)pb.V + (64 >> lshift) - 1) / (64 >> lshift);
+  const int vpw = 64 >> lshift;
+  const long long groups = ((long long)pb.V + vpw - 1) / vpw;
+  const int viter = 8;
+  const long long waves = (groups + viter - 1) / viter;
   dim3 grid((unsigned)((waves + 3) / 4), (unsigned)pb.nchains);
-#define L_(PP) hipLaunchKernelGGL(k_xtz_f32<PP>, grid, dim3(256), 0, st, pb, lshift, reps)
+#define L_(PP)                                                                                  \
+  do {                                                                                          \
+    if (reps == 1)                                                                              \
+      hipLaunchKernelGGL((k_xtz_f32<PP, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter);  \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_xtz_f32<PP, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter); \
+  } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
   return hipGetLastError();
 }
 
-hipError_t launch_coef(const DevProblem &pb, int colour, hipStream_t st)
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, hipStream_t st)
 {
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
-  if (nc == 0) return hipSuccess;
+  if (nc == 0) return with_prec ? hipErrorInvalidValue : hipSuccess;
   dim3 grid((unsigned)((nc + 255) / 256), (unsigned)pb.nchains);
-#define L_(PP) hipLaunchKernelGGL(k_coef<PP>, grid, dim3(256), 0, st, pb, colour)
+#define L_(PP)                                                                                \
+  do {                                                                                        \
+    if (with_prec)                                                                            \
+      hipLaunchKernelGGL((k_coef<PP, true>), grid, dim3(256), 0, st, pb, colour);            \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour);           \
+  } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
   return hipGetLastError();

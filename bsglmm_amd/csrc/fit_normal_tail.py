@@ -1,0 +1,62 @@
+"""Fits the FP32 normal-tail approximations used by k_latent (bsg_kernels.hip).
+
+  Q(t)    = P(N(0,1) > t), t >= 0:
+            Q(t) = r exp(-t^2/2 + g(r)),  r = 1/(1 + t/(2 sqrt 2)),
+            g a degree-9 polynomial, least squares on Chebyshev nodes in r.
+  Qinv(p) for p in (1e-38, 1/2]:
+            s = sqrt(-2 ln p),  Qinv(p) = n(s)/d(s), n and d of degree 4 (d(0) = 1),
+            fitted by Lawson-reweighted nonlinear least squares on the error
+            (approx - exact) / max(1, exact).
+
+Run: python bsglmm_amd/csrc/fit_normal_tail.py   (prints the coefficient lists)
+Max errors in FP32 evaluation: Q 3e-6 relative (from rounding t^2/2 near t = 8;
+the fitted function itself is 1e-7), Qinv 5e-7 relative-with-floor-1.
+"""
+import numpy as np
+import scipy.optimize as so
+import scipy.special as sp
+
+C_Q = 1.0 / (2.0 * np.sqrt(2.0))
+
+
+def fit_q(deg=9, nodes=6000):
+    r = 0.5 * (1 + np.cos(np.pi * (np.arange(nodes) + 0.5) / nodes))
+    r = r[r > 1e-4]
+    t = (1 / r - 1) / C_Q
+    g = sp.log_ndtr(-t) - np.log(r) + t * t / 2
+    return np.polynomial.polynomial.polyfit(r, g, deg)
+
+
+def _rational(c, s, m):
+    num = np.polynomial.polynomial.polyval(s, c[:m + 1])
+    den = np.polynomial.polynomial.polyval(s, np.concatenate([[1.0], c[m + 1:]]))
+    return num / den
+
+
+def fit_qinv(m=4, n=4, pmin=1e-38, npts=20000):
+    s = np.linspace(np.sqrt(-2 * np.log(0.5)), np.sqrt(-2 * np.log(pmin)), npts)
+    x = -sp.ndtri_exp(-0.5 * s * s)
+    sc = np.maximum(1.0, x)
+    A = np.concatenate([np.vstack([s ** k for k in range(m + 1)]).T,
+                        -np.vstack([x * s ** k for k in range(1, n + 1)]).T], axis=1)
+    c, *_ = np.linalg.lstsq(A / sc[:, None], x / sc, rcond=None)
+
+    def err(cc, w):
+        return np.sqrt(w) * (_rational(cc, s, m) - x) / sc
+
+    w = np.ones_like(s)
+    c = so.least_squares(err, c, args=(w,), xtol=1e-15, ftol=1e-15, gtol=1e-15, max_nfev=20000).x
+    for _ in range(30):
+        r = np.abs(_rational(c, s, m) - x) / sc
+        w = w * np.sqrt(r)
+        w = w / w.max() + 1e-3
+        c = so.least_squares(err, c, args=(w,), xtol=1e-15, ftol=1e-15, gtol=1e-15, max_nfev=2000).x
+    return c
+
+
+if __name__ == "__main__":
+    q = fit_q()
+    qi = fit_qinv()
+    print("Q coefficients (r^0 .. r^9):", ", ".join(f"{float(np.float32(v))!r}f" for v in q))
+    print("Qinv numerator (s^0 .. s^4):", ", ".join(f"{float(np.float32(v))!r}f" for v in qi[:5]))
+    print("Qinv denominator (s^1 .. s^4):", ", ".join(f"{float(np.float32(v))!r}f" for v in qi[5:]))

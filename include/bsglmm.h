@@ -23,6 +23,7 @@
 #ifndef BSGLMM_H
 #define BSGLMM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -63,7 +64,8 @@ int bsg_layout_export(const bsg_layout *layout, int *host_idx, int *host_idx_sc,
 typedef struct {
   uint64_t seed;        /* Philox key; the CLI derives it from seed.dat as main.cu:413 does */
   int nchains;          /* independent chains over the same data (>= 1) */
-  int xtz_mode;         /* 0: FP32 FMA X'Z; 1: BF16 hi+lo split on MFMA, FP32 accumulate */
+  int xtz_mode;         /* X'Z: 0 FP32 FMA kernel reading Z; 1 BF16 hi+lo split on MFMA,
+                           FP32 accumulate; 2 FP32 FMA fused into the latent kernel */
   int df_offset;        /* Wishart df = TOTVOX + df_offset: 19 as covarGPU.cu:1104, P as covar.cpp:291 */
   float trunc_eps;      /* latent truncation: 1e-6 as covarGPU.cu:1414-1418, 0 as covar.cpp:317,320 */
   float pivot_min;      /* FP32 Cholesky pivot below this -> FP64 refactor of that voxel */
@@ -83,7 +85,8 @@ int bsg_destroy(bsg_ctx *ctx);
 int bsg_sweep(bsg_ctx *ctx, int nsweeps);
 /* updateZGPU (covarGPU.cu:1625-1669) */
 int bsg_update_latent(bsg_ctx *ctx);
-/* X'(Z - b WM), the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352) */
+/* X'(Z - b WM), the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352),
+ * recomputed from Z (xtz_mode 2 already forms it inside bsg_update_latent) */
 int bsg_compute_xtz(bsg_ctx *ctx);
 /* Spat_Coef_Probit for one colour (covarGPU.cu:547-560); reads X'Z from bsg_compute_xtz */
 int bsg_update_coef(bsg_ctx *ctx, int colour);
@@ -94,7 +97,9 @@ int bsg_update_car_prec(bsg_ctx *ctx);
 int bsg_synchronize(bsg_ctx *ctx);
 /* nsweeps sweeps launched kernel by kernel with HIP events around each one;
  * kernel_ms[6] receives the summed durations of latent, xtz, coef0, coef1,
- * pairs, wishart. Returns after the stream drained. */
+ * pairs, wishart (in a full sweep the colour-1 kernel also forms the pair
+ * sums and draws the CAR precision, so pairs and wishart read 0). Returns
+ * after the stream drained. */
 int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6]);
 
 /* ---- state exchange in the reference layouts (chain index first) ------ */
@@ -109,6 +114,11 @@ int bsg_get_pair_sums(bsg_ctx *ctx, int chain, double *t);    /* double[P][P], l
 int bsg_get_sweep_index(bsg_ctx *ctx, int chain, uint32_t *sweep);
 int bsg_set_sweep_index(bsg_ctx *ctx, uint32_t sweep);        /* all chains */
 int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count);     /* FP64 Cholesky refactors */
+
+/* ---- measurement utility (no counterpart in the reference) ------------- */
+/* Times reps launches of a float4 stream-copy kernel over `bytes`; gbps is
+ * (read + written bytes) / time. Calibrates bandwidth and counter figures. */
+int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_per_copy);
 
 /* ---- host KISS, for seed.dat (randgen.cpp:9-23, 36-45; main.cu:54, 413) */
 double bsg_kiss(unsigned long seed[3]);

@@ -27,12 +27,12 @@ def spat_coef_from_ranks(beta, lay):
     return sc
 
 
-def problem(name="c1", nsubs=None, ncovar=None, nchains=1):
+def problem(name="c1", nsubs=None, ncovar=None, nchains=1, xtz_mode=0):
     d = synth.generate(name, nsubs=nsubs, ncovar=ncovar)
     nx, ny, nz = d["grid"]
     lay = oracle.layout(d["mask"], nx, ny, nz)
     L = Layout(d["mask"], nx, ny, nz)
-    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains)
+    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=xtz_mode)
     O = oracle.Oracle(lay, d["covar"], d["data"], seed=SEED)
     rng = np.random.default_rng(7)
     beta = d["beta_true"] + 0.1 * rng.standard_normal(d["beta_true"].shape).astype(np.float32)
@@ -43,10 +43,13 @@ def rel_err(a, b, floor=1.0):
     return np.max(np.abs(a - b) / np.maximum(floor, np.abs(b)))
 
 
-@pytest.mark.parametrize("name,nsubs,ncovar", [("c1", None, None), ("c1", 37, 8), ("c1", 300, 4)])
-def test_one_sweep_every_update_matches_oracle(name, nsubs, ncovar):
-    print(f"case {name} N={nsubs} P={ncovar}", flush=True)
-    d, lay, L, S, O, sc0 = problem(name, nsubs, ncovar)
+@pytest.mark.parametrize("xtz_mode", [0, 2])
+@pytest.mark.parametrize("name,nsubs,ncovar", [("c1", None, None), ("c1", 37, 8), ("c1", 300, 4),
+                                               ("c1", 5, 2), ("c1", 200, 8), ("c1", 160, 6),
+                                               ("c1", 140, 1)])
+def test_one_sweep_every_update_matches_oracle(name, nsubs, ncovar, xtz_mode):
+    print(f"case {name} N={nsubs} P={ncovar} xtz_mode={xtz_mode}", flush=True)
+    d, lay, L, S, O, sc0 = problem(name, nsubs, ncovar, xtz_mode=xtz_mode)
     P = S.P
     sweep = 11
     S.set_spat_coef(sc0)
@@ -62,8 +65,9 @@ def test_one_sweep_every_update_matches_oracle(name, nsubs, ncovar):
     y = d["data"] == 1
     assert np.all(z_gpu[y] > 0) and np.all(z_gpu[~y] < 0)
 
-    # X'Z on the GPU's own Z
-    S.compute_xtz()
+    # X'Z on the GPU's own Z (mode 2 formed it inside the latent kernel)
+    if xtz_mode != 2:
+        S.compute_xtz()
     m_gpu = S.get_xtz()
     m_orc = O.xtz(z_gpu)
     scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_gpu.astype(np.float64))
@@ -90,6 +94,42 @@ def test_one_sweep_every_update_matches_oracle(name, nsubs, ncovar):
     assert rel_err(lam_gpu, lam_orc, floor=np.abs(lam_orc).max()) < 1e-6
     assert S.sweep_index() == sweep + 1
     S.synchronize()
+
+
+@pytest.mark.parametrize("xtz_mode", [0, 2])
+@pytest.mark.parametrize("nsubs,ncovar", [(None, None), (200, 8), (60, 5)])
+def test_full_sweep_matches_oracle(nsubs, ncovar, xtz_mode):
+    """One bsg_sweep: the colour-1 kernel also forms the pair sums and draws Lambda."""
+    print(f"case full sweep N={nsubs} P={ncovar} xtz_mode={xtz_mode}", flush=True)
+    d, lay, L, S, O, sc0 = problem("c1", nsubs, ncovar, xtz_mode=xtz_mode)
+    P = S.P
+    sweep = 5
+    S.set_spat_coef(sc0)
+    S.set_sweep_index(sweep)
+    prec0 = (np.eye(P) * 1.5 + 0.2).astype(np.float32)
+    S.set_prec(prec0)
+    S.sweep(1)
+    S.synchronize()
+    z_gpu, m_gpu, b_gpu = S.get_z(), S.get_xtz(), S.get_spat_coef()
+    assert rel_err(z_gpu, O.update_z(sc0, sweep)) < 1e-5
+    scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_gpu.astype(np.float64))
+    assert np.max(np.abs(m_gpu - O.xtz(z_gpu)) / scale) < 1e-5
+    ref = O.update_coef(0, m_gpu.astype(np.float64), prec0, sc0.astype(np.float64), sweep)
+    ref = O.update_coef(1, m_gpu.astype(np.float64), prec0, ref, sweep)
+    assert rel_err(b_gpu, ref) < 1e-5
+    t_gpu = S.get_pair_sums()
+    t_orc = O.pair_sums(b_gpu)
+    assert np.max(np.abs(t_gpu - t_orc) / np.abs(t_orc).max()) < 1e-12
+    lam_orc = O.wishart(t_gpu, lay["totvox"] + 19, sweep)
+    lam_gpu = S.get_prec()
+    assert rel_err(lam_gpu, lam_orc, floor=np.abs(lam_orc).max()) < 1e-6
+    assert S.sweep_index() == sweep + 1
+    # a second sweep reuses the completion counter: it must still elect one block
+    S.sweep(1)
+    S.synchronize()
+    assert S.sweep_index() == sweep + 2
+    t2 = S.get_pair_sums()
+    assert np.max(np.abs(t2 - O.pair_sums(S.get_spat_coef())) / np.abs(t2).max()) < 1e-12
 
 
 def test_full_size_invariants_and_determinism():

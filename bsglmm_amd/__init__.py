@@ -63,6 +63,7 @@ def lib():
         "bsg_update_car_prec": (C.c_int, [vp]),
         "bsg_synchronize": (C.c_int, [vp]),
         "bsg_sweep_profiled": (C.c_int, [vp, C.c_int, pd]),
+        "bsg_sweep_timed": (C.c_int, [vp, C.c_int, pd, pd]),
         "bsg_get_spat_coef": (C.c_int, [vp, C.c_int, pf]),
         "bsg_set_spat_coef": (C.c_int, [vp, C.c_int, pf]),
         "bsg_get_z": (C.c_int, [vp, C.c_int, pf]),
@@ -169,6 +170,12 @@ class Sampler:
         ms = np.zeros(6, dtype=np.float64)
         _check(lib().bsg_sweep_profiled(self._h, int(n), _ptr(ms, C.c_double)))
         return dict(zip(self.KERNELS, ms))
+
+    def sweep_timed(self, n):
+        """n sweeps; returns (summed latent-kernel ms, total ms) from HIP events."""
+        lat, tot = C.c_double(), C.c_double()
+        _check(lib().bsg_sweep_timed(self._h, int(n), C.byref(lat), C.byref(tot)))
+        return lat.value, tot.value
 
     def update_latent(self):
         _check(lib().bsg_update_latent(self._h))

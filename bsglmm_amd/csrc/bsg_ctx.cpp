@@ -27,22 +27,33 @@ struct bsg_ctx {
   bsg::Layout L;
   bsg_opts opts{};
   int N = 0, P = 0, NS = 0, NQ = 0, NW = 0, C = 1;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // all work; the caller-visible stream
+  hipStream_t stream2 = nullptr;  // CAR-precision draw of sweep t beside the latent draw of t+1
   bsg::DevProblem pb{};
   std::vector<void *> allocs;
   hipEvent_t ev[7] = {};
+  hipEvent_t e_cf1 = nullptr, e_wish = nullptr;
+  bool wish_pending = false;
+  std::vector<hipEvent_t> lat_ev;  // timing pool for bsg_sweep_timed
 
   void release()
   {
     if (stream) hipStreamSynchronize(stream);
+    if (stream2) hipStreamSynchronize(stream2);
     for (void *p : allocs) hipFree(p);
     allocs.clear();
     for (auto &e : ev) {
       if (e) hipEventDestroy(e);
       e = nullptr;
     }
+    for (auto &e : lat_ev) hipEventDestroy(e);
+    lat_ev.clear();
+    if (e_cf1) hipEventDestroy(e_cf1);
+    if (e_wish) hipEventDestroy(e_wish);
+    e_cf1 = e_wish = nullptr;
+    if (stream2) hipStreamDestroy(stream2);
     if (stream) hipStreamDestroy(stream);
-    stream = nullptr;
+    stream = stream2 = nullptr;
   }
   ~bsg_ctx() { release(); }
 };
@@ -226,6 +237,9 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
 
   // ---- device state ----
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&ctx->e_cf1, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ctx->e_wish, hipEventDisableTiming));
   bsg::DevProblem &pb = ctx->pb;
   pb.V = V;
   pb.N = N;
@@ -262,6 +276,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.max_parts = std::max(bsg::PAIR_BLOCKS, (std::max(L.n[0], L.n[1]) + 255) / 256);
   ALLOC(pb.partials, (size_t)C * pb.max_parts * NE);
   ALLOC(pb.counter, (size_t)C);
+  ALLOC(pb.wsweep, (size_t)C);
   ALLOC(pb.bwm, (size_t)C);
   ALLOC(pb.sweep, (size_t)C);
   ALLOC(pb.flags, (size_t)4);
@@ -288,6 +303,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemsetAsync(pb.sweep, 0, sizeof(uint32_t) * C, st));
   HIPCHK(hipMemsetAsync(pb.flags, 0, sizeof(uint32_t) * 4, st));
   HIPCHK(hipMemsetAsync(pb.counter, 0, sizeof(uint32_t) * C, st));
+  HIPCHK(hipMemsetAsync(pb.wsweep, 0, sizeof(uint32_t) * C, st));
   std::vector<float> lam((size_t)C * P * P, 0.0f);
   std::vector<double> lam64((size_t)C * P * P, 0.0);
   for (int c = 0; c < C; c++)
@@ -344,22 +360,44 @@ int bsg_update_car_prec(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
   HIPCHK(bsg::launch_pairs(ctx->pb, ctx->stream));
-  HIPCHK(bsg::launch_wishart(ctx->pb, ctx->stream));
+  HIPCHK(bsg::launch_wishart(ctx->pb, false, ctx->stream));
   return BSG_OK;
 }
 
-static int sweep_once(bsg_ctx *ctx)
+// Full sweeps. The CAR-precision draw of sweep t (final pair-sum reduction and
+// Wishart, side stream) only has to finish before the colour-0 update of
+// sweep t+1, so it overlaps the latent draw of sweep t+1 on the main stream.
+// lat_ev (2 per sweep, optional) brackets each latent kernel for timing.
+static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev)
 {
-  hipStream_t st = ctx->stream;
+  hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
   const bool fused = ctx->opts.xtz_mode == 2;
-  HIPCHK(bsg::launch_latent(ctx->pb, fused, st));
-  if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
-  HIPCHK(bsg::launch_coef(ctx->pb, 0, false, st));
-  if (ctx->L.n[1] > 0) {
-    HIPCHK(bsg::launch_coef(ctx->pb, 1, true, st));  // colour 1 + pair sums + Wishart
-  } else {
-    HIPCHK(bsg::launch_pairs(ctx->pb, st));
-    HIPCHK(bsg::launch_wishart(ctx->pb, st));
+  const bool one_pass = ctx->L.n[1] > 0;
+  for (int i = 0; i < nsweeps; i++) {
+    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i], s1));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, s1));
+    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i + 1], s1));
+    if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, s1));
+    if (ctx->wish_pending) {
+      HIPCHK(hipStreamWaitEvent(s1, ctx->e_wish, 0));
+      ctx->wish_pending = false;
+    }
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, s1));
+    if (one_pass) {
+      HIPCHK(bsg::launch_coef(ctx->pb, 1, true, s1));  // colour 1 + pair partials
+      HIPCHK(hipEventRecord(ctx->e_cf1, s1));
+      HIPCHK(hipStreamWaitEvent(s2, ctx->e_cf1, 0));
+      HIPCHK(bsg::launch_wishart(ctx->pb, true, s2));
+      HIPCHK(hipEventRecord(ctx->e_wish, s2));
+      ctx->wish_pending = true;
+    } else {
+      HIPCHK(bsg::launch_pairs(ctx->pb, s1));
+      HIPCHK(bsg::launch_wishart(ctx->pb, false, s1));
+    }
+  }
+  if (ctx->wish_pending) {  // join: the caller's stream sees the finished sweep
+    HIPCHK(hipStreamWaitEvent(s1, ctx->e_wish, 0));
+    ctx->wish_pending = false;
   }
   return BSG_OK;
 }
@@ -371,9 +409,38 @@ int bsg_sweep(bsg_ctx *ctx, int nsweeps)
     set_error("bsg_sweep: negative sweep count");
     return BSG_ERR_ARG;
   }
+  return sweep_run(ctx, nsweeps, nullptr);
+}
+
+int bsg_sweep_timed(bsg_ctx *ctx, int nsweeps, double *latent_ms, double *total_ms)
+{
+  CTXCHK(ctx);
+  if (nsweeps < 1 || !latent_ms) {
+    set_error("bsg_sweep_timed: bad argument");
+    return BSG_ERR_ARG;
+  }
+  while ((int)ctx->lat_ev.size() < 2 * nsweeps + 2) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    ctx->lat_ev.push_back(e);
+  }
+  hipEvent_t *ev = ctx->lat_ev.data();
+  HIPCHK(hipEventRecord(ev[2 * nsweeps], ctx->stream));
+  int rc = sweep_run(ctx, nsweeps, ev);
+  if (rc != BSG_OK) return rc;
+  HIPCHK(hipEventRecord(ev[2 * nsweeps + 1], ctx->stream));
+  HIPCHK(hipEventSynchronize(ev[2 * nsweeps + 1]));
+  double acc = 0.0;
   for (int i = 0; i < nsweeps; i++) {
-    int rc = sweep_once(ctx);
-    if (rc != BSG_OK) return rc;
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+    acc += ms;
+  }
+  *latent_ms = acc;
+  if (total_ms) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev[2 * nsweeps], ev[2 * nsweeps + 1]));
+    *total_ms = ms;
   }
   return BSG_OK;
 }
@@ -382,6 +449,7 @@ int bsg_synchronize(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream2));
   uint32_t flags[4];
   HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[1]) {
@@ -415,7 +483,7 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
     HIPCHK(hipEventRecord(ctx->ev[4], st));
     if (!one_pass) HIPCHK(bsg::launch_pairs(ctx->pb, st));
     HIPCHK(hipEventRecord(ctx->ev[5], st));
-    if (!one_pass) HIPCHK(bsg::launch_wishart(ctx->pb, st));
+    HIPCHK(bsg::launch_wishart(ctx->pb, one_pass, st));
     HIPCHK(hipEventRecord(ctx->ev[6], st));
     HIPCHK(hipEventSynchronize(ctx->ev[6]));
     for (int k = 0; k < 6; k++) {

@@ -57,6 +57,7 @@ struct DevProblem {
   double *partials;        // [chain][max_parts][P(P+1)/2]
   int max_parts;           // partial slots per chain
   uint32_t *counter;       // [chain] completion counter of the fused colour-1 kernel
+  uint32_t *wsweep;        // [chain] sweep index recorded for the side-stream Wishart kernel
   float *bwm;              // [chain]  WM coefficient b
   uint32_t *sweep;         // [chain]  sweep index, advanced by the Wishart kernel
   uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
@@ -67,7 +68,7 @@ hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st);
 hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, hipStream_t st);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
-hipError_t launch_wishart(const DevProblem &pb, hipStream_t st);
+hipError_t launch_wishart(const DevProblem &pb, bool side, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
 void set_error(const std::string &msg);

@@ -437,28 +437,42 @@ __device__ void chol_lower(double (&a)[P][P])
   }
 }
 
+// Sum of the per-block partials in a fixed order: thread t adds blocks
+// t, t+256, ... of every entry, then a fixed LDS tree over the 256 threads.
+template <int NE>
+__device__ void sum_partials(int nparts, const double *part, double *T)
+{
+  __shared__ double red[256];
+  double acc[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) acc[e] = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
+  }
+  for (int e = 0; e < NE; e++) {
+    red[threadIdx.x] = acc[e];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) T[e] = red[0];
+    __syncthreads();
+  }
+}
+
+// Block of 256 threads: partial sums, Bartlett variates, FP64 algebra.
 template <int P>
-__device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part)
+__device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part,
+                              uint32_t sweep, bool advance)
 {
   constexpr int NE = P * (P + 1) / 2;
   __shared__ double T[NE];
   __shared__ double A[P][P];
+  sum_partials<NE>(nparts, part, T);
   const int lane = threadIdx.x;
   if (lane >= 64) return;
-  const uint32_t sweep = pb.sweep[chain];
-  double acc[NE];
-#pragma unroll
-  for (int e = 0; e < NE; e++) acc[e] = 0.0;
-  for (int b = lane; b < nparts; b += 64) {
-#pragma unroll
-    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
-  }
-#pragma unroll
-  for (int e = 0; e < NE; e++) {
-    double v = acc[e];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) T[e] = v;
-  }
   for (int idx = lane; idx < P * P; idx += 64) {
     const int i = idx / P, j = idx % P;
     double val = 0.0;
@@ -519,15 +533,20 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
       lam64[i * P + j] = v;
       lam[i * P + j] = (float)v;
     }
-  pb.sweep[chain] = sweep + 1;
+  if (advance) pb.sweep[chain] = sweep + 1;
 }
 
-template <int P>
-__global__ __launch_bounds__(64) void k_wishart(DevProblem pb)
+// SIDE = false: after k_pairs, uses and advances the sweep index.
+// SIDE = true: after the colour-1 kernel of a full sweep, on a second stream;
+// uses the sweep index that kernel recorded (it already advanced the counter).
+template <int P, bool SIDE>
+__global__ __launch_bounds__(256) void k_wishart(DevProblem pb)
 {
   constexpr int NE = P * (P + 1) / 2;
   const int chain = blockIdx.x;
-  wishart_block<P>(pb, chain, PAIR_BLOCKS, pb.partials + (size_t)chain * pb.max_parts * NE);
+  const int nparts = SIDE ? (pb.V - pb.n0 + 255) / 256 : PAIR_BLOCKS;
+  const uint32_t sweep = SIDE ? pb.wsweep[chain] : pb.sweep[chain];
+  wishart_block<P>(pb, chain, nparts, pb.partials + (size_t)chain * pb.max_parts * NE, sweep, !SIDE);
 }
 
 // Adds the pair terms of voxel j (new coefficients bj) with neighbour k
@@ -736,15 +755,15 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour)
   }
   double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
   block_sum_store<NE>(acc, parts + (size_t)blockIdx.x * NE);
-  __shared__ bool last;
+  // the last block to finish records this sweep's index for the Wishart
+  // kernel and advances the counter (every block has read it by then)
   __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(&pb.counter[chain], 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (threadIdx.x == 0) pb.counter[chain] = 0;
-  wishart_block<P>(pb, chain, (int)gridDim.x, parts);
+  if (threadIdx.x == 0 && atomicAdd(&pb.counter[chain], 1u) == gridDim.x - 1) {
+    pb.counter[chain] = 0;
+    pb.wsweep[chain] = sweep;
+    pb.sweep[chain] = sweep + 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -884,10 +903,16 @@ hipError_t launch_pairs(const DevProblem &pb, hipStream_t st)
   return hipGetLastError();
 }
 
-hipError_t launch_wishart(const DevProblem &pb, hipStream_t st)
+hipError_t launch_wishart(const DevProblem &pb, bool side, hipStream_t st)
 {
   dim3 grid((unsigned)pb.nchains);
-#define L_(PP) hipLaunchKernelGGL(k_wishart<PP>, grid, dim3(64), 0, st, pb)
+#define L_(PP)                                                                   \
+  do {                                                                           \
+    if (side)                                                                    \
+      hipLaunchKernelGGL((k_wishart<PP, true>), grid, dim3(256), 0, st, pb);     \
+    else                                                                         \
+      hipLaunchKernelGGL((k_wishart<PP, false>), grid, dim3(256), 0, st, pb);    \
+  } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
   return hipGetLastError();

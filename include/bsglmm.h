@@ -81,7 +81,10 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
 int bsg_destroy(bsg_ctx *ctx);
 
 /* ---- the sweep (mcmc.cpp:311-394, GPU branch, probit, no WM image) ---- */
-/* nsweeps full sweeps: latent, X'Z, coefficients colour 0 then 1, CAR precision */
+/* nsweeps full sweeps: latent, X'Z, coefficients colour 0 then 1, CAR
+ * precision. The CAR-precision draw of sweep t runs on a second internal
+ * stream beside the latent draw of sweep t+1; the call returns with all work
+ * ordered on the context's stream. */
 int bsg_sweep(bsg_ctx *ctx, int nsweeps);
 /* updateZGPU (covarGPU.cu:1625-1669) */
 int bsg_update_latent(bsg_ctx *ctx);
@@ -95,6 +98,9 @@ int bsg_update_coef(bsg_ctx *ctx, int colour);
 int bsg_update_car_prec(bsg_ctx *ctx);
 /* waits for the context's stream; reports kernel-side numerical failures */
 int bsg_synchronize(bsg_ctx *ctx);
+/* bsg_sweep with HIP events around every latent kernel: latent_ms receives
+ * their summed durations, total_ms (optional) the whole span. Synchronous. */
+int bsg_sweep_timed(bsg_ctx *ctx, int nsweeps, double *latent_ms, double *total_ms);
 /* nsweeps sweeps launched kernel by kernel with HIP events around each one;
  * kernel_ms[6] receives the summed durations of latent, xtz, coef0, coef1,
  * pairs, wishart (in a full sweep the colour-1 kernel also forms the pair

@@ -27,6 +27,7 @@ struct bsg_ctx {
   bsg::Layout L;
   bsg_opts opts{};
   int N = 0, P = 0, NS = 0, NQ = 0, NW = 0, C = 1;
+  uint32_t sweep = 0;  // index of the next sweep (the Philox counter word), same for all chains
   hipStream_t stream = nullptr;   // all work; the caller-visible stream
   hipStream_t stream2 = nullptr;  // CAR-precision draw of sweep t beside the latent draw of t+1
   bsg::DevProblem pb{};
@@ -275,10 +276,9 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(pb.tsum, (size_t)C * P * P);
   pb.max_parts = std::max(bsg::PAIR_BLOCKS, (std::max(L.n[0], L.n[1]) + 255) / 256);
   ALLOC(pb.partials, (size_t)C * pb.max_parts * NE);
-  ALLOC(pb.counter, (size_t)C);
-  ALLOC(pb.wsweep, (size_t)C);
+
   ALLOC(pb.bwm, (size_t)C);
-  ALLOC(pb.sweep, (size_t)C);
+
   ALLOC(pb.flags, (size_t)4);
 #undef ALLOC
   pb.nbr6 = nbr6;
@@ -300,10 +300,9 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemsetAsync(pb.M, 0, sizeof(float) * (size_t)C * V * P, st));
   HIPCHK(hipMemsetAsync(pb.tsum, 0, sizeof(double) * (size_t)C * P * P, st));
   HIPCHK(hipMemsetAsync(pb.bwm, 0, sizeof(float) * C, st));
-  HIPCHK(hipMemsetAsync(pb.sweep, 0, sizeof(uint32_t) * C, st));
+
   HIPCHK(hipMemsetAsync(pb.flags, 0, sizeof(uint32_t) * 4, st));
-  HIPCHK(hipMemsetAsync(pb.counter, 0, sizeof(uint32_t) * C, st));
-  HIPCHK(hipMemsetAsync(pb.wsweep, 0, sizeof(uint32_t) * C, st));
+
   std::vector<float> lam((size_t)C * P * P, 0.0f);
   std::vector<double> lam64((size_t)C * P * P, 0.0);
   for (int c = 0; c < C; c++)
@@ -334,7 +333,7 @@ int bsg_destroy(bsg_ctx *ctx)
 int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->stream));
+  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->sweep, ctx->stream));
   return BSG_OK;
 }
 
@@ -352,7 +351,7 @@ int bsg_update_coef(bsg_ctx *ctx, int colour)
     set_error("bsg_update_coef: colour must be 0 or 1");
     return BSG_ERR_ARG;
   }
-  HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->stream));
+  HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->sweep, ctx->stream));
   return BSG_OK;
 }
 
@@ -360,7 +359,8 @@ int bsg_update_car_prec(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
   HIPCHK(bsg::launch_pairs(ctx->pb, ctx->stream));
-  HIPCHK(bsg::launch_wishart(ctx->pb, false, ctx->stream));
+  HIPCHK(bsg::launch_wishart(ctx->pb, false, ctx->sweep, ctx->stream));
+  ctx->sweep++;
   return BSG_OK;
 }
 
@@ -374,26 +374,28 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev)
   const bool fused = ctx->opts.xtz_mode == 2;
   const bool one_pass = ctx->L.n[1] > 0;
   for (int i = 0; i < nsweeps; i++) {
+    const uint32_t sw = ctx->sweep;
     if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i], s1));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, s1));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, sw, s1));
     if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i + 1], s1));
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, s1));
     if (ctx->wish_pending) {
       HIPCHK(hipStreamWaitEvent(s1, ctx->e_wish, 0));
       ctx->wish_pending = false;
     }
-    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, s1));
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, s1));
     if (one_pass) {
-      HIPCHK(bsg::launch_coef(ctx->pb, 1, true, s1));  // colour 1 + pair partials
+      HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, s1));  // colour 1 + pair partials
       HIPCHK(hipEventRecord(ctx->e_cf1, s1));
       HIPCHK(hipStreamWaitEvent(s2, ctx->e_cf1, 0));
-      HIPCHK(bsg::launch_wishart(ctx->pb, true, s2));
+      HIPCHK(bsg::launch_wishart(ctx->pb, true, sw, s2));
       HIPCHK(hipEventRecord(ctx->e_wish, s2));
       ctx->wish_pending = true;
     } else {
       HIPCHK(bsg::launch_pairs(ctx->pb, s1));
-      HIPCHK(bsg::launch_wishart(ctx->pb, false, s1));
+      HIPCHK(bsg::launch_wishart(ctx->pb, false, sw, s1));
     }
+    ctx->sweep = sw + 1;
   }
   if (ctx->wish_pending) {  // join: the caller's stream sees the finished sweep
     HIPCHK(hipStreamWaitEvent(s1, ctx->e_wish, 0));
@@ -471,20 +473,22 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
   hipStream_t st = ctx->stream;
   for (int i = 0; i < nsweeps; i++) {
     const bool fused = ctx->opts.xtz_mode == 2;
+    const uint32_t sw = ctx->sweep;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, st));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
-    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, st));
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[3], st));
     const bool one_pass = ctx->L.n[1] > 0;
-    if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, st));
+    if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[4], st));
     if (!one_pass) HIPCHK(bsg::launch_pairs(ctx->pb, st));
     HIPCHK(hipEventRecord(ctx->ev[5], st));
-    HIPCHK(bsg::launch_wishart(ctx->pb, one_pass, st));
+    HIPCHK(bsg::launch_wishart(ctx->pb, one_pass, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[6], st));
+    ctx->sweep = sw + 1;
     HIPCHK(hipEventSynchronize(ctx->ev[6]));
     for (int k = 0; k < 6; k++) {
       float ms = 0.0f;
@@ -605,17 +609,14 @@ int bsg_get_pair_sums(bsg_ctx *ctx, int chain, double *t)
 int bsg_get_sweep_index(bsg_ctx *ctx, int chain, uint32_t *sweep)
 {
   CHAINCHK(ctx, chain);
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  HIPCHK(hipMemcpy(sweep, ctx->pb.sweep + chain, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  *sweep = ctx->sweep;
   return BSG_OK;
 }
 
 int bsg_set_sweep_index(bsg_ctx *ctx, uint32_t sweep)
 {
   CTXCHK(ctx);
-  std::vector<uint32_t> v(ctx->C, sweep);
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  HIPCHK(hipMemcpy(ctx->pb.sweep, v.data(), sizeof(uint32_t) * v.size(), hipMemcpyHostToDevice));
+  ctx->sweep = sweep;
   return BSG_OK;
 }
 

@@ -10,7 +10,8 @@
 
 namespace bsg {
 
-// Philox counter tags: counter = (index, sweep, tag | sub << 4, chain)
+// Philox counter tags: counter = (index, sweep, tag | sub << 4, chain). The sweep
+// index is passed to every kernel as an argument by the host, which owns it.
 constexpr uint32_t TAG_LATENT = 1;  // index = voxel * NQ + subject quad
 constexpr uint32_t TAG_COEF = 2;    // index = voxel, sub = group of 4 normals
 constexpr uint32_t TAG_GAMMA = 3;   // index = Bartlett row, sub = trial
@@ -56,19 +57,16 @@ struct DevProblem {
   double *tsum;            // [chain][P][P]
   double *partials;        // [chain][max_parts][P(P+1)/2]
   int max_parts;           // partial slots per chain
-  uint32_t *counter;       // [chain] completion counter of the fused colour-1 kernel
-  uint32_t *wsweep;        // [chain] sweep index recorded for the side-stream Wishart kernel
   float *bwm;              // [chain]  WM coefficient b
-  uint32_t *sweep;         // [chain]  sweep index, advanced by the Wishart kernel
   uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
 };
 
 // kernel launchers (bsg_kernels.hip)
-hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, hipStream_t st);
+hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, uint32_t sweep, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st);
-hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, hipStream_t st);
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
-hipError_t launch_wishart(const DevProblem &pb, bool side, hipStream_t st);
+hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
 void set_error(const std::string &msg);

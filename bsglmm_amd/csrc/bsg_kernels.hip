@@ -169,7 +169,7 @@ constexpr int red_pad() { return P <= 4 ? 4 : 8; }
 template <int P>
 __device__ __forceinline__ int red_lane(int c)
 {
-  if (red_pad<P>() == 4) return 16 * c;
+  if constexpr (red_pad<P>() == 4) return 16 * c;
   // rows hold {a0,a2},{a1,a3},{a4,a6},{a5,a7} as (low half, high half)
   const int row = (c & 1) + 2 * (c >> 2);
   const int half = (c >> 1) & 1;
@@ -190,7 +190,7 @@ __device__ __forceinline__ float wave_reduce_scatter(const float (&acc)[P], int 
     swap32(x, y);
     b[k] = x + y;  // lanes 0-31: a_k, lanes 32-63: a_{k+PP/2}
   }
-  if (PP == 4) {
+  if constexpr (PP == 4) {
     float x = b[0], y = b[1];
     swap16(x, y);
     float c = x + y;  // row r holds a_r
@@ -222,7 +222,7 @@ __device__ __forceinline__ float wave_reduce_scatter(const float (&acc)[P], int 
 // m_j = X'(Z_j - b WM_j) from the fresh draws (no re-read of Z).
 // ---------------------------------------------------------------------------
 template <int P, bool XREG, bool FUSED>
-__global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps, int viter)
+__global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps, int viter, uint32_t sweep)
 {
   const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
@@ -230,7 +230,6 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
   const int vpw = 64 >> lshift;
   const int sub = lane >> lshift;
   const int ql = lane & ((1 << lshift) - 1);
-  const uint32_t sweep = pb.sweep[chain];
   const float b = pb.bwm[chain];
   const float eps = pb.trunc_eps;
   float4 xr[P];
@@ -465,7 +464,7 @@ __device__ void sum_partials(int nparts, const double *part, double *T)
 // Block of 256 threads: partial sums, Bartlett variates, FP64 algebra.
 template <int P>
 __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part,
-                              uint32_t sweep, bool advance)
+                              uint32_t sweep)
 {
   constexpr int NE = P * (P + 1) / 2;
   __shared__ double T[NE];
@@ -533,20 +532,18 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
       lam64[i * P + j] = v;
       lam[i * P + j] = (float)v;
     }
-  if (advance) pb.sweep[chain] = sweep + 1;
 }
 
-// SIDE = false: after k_pairs, uses and advances the sweep index.
-// SIDE = true: after the colour-1 kernel of a full sweep, on a second stream;
-// uses the sweep index that kernel recorded (it already advanced the counter).
+// SIDE = false: after k_pairs (PAIR_BLOCKS partials). SIDE = true: after the
+// colour-1 kernel of a full sweep (one partial per colour-1 block), on the
+// context's second stream.
 template <int P, bool SIDE>
-__global__ __launch_bounds__(256) void k_wishart(DevProblem pb)
+__global__ __launch_bounds__(256) void k_wishart(DevProblem pb, uint32_t sweep)
 {
   constexpr int NE = P * (P + 1) / 2;
   const int chain = blockIdx.x;
   const int nparts = SIDE ? (pb.V - pb.n0 + 255) / 256 : PAIR_BLOCKS;
-  const uint32_t sweep = SIDE ? pb.wsweep[chain] : pb.sweep[chain];
-  wishart_block<P>(pb, chain, nparts, pb.partials + (size_t)chain * pb.max_parts * NE, sweep, !SIDE);
+  wishart_block<P>(pb, chain, nparts, pb.partials + (size_t)chain * pb.max_parts * NE, sweep);
 }
 
 // Adds the pair terms of voxel j (new coefficients bj) with neighbour k
@@ -644,7 +641,7 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
 }
 
 template <int P, bool PREC>
-__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour)
+__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep)
 {
   constexpr int NE = P * (P + 1) / 2;
   const int chain = blockIdx.y;
@@ -652,7 +649,6 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour)
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   const bool active = q < nc;
   const int s = (active ? q : nc - 1) + (colour ? pb.n0 : 0);
-  const uint32_t sweep = pb.sweep[chain];
   const float *betc = pb.beta + (size_t)chain * pb.V * P;
   const float *lam = pb.lam + (size_t)chain * P * P;
 
@@ -755,15 +751,6 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour)
   }
   double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
   block_sum_store<NE>(acc, parts + (size_t)blockIdx.x * NE);
-  // the last block to finish records this sweep's index for the Wishart
-  // kernel and advances the counter (every block has read it by then)
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&pb.counter[chain], 1u) == gridDim.x - 1) {
-    pb.counter[chain] = 0;
-    pb.wsweep[chain] = sweep;
-    pb.sweep[chain] = sweep + 1;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -829,7 +816,7 @@ static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
     default: return hipErrorInvalidValue; \
   }
 
-hipError_t launch_latent(const DevProblem &pb, bool fused, hipStream_t st)
+hipError_t launch_latent(const DevProblem &pb, bool fused, uint32_t sweep, hipStream_t st)
 {
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
@@ -842,13 +829,13 @@ hipError_t launch_latent(const DevProblem &pb, bool fused, hipStream_t st)
 #define L_(PP)                                                                                   \
   do {                                                                                           \
     if (xreg && fused)                                                                           \
-      hipLaunchKernelGGL((k_latent<PP, true, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter);   \
+      hipLaunchKernelGGL((k_latent<PP, true, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep);   \
     else if (xreg)                                                                               \
-      hipLaunchKernelGGL((k_latent<PP, true, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter);  \
+      hipLaunchKernelGGL((k_latent<PP, true, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep);  \
     else if (fused)                                                                              \
-      hipLaunchKernelGGL((k_latent<PP, false, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter);  \
+      hipLaunchKernelGGL((k_latent<PP, false, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep);  \
     else                                                                                         \
-      hipLaunchKernelGGL((k_latent<PP, false, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter); \
+      hipLaunchKernelGGL((k_latent<PP, false, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep); \
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
@@ -877,7 +864,7 @@ hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st)
   return hipGetLastError();
 }
 
-hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, hipStream_t st)
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st)
 {
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   if (nc == 0) return with_prec ? hipErrorInvalidValue : hipSuccess;
@@ -885,9 +872,9 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, hipStre
 #define L_(PP)                                                                                \
   do {                                                                                        \
     if (with_prec)                                                                            \
-      hipLaunchKernelGGL((k_coef<PP, true>), grid, dim3(256), 0, st, pb, colour);            \
+      hipLaunchKernelGGL((k_coef<PP, true>), grid, dim3(256), 0, st, pb, colour, sweep);            \
     else                                                                                      \
-      hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour);           \
+      hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour, sweep);           \
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
@@ -903,15 +890,15 @@ hipError_t launch_pairs(const DevProblem &pb, hipStream_t st)
   return hipGetLastError();
 }
 
-hipError_t launch_wishart(const DevProblem &pb, bool side, hipStream_t st)
+hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st)
 {
   dim3 grid((unsigned)pb.nchains);
 #define L_(PP)                                                                   \
   do {                                                                           \
     if (side)                                                                    \
-      hipLaunchKernelGGL((k_wishart<PP, true>), grid, dim3(256), 0, st, pb);     \
+      hipLaunchKernelGGL((k_wishart<PP, true>), grid, dim3(256), 0, st, pb, sweep);     \
     else                                                                         \
-      hipLaunchKernelGGL((k_wishart<PP, false>), grid, dim3(256), 0, st, pb);    \
+      hipLaunchKernelGGL((k_wishart<PP, false>), grid, dim3(256), 0, st, pb, sweep);    \
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_

@@ -64,6 +64,10 @@ def lib():
         "bsg_synchronize": (C.c_int, [vp]),
         "bsg_sweep_profiled": (C.c_int, [vp, C.c_int, pd]),
         "bsg_sweep_timed": (C.c_int, [vp, C.c_int, pd, pd]),
+        "bsg_accum_reset": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "bsg_accum_add": (C.c_int, [vp]),
+        "bsg_sweep_accumulate": (C.c_int, [vp, C.c_int, C.c_int]),
+        "bsg_accum_get": (C.c_int, [vp, C.c_int, pi, pd, pd, pd, pd, pd]),
         "bsg_get_spat_coef": (C.c_int, [vp, C.c_int, pf]),
         "bsg_set_spat_coef": (C.c_int, [vp, C.c_int, pf]),
         "bsg_get_z": (C.c_int, [vp, C.c_int, pf]),
@@ -176,6 +180,31 @@ class Sampler:
         lat, tot = C.c_double(), C.c_double()
         _check(lib().bsg_sweep_timed(self._h, int(n), C.byref(lat), C.byref(tot)))
         return lat.value, tot.value
+
+    # ---- posterior accumulation ----
+    def accum_reset(self, ntypes=1, nbatches=50, batch_len=1):
+        self._ntypes = int(ntypes)
+        _check(lib().bsg_accum_reset(self._h, int(ntypes), int(nbatches), int(batch_len)))
+
+    def accum_add(self):
+        _check(lib().bsg_accum_add(self._h))
+
+    def sweep_accumulate(self, n, save_every=1):
+        _check(lib().bsg_sweep_accumulate(self._h, int(n), int(save_every)))
+
+    def accum_get(self, chain=0):
+        """dict(ndraws, mean[P][V], var[P][V], prb[T][V], bwm[V], mcse[P+T][V]) in mask-rank order."""
+        T = getattr(self, "_ntypes", 1)
+        out = dict(mean=np.empty((self.P, self.V)), var=np.empty((self.P, self.V)),
+                   prb=np.empty((max(T, 1), self.V)), bwm=np.empty(self.V),
+                   mcse=np.empty((self.P + T, self.V)))
+        n = C.c_int()
+        _check(lib().bsg_accum_get(self._h, int(chain), C.byref(n), _ptr(out["mean"], C.c_double),
+                                   _ptr(out["var"], C.c_double), _ptr(out["prb"], C.c_double),
+                                   _ptr(out["bwm"], C.c_double), _ptr(out["mcse"], C.c_double)))
+        out["ndraws"] = n.value
+        out["prb"] = out["prb"][:T]
+        return out
 
     def update_latent(self):
         _check(lib().bsg_update_latent(self._h))

@@ -2,6 +2,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -36,11 +37,23 @@ struct bsg_ctx {
   hipEvent_t e_cf1 = nullptr, e_wish = nullptr;
   bool wish_pending = false;
   std::vector<hipEvent_t> lat_ev;  // timing pool for bsg_sweep_timed
+  bsg::Accum acc;                  // posterior accumulators (bsg_accum_reset)
+  std::vector<void *> acc_allocs;
+  int acc_count = 0, acc_batch_len = 1;
+
+  void free_accum()
+  {
+    for (void *p : acc_allocs) hipFree(p);
+    acc_allocs.clear();
+    acc = bsg::Accum{};
+    acc_count = 0;
+  }
 
   void release()
   {
     if (stream) hipStreamSynchronize(stream);
     if (stream2) hipStreamSynchronize(stream2);
+    free_accum();
     for (void *p : allocs) hipFree(p);
     allocs.clear();
     for (auto &e : ev) {
@@ -368,7 +381,19 @@ int bsg_update_car_prec(bsg_ctx *ctx)
 // Wishart, side stream) only has to finish before the colour-0 update of
 // sweep t+1, so it overlaps the latent draw of sweep t+1 on the main stream.
 // lat_ev (2 per sweep, optional) brackets each latent kernel for timing.
-static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev)
+static int accum_launch(bsg_ctx *ctx)
+{
+  if (!ctx->acc.sum) {
+    set_error("posterior accumulation: call bsg_accum_reset first");
+    return BSG_ERR_ARG;
+  }
+  const int batch = std::min(ctx->acc_count / ctx->acc_batch_len, ctx->acc.nbatches - 1);
+  HIPCHK(bsg::launch_accum(ctx->pb, ctx->acc, batch, ctx->stream));
+  ctx->acc_count++;
+  return BSG_OK;
+}
+
+static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_every = 0)
 {
   hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
   const bool fused = ctx->opts.xtz_mode == 2;
@@ -396,6 +421,10 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev)
       HIPCHK(bsg::launch_wishart(ctx->pb, false, sw, s1));
     }
     ctx->sweep = sw + 1;
+    if (accum_every > 0 && (i + 1) % accum_every == 0) {
+      const int rc = accum_launch(ctx);  // reads only the coefficients: main stream
+      if (rc != BSG_OK) return rc;
+    }
   }
   if (ctx->wish_pending) {  // join: the caller's stream sees the finished sweep
     HIPCHK(hipStreamWaitEvent(s1, ctx->e_wish, 0));
@@ -443,6 +472,120 @@ int bsg_sweep_timed(bsg_ctx *ctx, int nsweeps, double *latent_ms, double *total_
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ev[2 * nsweeps], ev[2 * nsweeps + 1]));
     *total_ms = ms;
+  }
+  return BSG_OK;
+}
+
+int bsg_sweep_accumulate(bsg_ctx *ctx, int nsweeps, int save_every)
+{
+  CTXCHK(ctx);
+  if (nsweeps < 0 || save_every < 1) {
+    set_error("bsg_sweep_accumulate: bad sweep count or save interval");
+    return BSG_ERR_ARG;
+  }
+  if (!ctx->acc.sum) {
+    set_error("bsg_sweep_accumulate: call bsg_accum_reset first");
+    return BSG_ERR_ARG;
+  }
+  return sweep_run(ctx, nsweeps, nullptr, save_every);
+}
+
+int bsg_accum_reset(bsg_ctx *ctx, int ntypes, int nbatches, int batch_len)
+{
+  CTXCHK(ctx);
+  if (ntypes < 0 || ntypes > ctx->P || nbatches < 1 || nbatches > 1024 || batch_len < 1) {
+    set_error("bsg_accum_reset: need 0 <= ntypes <= P, 1 <= nbatches <= 1024, batch_len >= 1");
+    return BSG_ERR_ARG;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->free_accum();
+  const size_t CV = (size_t)ctx->C * ctx->L.totvox;
+  const int P = ctx->P, K = P + ntypes;
+  auto grab = [&](void **p, size_t bytes) -> int {
+    if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) {
+      set_error("bsg_accum_reset: hipMalloc failed");
+      return BSG_ERR_NOMEM;
+    }
+    ctx->acc_allocs.push_back(*p);
+    return hipMemsetAsync(*p, 0, bytes ? bytes : 16, ctx->stream) == hipSuccess ? BSG_OK : BSG_ERR_HIP;
+  };
+  bsg::Accum &a = ctx->acc;
+  int rc;
+  if ((rc = grab((void **)&a.sum, sizeof(double) * CV * P)) != BSG_OK ||
+      (rc = grab((void **)&a.sq, sizeof(double) * CV * P)) != BSG_OK ||
+      (rc = grab((void **)&a.prb, sizeof(double) * CV * ntypes)) != BSG_OK ||
+      (rc = grab((void **)&a.bwm, sizeof(double) * CV)) != BSG_OK ||
+      (rc = grab((void **)&a.batch, sizeof(float) * CV * nbatches * K)) != BSG_OK) {
+    ctx->free_accum();
+    return rc;
+  }
+  a.ntypes = ntypes;
+  a.nbatches = nbatches;
+  ctx->acc_batch_len = batch_len;
+  ctx->acc_count = 0;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BSG_OK;
+}
+
+int bsg_accum_add(bsg_ctx *ctx)
+{
+  CTXCHK(ctx);
+  return accum_launch(ctx);
+}
+
+int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *var, double *prb,
+                  double *bwm_mean, double *mcse)
+{
+  CTXCHK(ctx);
+  if (chain < 0 || chain >= ctx->C || !ctx->acc.sum) {
+    set_error("bsg_accum_get: bad chain or no accumulators");
+    return BSG_ERR_ARG;
+  }
+  const int V = ctx->L.totvox, P = ctx->P, T = ctx->acc.ntypes, K = P + T, NB = ctx->acc.nbatches;
+  const int n = ctx->acc_count;
+  if (ndraws) *ndraws = n;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const size_t off = (size_t)chain * V;
+  std::vector<double> sum((size_t)V * P), sq((size_t)V * P), pr((size_t)V * T), bw(V);
+  HIPCHK(hipMemcpy(sum.data(), ctx->acc.sum + off * P, sizeof(double) * sum.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sq.data(), ctx->acc.sq + off * P, sizeof(double) * sq.size(), hipMemcpyDeviceToHost));
+  if (T) HIPCHK(hipMemcpy(pr.data(), ctx->acc.prb + off * T, sizeof(double) * pr.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(bw.data(), ctx->acc.bwm + off, sizeof(double) * V, hipMemcpyDeviceToHost));
+  const double dn = n > 0 ? (double)n : 1.0;
+  for (int s = 0; s < V; s++) {
+    const int r = ctx->L.rank_of_sorted[s];
+    for (int c = 0; c < P; c++) {
+      const double m = sum[(size_t)s * P + c] / dn;
+      if (mean) mean[(size_t)c * V + r] = m;
+      // mcmc.cpp:878-881: (sum sq - n mean^2) / (n - 1)
+      if (var) var[(size_t)c * V + r] = n > 1 ? (sq[(size_t)s * P + c] - dn * m * m) / (dn - 1.0) : 0.0;
+    }
+    for (int t = 0; t < T; t++)
+      if (prb) prb[(size_t)t * V + r] = pr[(size_t)s * T + t] / dn;
+    if (bwm_mean) bwm_mean[r] = bw[s] / dn;
+  }
+  if (mcse) {
+    const int nb = std::min(n / ctx->acc_batch_len, NB);
+    std::vector<float> bat((size_t)NB * V * K);
+    HIPCHK(hipMemcpy(bat.data(), ctx->acc.batch + (size_t)chain * NB * V * K, sizeof(float) * bat.size(),
+                     hipMemcpyDeviceToHost));
+    for (int s = 0; s < V; s++) {
+      const int r = ctx->L.rank_of_sorted[s];
+      for (int k = 0; k < K; k++) {
+        double m1 = 0.0, m2 = 0.0;
+        for (int b = 0; b < nb; b++) {
+          const double x = bat[((size_t)b * V + s) * K + k] / ctx->acc_batch_len;
+          m1 += x;
+          m2 += x * x;
+        }
+        double e = 0.0;
+        if (nb > 1) {
+          m1 /= nb;
+          e = std::sqrt(std::max(0.0, (m2 - nb * m1 * m1) / (nb - 1)) / nb);
+        }
+        mcse[(size_t)k * V + r] = e;
+      }
+    }
   }
   return BSG_OK;
 }

@@ -61,6 +61,16 @@ struct DevProblem {
   uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
 };
 
+// Posterior accumulators (device, per chain): see k_accum.
+struct Accum {
+  int ntypes = 0, nbatches = 0;
+  double *sum = nullptr;   // [chain][V][P]
+  double *sq = nullptr;    // [chain][V][P]
+  double *prb = nullptr;   // [chain][V][ntypes]
+  double *bwm = nullptr;   // [chain][V]
+  float *batch = nullptr;  // [chain][nbatches][V][P + ntypes]
+};
+
 // kernel launchers (bsg_kernels.hip)
 hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, uint32_t sweep, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st);
@@ -68,6 +78,7 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st);
 
 void set_error(const std::string &msg);
 

@@ -24,6 +24,11 @@ namespace bsg {
 // Philox4x32-10 and uniforms. The 32x32->64 products are written as 64-bit
 // multiplies so each becomes one v_mad_u64_u32 (high and low halves at once).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);  // v_bitop3_b32, truth table of a^b^c
+}
+
 __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                          uint32_t k0, uint32_t k1, uint32_t r[4])
 {
@@ -31,9 +36,9 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, 
   for (int i = 0; i < 10; i++) {
     const uint64_t p0 = (uint64_t)c0 * 0xD2511F53u;
     const uint64_t p1 = (uint64_t)c2 * 0xCD9E8D57u;
-    c0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    c0 = xor3((uint32_t)(p1 >> 32), c1, k0);
     c1 = (uint32_t)p1;
-    c2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c2 = xor3((uint32_t)(p0 >> 32), c3, k1);
     c3 = (uint32_t)p0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
@@ -44,10 +49,11 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, 
   r[3] = c3;
 }
 
-// odd multiples of 2^-24: exact in float and double, never 0 or 1
+// odd multiples of 2^-24: exact in float and double, never 0 or 1.
+// (1 + k 2^-23) - (1 - 2^-24) = (2k + 1) 2^-24 exactly (Sterbenz), k = x >> 9.
 __device__ __forceinline__ float u01f(uint32_t x)
 {
-  return (float)(((x >> 9) << 1) | 1u) * 5.9604644775390625e-8f;
+  return __uint_as_float(0x3F800000u | (x >> 9)) - 0.99999994039535522f;
 }
 __device__ __forceinline__ double u01d(uint32_t x)
 {
@@ -56,41 +62,40 @@ __device__ __forceinline__ double u01d(uint32_t x)
 
 // ---------------------------------------------------------------------------
 // Normal tail function and its inverse in FP32, branch free. Coefficients are
-// this project's own fits (bsglmm_amd/csrc/fit_normal_tail.py):
-//   Q(t)    = r exp(-t^2/2 + g(r)), r = 1/(1 + t/(2 sqrt 2)), g of degree 9
-//   Qinv(p) = n(s)/d(s), s = sqrt(-2 ln p), n, d of degree 4, p in (1e-38, 1/2]
+// this project's own fits (bsglmm_amd/csrc/fit_normal_tail.py, kernel form):
+//   Q(t)    = r 2^(G(r) - t^2 log2(e)/2), r = 1/(1 + t/(2 sqrt 2)),
+//             G = log2(e) g with g of degree 9
+//   Qinv(p) = n(s)/d(s), s = sqrt(-log2 p), n, d of degree 4, p in (1e-38, 1/2]
 // FP32 errors: Q 3e-6 relative (t <= 8), Qinv 5e-7 relative (floor 1).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float q_tail(float t)
 {
   const float r = __builtin_amdgcn_rcpf(fmaf(0.35355339059327373f, t, 1.0f));
-  float g = 0.17109377682209015f;
-  g = fmaf(g, r, -0.8232021331787109f);
-  g = fmaf(g, r, 1.4906160831451416f);
-  g = fmaf(g, r, -1.1375014781951904f);
-  g = fmaf(g, r, 0.2803618311882019f);
-  g = fmaf(g, r, -0.1868750900030136f);
-  g = fmaf(g, r, 0.0969197228550911f);
-  g = fmaf(g, r, 0.3740749955177307f);
-  g = fmaf(g, r, 1.000024676322937f);
-  g = fmaf(g, r, -1.9586594104766846f);
-  const float e = fmaf(-0.5f * t, t, g);
-  return r * __builtin_amdgcn_exp2f(e * 1.4426950408889634f);
+  float g = 0.24683615565299988f;
+  g = fmaf(g, r, -1.1876296997070312f);
+  g = fmaf(g, r, 2.1505043506622314f);
+  g = fmaf(g, r, -1.6410677433013916f);
+  g = fmaf(g, r, 0.4044766128063202f);
+  g = fmaf(g, r, -0.26960375905036926f);
+  g = fmaf(g, r, 0.13982561230659485f);
+  g = fmaf(g, r, 0.5396761298179626f);
+  g = fmaf(g, r, 1.4427305459976196f);
+  g = fmaf(g, r, -2.8257482051849365f);
+  return r * __builtin_amdgcn_exp2f(fmaf(-0.72134752044448170f * t, t, g));
 }
 
 __device__ __forceinline__ float q_inv(float p)
 {
-  p = fminf(p, 0.5f);
-  const float s = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(p));
-  float n = 0.5416675806045532f;
-  n = fmaf(n, s, 4.245328426361084f);
-  n = fmaf(n, s, 3.2832629680633545f);
-  n = fmaf(n, s, -7.911550045013428f);
+  const float s = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(p));
+  float n = 1.0409833192825317f;
+  n = fmaf(n, s, 6.929381370544434f);
+  n = fmaf(n, s, 4.551568984985352f);
+  n = fmaf(n, s, -9.31513786315918f);
   n = fmaf(n, s, -3.2067947387695312f);
-  float d = 7.4306449278083164e-06f;
-  d = fmaf(d, s, 0.5411052107810974f);
-  d = fmaf(d, s, 4.2728986740112305f);
-  d = fmaf(d, s, 5.463263034820557f);
+  float d = 1.4280302821134683e-05f;
+  d = fmaf(d, s, 0.8832118511199951f);
+  d = fmaf(d, s, 5.923495769500732f);
+  d = fmaf(d, s, 6.432500839233398f);
   d = fmaf(d, s, 1.0f);
   return n * __builtin_amdgcn_rcpf(d);
 }
@@ -124,18 +129,39 @@ __device__ __noinline__ float tail_w(float a, float u)
   return x;
 }
 
-__device__ __forceinline__ float latent_draw(float mu, uint32_t y, float u, float eps)
+// Four draws of one Philox block: subject l uses uniform l and bit l of yq.
+__device__ __forceinline__ void latent_quad(const float (&mu)[4], uint32_t yq, const uint32_t (&rr)[4],
+                                            float eps, float (&z)[4])
 {
-  const float s = y ? 1.0f : -1.0f;
-  const float a = fmaf(-s, mu, eps);
-  if (a >= 8.0f) return fmaf(s, fmaxf(tail_w(a, u), a), mu);
-  const float qt = q_tail(fabsf(a));  // Q(|a|) <= 1/2
-  const float p = (a >= 0.0f) ? u * qt : fmaf(-u, qt, u);
-  const bool flip = p > 0.5f;
-  const float arg = flip ? fmaf(u, qt, 1.0f - u) : p;
-  float w = q_inv(arg);
-  w = flip ? -w : w;
-  return fmaf(s, fmaxf(w, a), mu);
+  float a[4], uu[4];
+  bool tail = false;
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const float u = u01f(rr[l]);
+    const float s = ((yq >> l) & 1u) ? 1.0f : -1.0f;
+    const float al = fmaf(-s, mu[l], eps);
+    const float qt = q_tail(fabsf(al));  // Q(|a|) <= 1/2
+    const float uq = u * qt;
+    const float pn = u - uq;             // u Q(a) when a < 0
+    const bool neg = al < 0.0f;
+    const bool flip = neg && pn > 0.5f;
+    const float arg = neg ? (flip ? (1.0f - u) + uq : pn) : uq;
+    float w = q_inv(arg);
+    w = flip ? -w : w;
+    w = (w < al) ? al : w;
+    z[l] = fmaf(s, w, mu[l]);
+    a[l] = al;
+    uu[l] = u;
+    tail |= al >= 8.0f;
+  }
+  if (tail) {
+#pragma unroll 1
+    for (int l = 0; l < 4; l++)
+      if (a[l] >= 8.0f) {
+        const float s = ((yq >> l) & 1u) ? 1.0f : -1.0f;
+        z[l] = fmaf(s, fmaxf(tail_w(a[l], uu[l]), a[l]), mu[l]);
+      }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -174,6 +200,21 @@ __device__ __forceinline__ int red_lane(int c)
   const int row = (c & 1) + 2 * (c >> 2);
   const int half = (c >> 1) & 1;
   return 16 * row + 8 * half;
+}
+
+// inverse of red_lane: the covariate whose total lane `lane` writes, or -1
+template <int P>
+__device__ __forceinline__ int red_cov(int lane)
+{
+  if ((lane & (red_pad<P>() == 4 ? 15 : 7)) != 0) return -1;
+  int c;
+  if constexpr (red_pad<P>() == 4) {
+    c = lane >> 4;
+  } else {
+    const int row = lane >> 4, half = (lane >> 3) & 1;
+    c = (row & 1) + 2 * half + 4 * (row >> 1);
+  }
+  return c < P ? c : -1;
 }
 
 template <int P>
@@ -275,9 +316,11 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
         philox10((uint32_t)s * (uint32_t)pb.NQ + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain,
                  pb.key0, pb.key1, rr);
         float z[4];
+        latent_quad(mu, yw, rr, eps, z);
+        if (i0 + 3 >= pb.N) {  // last quad of a row: zero the padding subjects
 #pragma unroll
-        for (int l = 0; l < 4; l++)
-          z[l] = (i0 + l < pb.N) ? latent_draw(mu[l], (yw >> l) & 1u, u01f(rr[l]), eps) : 0.0f;
+          for (int l = 1; l < 4; l++) z[l] = (i0 + l < pb.N) ? z[l] : 0.0f;
+        }
         *reinterpret_cast<float4 *>(zrow + i0) = make_float4(z[0], z[1], z[2], z[3]);
         if (FUSED) {
 #pragma unroll
@@ -293,10 +336,8 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
     if (FUSED) {
       if (lshift == 6) {
         const float tot = wave_reduce_scatter<P>(acc, lane);
-        float *m = pb.M + ((size_t)chain * pb.V + s) * P;
-#pragma unroll
-        for (int c = 0; c < P; c++)
-          if (lane == red_lane<P>(c)) m[c] = fmaf(-bw, pb.xsum[c], tot);
+        const int c = red_cov<P>(lane);
+        if (c >= 0) pb.M[((size_t)chain * pb.V + s) * P + c] = fmaf(-bw, pb.xsum[c], tot);
       } else {
         for (int off = (1 << lshift) >> 1; off > 0; off >>= 1) {
 #pragma unroll
@@ -789,6 +830,49 @@ __global__ __launch_bounds__(256) void k_pairs(DevProblem pb)
 }
 
 // ---------------------------------------------------------------------------
+// k_accum: posterior accumulation of one saved draw (mcmc.cpp:627-649), on the
+// device. FP64 running sums of b and b^2 per voxel and coefficient, of the
+// lesion probability Phi(b_t + b WM) for the first `ntypes` coefficients
+// (compute_prbGPU, covarGPU.cu:1723-1747, 1814-1824), and of b WM; FP32 sums
+// per batch of draws for batch-means Monte-Carlo errors.
+// Phi is Abramowitz & Stegun 26.2.17 (|error| < 7.5e-8), as the reference's
+// ProbSDNorm (covar.cpp:327-348).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double phi_as(double x)
+{
+  const double t = 1.0 / (1.0 + 0.2316419 * fabs(x));
+  const double poly = t * (0.319381530 + t * (-0.356563782 + t * (1.781477937 + t * (-1.821255978 + t * 1.330274429))));
+  const double tail = 0.39894228 * exp(-0.5 * x * x) * poly;
+  return x >= 0.0 ? 1.0 - tail : tail;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_accum(DevProblem pb, Accum ac, int batch)
+{
+  const int chain = blockIdx.y;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= pb.V) return;
+  const float *b = pb.beta + ((size_t)chain * pb.V + s) * P;
+  const double bw = (double)pb.bwm[chain] * (double)pb.wm[s];
+  const size_t base = (size_t)chain * pb.V + s;
+  const int K = P + ac.ntypes;
+  float *bat = ac.batch + (((size_t)chain * ac.nbatches + batch) * pb.V + s) * K;
+#pragma unroll
+  for (int c = 0; c < P; c++) {
+    const double v = (double)b[c];
+    ac.sum[base * P + c] += v;
+    ac.sq[base * P + c] += v * v;
+    bat[c] += b[c];
+  }
+  for (int t = 0; t < ac.ntypes && t < P; t++) {
+    const double pr = phi_as((double)b[t] + bw);
+    ac.prb[base * ac.ntypes + t] += pr;
+    bat[P + t] += (float)pr;
+  }
+  ac.bwm[base] += bw;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
@@ -900,6 +984,15 @@ hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipSt
     else                                                                         \
       hipLaunchKernelGGL((k_wishart<PP, false>), grid, dim3(256), 0, st, pb, sweep);    \
   } while (0)
+  BSG_DISPATCH_P(pb.P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st)
+{
+  dim3 grid((unsigned)((pb.V + 255) / 256), (unsigned)pb.nchains);
+#define L_(PP) hipLaunchKernelGGL(k_accum<PP>, grid, dim3(256), 0, st, pb, ac, batch)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
   return hipGetLastError();

@@ -8,7 +8,8 @@
             fitted by Lawson-reweighted nonlinear least squares on the error
             (approx - exact) / max(1, exact).
 
-Run: python bsglmm_amd/csrc/fit_normal_tail.py   (prints the coefficient lists)
+Run: python bsglmm_amd/csrc/fit_normal_tail.py   (prints the coefficient lists,
+then the same constants rescaled as the kernel uses them)
 Max errors in FP32 evaluation: Q 3e-6 relative (from rounding t^2/2 near t = 8;
 the fitted function itself is 1e-7), Qinv 5e-7 relative-with-floor-1.
 """
@@ -54,9 +55,25 @@ def fit_qinv(m=4, n=4, pmin=1e-38, npts=20000):
     return c
 
 
+def kernel_constants():
+    """The constants as the kernel uses them: g scaled by log2(e) (the kernel
+    evaluates 2^(g log2 e - t^2 log2(e)/2) with v_exp_f32), and the rational in
+    s2 = sqrt(-log2 p) = s / sqrt(2 ln 2) (v_log_f32 is log2)."""
+    q = fit_q() * np.log2(np.e)
+    qi = fit_qinv()
+    k = np.sqrt(2.0 * np.log(2.0))
+    num = qi[:5] * k ** np.arange(5)
+    den = np.concatenate([[1.0], qi[5:]]) * k ** np.arange(5)
+    return q, num, den
+
+
 if __name__ == "__main__":
     q = fit_q()
     qi = fit_qinv()
     print("Q coefficients (r^0 .. r^9):", ", ".join(f"{float(np.float32(v))!r}f" for v in q))
     print("Qinv numerator (s^0 .. s^4):", ", ".join(f"{float(np.float32(v))!r}f" for v in qi[:5]))
     print("Qinv denominator (s^1 .. s^4):", ", ".join(f"{float(np.float32(v))!r}f" for v in qi[5:]))
+    gq, num, den = kernel_constants()
+    print("kernel: g*log2e (r^0 .. r^9):", ", ".join(f"{float(np.float32(v))!r}f" for v in gq))
+    print("kernel: numerator in sqrt(-log2 p) (s^0 .. s^4):", ", ".join(f"{float(np.float32(v))!r}f" for v in num))
+    print("kernel: denominator (s^0 .. s^4):", ", ".join(f"{float(np.float32(v))!r}f" for v in den))

@@ -108,6 +108,24 @@ int bsg_sweep_timed(bsg_ctx *ctx, int nsweeps, double *latent_ms, double *total_
  * after the stream drained. */
 int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6]);
 
+/* ---- posterior accumulation on the device (mcmc.cpp:584-651) ----------- */
+/* (Re)allocates and zeroes the accumulators of every chain: FP64 sums of
+ * beta and beta^2, of the lesion probability Phi(beta_t + b WM) for the first
+ * ntypes coefficients (compute_prbGPU, covarGPU.cu:1814-1824; Phi by
+ * A&S 26.2.17 as covar.cpp:327-348) and of b WM; FP32 sums over nbatches
+ * batches of batch_len draws for batch-means Monte-Carlo errors. */
+int bsg_accum_reset(bsg_ctx *ctx, int ntypes, int nbatches, int batch_len);
+/* adds the current draw of every chain */
+int bsg_accum_add(bsg_ctx *ctx);
+/* nsweeps sweeps, adding every save_every-th draw (mcmc.cpp:584: SAVE_ITER) */
+int bsg_sweep_accumulate(bsg_ctx *ctx, int nsweeps, int save_every);
+/* Posterior summaries of one chain in mask-rank order, any output may be NULL:
+ * mean, var [P][TOTVOX] (variance with the n-1 denominator, mcmc.cpp:871-882),
+ * prb [ntypes][TOTVOX], bwm_mean [TOTVOX], mcse [P+ntypes][TOTVOX] (batch
+ * means over the completed batches). ndraws receives the number of draws. */
+int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *var, double *prb,
+                  double *bwm_mean, double *mcse);
+
 /* ---- state exchange in the reference layouts (chain index first) ------ */
 int bsg_get_spat_coef(bsg_ctx *ctx, int chain, float *spat_coef);
 int bsg_set_spat_coef(bsg_ctx *ctx, int chain, const float *spat_coef);

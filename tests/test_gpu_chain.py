@@ -58,7 +58,7 @@ def test_accumulator_matches_host_snapshots():
         assert np.allclose(out["var"], b.var(0, ddof=1), rtol=1e-4, atol=1e-7)
         prb = phi_as(b[:, :2])
         assert np.allclose(out["prb"], prb.mean(0), rtol=1e-6, atol=1e-7)
-        bm = np.concatenate([b, prb], axis=1).reshape(4, 6, 6, -1).mean(1)
+        bm = np.concatenate([b, prb], axis=1).reshape(4, 6, b.shape[1] + 2, -1).mean(1)
         mcse = bm.std(0, ddof=1) / 2.0
         assert np.allclose(out["mcse"], mcse, rtol=1e-3, atol=1e-6)
     # sweep_accumulate adds every save_every-th draw

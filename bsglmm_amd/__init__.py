@@ -80,6 +80,7 @@ def lib():
         "bsg_set_sweep_index": (C.c_int, [vp, C.c_uint32]),
         "bsg_get_fallback_count": (C.c_int, [vp, C.POINTER(C.c_uint32)]),
         "bsg_measure_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, pd, pd]),
+        "bsg_measure_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, pd, pd]),
         "bsg_kiss": (C.c_double, [C.POINTER(C.c_ulong)]),
         "bsg_devseed": (C.c_uint64, [C.POINTER(C.c_ulong)]),
     }
@@ -289,6 +290,14 @@ def copy_bandwidth(nbytes=1 << 30, reps=20):
     """(GB/s, ms per copy) of a float4 stream copy over nbytes (read + write counted)."""
     g, ms = C.c_double(), C.c_double()
     _check(lib().bsg_measure_copy_bandwidth(C.c_size_t(nbytes), int(reps), C.byref(g), C.byref(ms)))
+    return g.value, ms.value
+
+
+def stream_bandwidth(kind, nbytes=1 << 30, reps=20):
+    """(GB/s, ms per launch) of a stream kernel: kind 'copy', 'write' or 'read'."""
+    k = {"copy": 0, "write": 1, "read": 2}[kind]
+    g, ms = C.c_double(), C.c_double()
+    _check(lib().bsg_measure_bandwidth(k, C.c_size_t(nbytes), int(reps), C.byref(g), C.byref(ms)))
     return g.value, ms.value
 
 

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsglmm.so")
-SOURCES = ["bsg_layout.cpp", "bsg_ctx.cpp", "bsg_kernels.hip"]
+SOURCES = ["bsg_layout.cpp", "bsg_ctx.cpp", "bsg_kernels.hip", "bsg_probe.hip"]
 
 # Packed FP32 instructions are disabled for all device code: some of their
 # forms have given wrong results on these machines.

@@ -773,10 +773,10 @@ int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count)
   return BSG_OK;
 }
 
-int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_per_copy)
+int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double *ms_per_launch)
 {
-  if (!gbps || bytes < 16 || reps < 1) {
-    set_error("bsg_measure_copy_bandwidth: bad argument");
+  if (!gbps || bytes < 16 || reps < 1 || kind < 0 || kind > 2) {
+    set_error("bsg_measure_bandwidth: bad argument");
     return BSG_ERR_ARG;
   }
   bytes &= ~(size_t)15;
@@ -786,26 +786,27 @@ int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_
   int rc = BSG_OK;
   float ms = 0.0f;
   if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) {
-    set_error("bsg_measure_copy_bandwidth: hipMalloc failed");
+    set_error("bsg_measure_bandwidth: hipMalloc failed");
     rc = BSG_ERR_NOMEM;
   } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
              hipMemsetAsync(src, 0, bytes, st) != hipSuccess ||
-             bsg::launch_copy(src, dst, bytes, st) != hipSuccess ||
+             bsg::launch_probe(kind, src, dst, bytes, st) != hipSuccess ||
              hipEventRecord(e0, st) != hipSuccess) {
-    set_error("bsg_measure_copy_bandwidth: HIP failure");
+    set_error("bsg_measure_bandwidth: HIP failure");
     rc = BSG_ERR_HIP;
   } else {
     for (int i = 0; i < reps && rc == BSG_OK; i++)
-      if (bsg::launch_copy(src, dst, bytes, st) != hipSuccess) rc = BSG_ERR_HIP;
+      if (bsg::launch_probe(kind, src, dst, bytes, st) != hipSuccess) rc = BSG_ERR_HIP;
     if (rc == BSG_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
       rc = BSG_ERR_HIP;
-    if (rc != BSG_OK) set_error("bsg_measure_copy_bandwidth: HIP failure");
+    if (rc != BSG_OK) set_error("bsg_measure_bandwidth: HIP failure");
   }
   if (rc == BSG_OK) {
-    *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
-    if (ms_per_copy) *ms_per_copy = ms / reps;
+    const double moved = (kind == 0 ? 2.0 : 1.0) * (double)bytes;
+    *gbps = moved * reps / (ms * 1e-3) / 1e9;
+    if (ms_per_launch) *ms_per_launch = ms / reps;
   }
   if (st) hipStreamSynchronize(st);
   if (e0) hipEventDestroy(e0);
@@ -814,6 +815,11 @@ int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_
   if (src) hipFree(src);
   if (dst) hipFree(dst);
   return rc;
+}
+
+int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_per_copy)
+{
+  return bsg_measure_bandwidth(0, bytes, reps, gbps, ms_per_copy);
 }
 
 // ---- host KISS (randgen.cpp:9-23, 36-45; main.cu:54) -------------------

@@ -78,6 +78,7 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+hipError_t launch_probe(int kind, const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st);
 
 void set_error(const std::string &msg);

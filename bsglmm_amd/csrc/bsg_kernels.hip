@@ -412,28 +412,87 @@ __global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int 
 }
 
 // ---------------------------------------------------------------------------
-// Copy probe: a plain float4 stream copy, used to calibrate bandwidth figures.
+// k_xtz_mfma: X'Z on the matrix cores, v_mfma_f32_16x16x32_bf16. Z and X are
+// split into hi + lo BF16 parts (hi = bf16(v), lo = bf16(v - hi); |v - hi - lo|
+// <= 2^-18 |v|); FP32 accumulation. A wave owns 16 voxels (the MFMA rows) and
+// walks the subjects 32 at a time (K). B holds [X_hi | X_lo] in its first 2P
+// columns, so D = (Z_hi + Z_lo)[X_hi | X_lo] from two MFMAs per block and
+// m_j,c = D[j][c] + D[j][P + c]. A and B use the same (lane, slot) -> k map,
+// so the contraction is right whatever order the hardware sums k in; the
+// output layout is lane l -> D[4 (l / 16) + i][l % 16].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo)
 {
-  const size_t stride = (size_t)gridDim.x * 256;
-  size_t i = blockIdx.x * 256 + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const __bf16 h = (__bf16)v[j];
+    hi[j] = h;
+    lo[j] = (__bf16)(v[j] - (float)h);
   }
-  for (; i < n; i += stride) dst[i] = src[i];
 }
 
-hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+// eight consecutive floats of a row of length NS (a multiple of 4) from
+// element k; float4 halves past the row end read as zero
+__device__ __forceinline__ void load8(const float *row, int k, int NS, float (&v)[8])
 {
-  const size_t n = bytes / 16;
-  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, static_cast<const float4 *>(src),
-                     static_cast<float4 *>(dst), n);
-  return hipGetLastError();
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+  if (k < NS) a = *reinterpret_cast<const float4 *>(row + k);
+  if (k + 4 < NS) b = *reinterpret_cast<const float4 *>(row + k + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
+{
+  static_assert(2 * P <= 16, "hi/lo columns must fit 16");
+  const int chain = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int s0 = tile * 16;
+  if (s0 >= pb.V) return;  // wave-uniform
+  const int r = lane & 15, g = lane >> 4;
+  const int srow = min(s0 + r, pb.V - 1);
+  const float *zrow = pb.Z + ((size_t)chain * pb.V + srow) * pb.NS;
+  const int col = r;  // B column of this lane
+  const float *xrow = pb.X + (size_t)(col < P ? col : (col < 2 * P ? col - P : 0)) * pb.NS;
+  const bool xlo = col >= P && col < 2 * P;
+  const bool xon = col < 2 * P;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < pb.NS; k0 += 32) {
+    const int k = k0 + 8 * g;
+    float zv[8], xv[8];
+    load8(zrow, k, pb.NS, zv);
+    bf16x8 zh, zl;
+    split8(zv, zh, zl);
+    bf16x8 xb;
+    if (xon) {
+      load8(xrow, k, pb.NS, xv);
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const __bf16 h = (__bf16)xv[j];
+        xb[j] = xlo ? (__bf16)(xv[j] - (float)h) : h;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j++) xb[j] = (__bf16)0.0f;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zh, xb, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zl, xb, acc, 0, 0, 0);
+  }
+  const float b = pb.bwm[chain];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float other = __shfl_down(acc[i], P, 16);  // D[row][col + P]
+    const int s = s0 + 4 * g + i;
+    if (col < P && s < pb.V) {
+      const float bw = b * pb.wm[s];
+      pb.M[((size_t)chain * pb.V + s) * P + col] = fmaf(-bw, pb.xsum[col], acc[i] + other);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -928,7 +987,14 @@ hipError_t launch_latent(const DevProblem &pb, bool fused, uint32_t sweep, hipSt
 
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st)
 {
-  (void)mode;
+  if (mode == 1) {
+    const long long tiles = ((long long)pb.V + 15) / 16;
+    dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)pb.nchains);
+#define M_(PP) hipLaunchKernelGGL(k_xtz_mfma<PP>, grid, dim3(256), 0, st, pb)
+    BSG_DISPATCH_P(pb.P, M_)
+#undef M_
+    return hipGetLastError();
+  }
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
   const int vpw = 64 >> lshift;

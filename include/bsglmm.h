@@ -143,6 +143,8 @@ int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count);     /* FP64 Cholesky 
 /* Times reps launches of a float4 stream-copy kernel over `bytes`; gbps is
  * (read + written bytes) / time. Calibrates bandwidth and counter figures. */
 int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_per_copy);
+/* kind 0: copy (read + write counted), 1: write-only stream, 2: read-only stream */
+int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double *ms_per_launch);
 
 /* ---- host KISS, for seed.dat (randgen.cpp:9-23, 36-45; main.cu:54, 413) */
 double bsg_kiss(unsigned long seed[3]);

@@ -43,7 +43,7 @@ def rel_err(a, b, floor=1.0):
     return np.max(np.abs(a - b) / np.maximum(floor, np.abs(b)))
 
 
-@pytest.mark.parametrize("xtz_mode", [0, 2])
+@pytest.mark.parametrize("xtz_mode", [0, 1, 2])
 @pytest.mark.parametrize("name,nsubs,ncovar", [("c1", None, None), ("c1", 37, 8), ("c1", 300, 4),
                                                ("c1", 5, 2), ("c1", 200, 8), ("c1", 160, 6),
                                                ("c1", 140, 1)])
@@ -71,7 +71,9 @@ def test_one_sweep_every_update_matches_oracle(name, nsubs, ncovar, xtz_mode):
     m_gpu = S.get_xtz()
     m_orc = O.xtz(z_gpu)
     scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_gpu.astype(np.float64))
-    assert np.max(np.abs(m_gpu - m_orc) / scale) < 1e-5
+    # FP32 FMA: 1e-5 of sum |x||z|; BF16 hi+lo on MFMA (xtz_mode 1): 2e-5, from
+    # the 2^-18 split residual of both operands plus FP32 accumulation
+    assert np.max(np.abs(m_gpu - m_orc) / scale) < (2e-5 if xtz_mode == 1 else 1e-5)
 
     # coefficient updates, colour 0 then colour 1, on the same M
     sc = sc0.astype(np.float64)
@@ -96,7 +98,7 @@ def test_one_sweep_every_update_matches_oracle(name, nsubs, ncovar, xtz_mode):
     S.synchronize()
 
 
-@pytest.mark.parametrize("xtz_mode", [0, 2])
+@pytest.mark.parametrize("xtz_mode", [0, 1, 2])
 @pytest.mark.parametrize("nsubs,ncovar", [(None, None), (200, 8), (60, 5)])
 def test_full_sweep_matches_oracle(nsubs, ncovar, xtz_mode):
     """One bsg_sweep: the colour-1 kernel also forms the pair sums and draws Lambda."""
@@ -113,7 +115,7 @@ def test_full_sweep_matches_oracle(nsubs, ncovar, xtz_mode):
     z_gpu, m_gpu, b_gpu = S.get_z(), S.get_xtz(), S.get_spat_coef()
     assert rel_err(z_gpu, O.update_z(sc0, sweep)) < 1e-5
     scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_gpu.astype(np.float64))
-    assert np.max(np.abs(m_gpu - O.xtz(z_gpu)) / scale) < 1e-5
+    assert np.max(np.abs(m_gpu - O.xtz(z_gpu)) / scale) < (2e-5 if xtz_mode == 1 else 1e-5)
     ref = O.update_coef(0, m_gpu.astype(np.float64), prec0, sc0.astype(np.float64), sweep)
     ref = O.update_coef(1, m_gpu.astype(np.float64), prec0, ref, sweep)
     assert rel_err(b_gpu, ref) < 1e-5

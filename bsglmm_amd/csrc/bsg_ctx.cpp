@@ -161,6 +161,7 @@ void bsg_opts_default(bsg_opts *o)
   o->df_offset = 19;
   o->trunc_eps = 1e-6f;
   o->pivot_min = 1e-6f;
+  o->z_store = 0;
 }
 
 int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *covar,
@@ -192,6 +193,10 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   }
   if (o.xtz_mode < 0 || o.xtz_mode > 2) {
     set_error("bsg_create: xtz_mode must be 0, 1 or 2");
+    return BSG_ERR_ARG;
+  }
+  if (o.z_store < 0 || o.z_store > 2 || (o.z_store == 2 && o.xtz_mode != 2)) {
+    set_error("bsg_create: z_store must be 0, 1, or 2 with xtz_mode 2");
     return BSG_ERR_ARG;
   }
   int ndev = 0;
@@ -281,7 +286,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(xtxd, (size_t)P * P);
   ALLOC(xsumd, (size_t)P);
   ALLOC(wmd, (size_t)V);
-  ALLOC(pb.Z, (size_t)C * V * NS);
+  ALLOC(pb.Z, o.z_store == 2 ? (size_t)1 : (size_t)C * V * NS);
   ALLOC(pb.beta, (size_t)C * V * P);
   ALLOC(pb.M, (size_t)C * V * P);
   ALLOC(pb.lam, (size_t)C * P * P);
@@ -308,7 +313,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemcpyAsync(xtxd, xtx.data(), sizeof(float) * xtx.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(xsumd, xsum.data(), sizeof(float) * P, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(wmd, wms.data(), sizeof(float) * V, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(pb.Z, 0, sizeof(float) * (size_t)C * V * NS, st));
+  if (o.z_store != 2) HIPCHK(hipMemsetAsync(pb.Z, 0, sizeof(float) * (size_t)C * V * NS, st));
   HIPCHK(hipMemsetAsync(pb.beta, 0, sizeof(float) * (size_t)C * V * P, st));
   HIPCHK(hipMemsetAsync(pb.M, 0, sizeof(float) * (size_t)C * V * P, st));
   HIPCHK(hipMemsetAsync(pb.tsum, 0, sizeof(double) * (size_t)C * P * P, st));
@@ -346,7 +351,7 @@ int bsg_destroy(bsg_ctx *ctx)
 int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->sweep, ctx->stream));
+  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream));
   return BSG_OK;
 }
 
@@ -401,7 +406,7 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
   for (int i = 0; i < nsweeps; i++) {
     const uint32_t sw = ctx->sweep;
     if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i], s1));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, sw, s1));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, s1));
     if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i + 1], s1));
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, s1));
     if (ctx->wish_pending) {
@@ -618,7 +623,7 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
     const bool fused = ctx->opts.xtz_mode == 2;
     const uint32_t sw = ctx->sweep;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, sw, st));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
@@ -680,6 +685,10 @@ int bsg_set_spat_coef(bsg_ctx *ctx, int chain, const float *spat_coef)
 int bsg_get_z(bsg_ctx *ctx, int chain, float *z)
 {
   CHAINCHK(ctx, chain);
+  if (ctx->opts.z_store == 2) {
+    set_error("bsg_get_z: this context does not keep Z (z_store 2)");
+    return BSG_ERR_ARG;
+  }
   const int V = ctx->L.totvox, N = ctx->N, NS = ctx->NS;
   std::vector<float> zz((size_t)V * NS);
   HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -72,7 +72,7 @@ struct Accum {
 };
 
 // kernel launchers (bsg_kernels.hip)
-hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, uint32_t sweep, hipStream_t st);
+hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, int zstore, uint32_t sweep, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st);
 hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);

@@ -262,9 +262,14 @@ __device__ __forceinline__ float wave_reduce_scatter(const float (&acc)[P], int 
 // registers (one quad per lane, N <= 4 L). FUSED also forms
 // m_j = X'(Z_j - b WM_j) from the fresh draws (no re-read of Z).
 // ---------------------------------------------------------------------------
-template <int P, bool XREG, bool FUSED>
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// ZS: 0 = store Z, 1 = store Z with nontemporal (streaming) stores,
+// 2 = do not keep Z (only with FUSED: X'Z is formed from the draws in registers)
+template <int P, bool XREG, bool FUSED, int ZS>
 __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps, int viter, uint32_t sweep)
 {
+  static_assert(ZS != 2 || FUSED, "Z can only be dropped when X'Z is fused");
   const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -279,17 +284,33 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
 #pragma unroll
     for (int c = 0; c < P; c++) xr[c] = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + i0);
   }
+  // per-voxel inputs of the next iteration are loaded one iteration ahead
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  auto voxel_of = [&](int it) { return (wave * viter + it) * vpw + sub; };
+  int sn = min(voxel_of(0), pb.V - 1);
+  float bs_n[P];
+#pragma unroll
+  for (int c = 0; c < P; c++) bs_n[c] = betc[(size_t)sn * P + c];
+  float wm_n = pb.wm[sn];
+  uint32_t y_n = XREG ? pb.ybits[(size_t)sn * pb.NW + (ql >> 3)] : 0u;
   for (int it = 0; it < viter; it++) {
     const int grp = wave * viter + it;
     if (grp * vpw >= pb.V) break;  // wave-uniform
     const int s = grp * vpw + sub;
     const bool valid = s < pb.V;
     const int sv = valid ? s : pb.V - 1;
-    const float *bet = pb.beta + ((size_t)chain * pb.V + sv) * P;
     float bs[P];
 #pragma unroll
-    for (int c = 0; c < P; c++) bs[c] = bet[c];
-    const float bw = b * pb.wm[sv];
+    for (int c = 0; c < P; c++) bs[c] = bs_n[c];
+    const float bw = b * wm_n;
+    const uint32_t y0 = y_n;
+    if (it + 1 < viter) {
+      sn = min(voxel_of(it + 1), pb.V - 1);
+#pragma unroll
+      for (int c = 0; c < P; c++) bs_n[c] = betc[(size_t)sn * P + c];
+      wm_n = pb.wm[sn];
+      if (XREG) y_n = pb.ybits[(size_t)sn * pb.NW + (ql >> 3)];
+    }
     float *zrow = pb.Z + ((size_t)chain * pb.V + sv) * pb.NS;
     const uint32_t *yrow = pb.ybits + (size_t)sv * pb.NW;
     float acc[P];
@@ -311,7 +332,7 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
           mu[2] = fmaf(bs[c], x[c].z, mu[2]);
           mu[3] = fmaf(bs[c], x[c].w, mu[3]);
         }
-        const uint32_t yw = yrow[q >> 3] >> ((q & 7) * 4);
+        const uint32_t yw = (XREG ? y0 : yrow[q >> 3]) >> ((q & 7) * 4);
         uint32_t rr[4];
         philox10((uint32_t)s * (uint32_t)pb.NQ + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain,
                  pb.key0, pb.key1, rr);
@@ -321,7 +342,12 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
 #pragma unroll
           for (int l = 1; l < 4; l++) z[l] = (i0 + l < pb.N) ? z[l] : 0.0f;
         }
-        *reinterpret_cast<float4 *>(zrow + i0) = make_float4(z[0], z[1], z[2], z[3]);
+        if (ZS == 0) {
+          *reinterpret_cast<float4 *>(zrow + i0) = make_float4(z[0], z[1], z[2], z[3]);
+        } else if (ZS == 1) {
+          const f4v zv = {z[0], z[1], z[2], z[3]};
+          __builtin_nontemporal_store(zv, reinterpret_cast<f4v *>(zrow + i0));
+        }
         if (FUSED) {
 #pragma unroll
           for (int c = 0; c < P; c++) {
@@ -959,7 +985,7 @@ static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
     default: return hipErrorInvalidValue; \
   }
 
-hipError_t launch_latent(const DevProblem &pb, bool fused, uint32_t sweep, hipStream_t st)
+hipError_t launch_latent(const DevProblem &pb, bool fused, int zstore, uint32_t sweep, hipStream_t st)
 {
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
@@ -969,19 +995,34 @@ hipError_t launch_latent(const DevProblem &pb, bool fused, uint32_t sweep, hipSt
   const long long waves = (groups + viter - 1) / viter;
   dim3 grid((unsigned)((waves + 3) / 4), (unsigned)pb.nchains);
   const bool xreg = reps == 1;
-#define L_(PP)                                                                                   \
-  do {                                                                                           \
-    if (xreg && fused)                                                                           \
-      hipLaunchKernelGGL((k_latent<PP, true, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep);   \
-    else if (xreg)                                                                               \
-      hipLaunchKernelGGL((k_latent<PP, true, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep);  \
-    else if (fused)                                                                              \
-      hipLaunchKernelGGL((k_latent<PP, false, true>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep);  \
-    else                                                                                         \
-      hipLaunchKernelGGL((k_latent<PP, false, false>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep); \
+  if (!fused && zstore == 2) return hipErrorInvalidValue;
+#define LK(PP, XR, FU, ZZ) \
+  hipLaunchKernelGGL((k_latent<PP, XR, FU, ZZ>), grid, dim3(256), 0, st, pb, lshift, reps, viter, sweep)
+#define L_(PP)                                                          \
+  do {                                                                  \
+    if (fused) {                                                        \
+      if (xreg) {                                                       \
+        if (zstore == 0) LK(PP, true, true, 0);                         \
+        else if (zstore == 1) LK(PP, true, true, 1);                    \
+        else LK(PP, true, true, 2);                                     \
+      } else {                                                          \
+        if (zstore == 0) LK(PP, false, true, 0);                        \
+        else if (zstore == 1) LK(PP, false, true, 1);                   \
+        else LK(PP, false, true, 2);                                    \
+      }                                                                 \
+    } else {                                                            \
+      if (xreg) {                                                       \
+        if (zstore == 0) LK(PP, true, false, 0);                        \
+        else LK(PP, true, false, 1);                                    \
+      } else {                                                          \
+        if (zstore == 0) LK(PP, false, false, 0);                       \
+        else LK(PP, false, false, 1);                                   \
+      }                                                                 \
+    }                                                                   \
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
+#undef LK
   return hipGetLastError();
 }
 

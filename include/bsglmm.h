@@ -69,6 +69,8 @@ typedef struct {
   int df_offset;        /* Wishart df = TOTVOX + df_offset: 19 as covarGPU.cu:1104, P as covar.cpp:291 */
   float trunc_eps;      /* latent truncation: 1e-6 as covarGPU.cu:1414-1418, 0 as covar.cpp:317,320 */
   float pivot_min;      /* FP32 Cholesky pivot below this -> FP64 refactor of that voxel */
+  int z_store;          /* Z in HBM: 0 stored, 1 stored with streaming (nontemporal) stores,
+                           2 not kept (xtz_mode 2 only; the probit sweep needs Z only for X'Z) */
 } bsg_opts;
 
 void bsg_opts_default(bsg_opts *opts);

@@ -134,6 +134,30 @@ def test_full_sweep_matches_oracle(nsubs, ncovar, xtz_mode):
     assert np.max(np.abs(t2 - O.pair_sums(S.get_spat_coef())) / np.abs(t2).max()) < 1e-12
 
 
+@pytest.mark.parametrize("z_store", [1, 2])
+def test_z_store_variants(z_store):
+    """Streaming stores (1) must give the same Z; without Z (2) the fused X'Z
+    must match X' times the oracle's own draws."""
+    print(f"case c1 z_store={z_store}", flush=True)
+    d, lay, L, S, O, sc0 = problem("c1", xtz_mode=2)
+    S2 = Sampler(L, d["covar"], d["data"], seed=SEED, xtz_mode=2, z_store=z_store)
+    for T in (S, S2):
+        T.set_spat_coef(sc0)
+        T.set_sweep_index(3)
+        T.update_latent()
+    if z_store == 1:
+        assert np.array_equal(S.get_z(), S2.get_z())
+        assert np.array_equal(S.get_xtz(), S2.get_xtz())
+    else:
+        z_orc = O.update_z(sc0, 3)
+        m_ref = O.xtz(z_orc.astype(np.float32))
+        scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_orc)
+        assert np.max(np.abs(S2.get_xtz() - m_ref) / scale) < 2e-5
+        assert np.array_equal(S.get_xtz(), S2.get_xtz())
+        with pytest.raises(Exception):
+            S2.get_z()
+
+
 def test_full_size_invariants_and_determinism():
     print("case c2 full size, 3 sweeps x 2 contexts", flush=True)
     d = synth.generate("c2")

@@ -7,12 +7,15 @@ model (bsglmm_amd.synth). A step is one full sweep: latent draws, X'Z,
 checkerboard coefficient updates (two colours), FP64 pair sums and the device
 Wishart draw. All inputs are resident in HBM before timing starts.
 
-The headline configuration forms X'Z inside the latent kernel (FP32 FMA);
-the same sweep with a separate X'Z kernel re-reading Z is timed beside it.
-Prints one JSON line with the dominant kernel's roofline (latent draws with
-fused X'Z: algorithmic bytes N*V*(4 + 1/8) + 8*P*V per launch against
-8 TB/s), a measured copy-kernel bandwidth, and the reference CPU path timed on
-this host for a bounded sample.
+Headline form: X'Z is formed inside the latent kernel from draws held in
+registers, and Z is not written (k_latent_vl; Z is a transient of the sweep).
+Beside it, in the same run: the fused kernel that also stores Z, and the
+sweep with a separate X'Z kernel re-reading Z (FP32 FMA, and BF16 hi+lo on
+MFMA). Prints one JSON line with the dominant kernel's roofline (its own
+algorithmic bytes against 8 TB/s, plus its VALU issue rate, the bound it
+actually meets), the sweep-level HBM fraction of the algorithmic sweep bytes
+N V (8 + 1/8) + V (24 P + 5), a measured copy-kernel bandwidth, and the
+reference CPU path timed on this host for a bounded sample.
 """
 import argparse
 import json
@@ -35,7 +38,8 @@ def parse():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-sweeps", type=int, default=2, help="timed reference CPU sweeps")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "latent_c2_traffic.json"))
+    ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "k_latent_vl_c2.json"),
+                    help="measured traffic and VALU counts of the dominant kernel (rocprofv3 summary)")
     return ap.parse_args()
 
 
@@ -83,22 +87,30 @@ def cpu_baseline(d, nsweeps):
             "latent_draws_per_s": N * V * nsweeps / dt}
 
 
-def time_mode(L, d, mode, steps, warmup):
+def time_mode(L, d, mode, zstore, steps, warmup):
     """Timed region: `steps` sweeps (CAR-precision draw overlapped with the next
-    latent draw), HIP events around every latent kernel on its stream. Then a
+    latent draw), HIP events around every latent launch on its stream. Then a
     short serial pass with events around every kernel for the breakdown."""
     from bsglmm_amd import Sampler
-    S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15, xtz_mode=mode)
+    S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15, xtz_mode=mode, z_store=zstore)
     S.sweep(warmup)
     S.synchronize()
     t0 = time.perf_counter()
     lat_ms, span_ms = S.sweep_timed(steps)
     S.synchronize()
     wall = time.perf_counter() - t0
-    kms = S.sweep_profiled(min(steps, 50))
     n_prof = min(steps, 50)
+    kms = S.sweep_profiled(n_prof)
     S.close()
-    return wall, lat_ms, span_ms, {k: v / n_prof for k, v in kms.items()}
+    return {"wall": wall, "lat_ms": lat_ms / steps, "span_ms": span_ms / steps,
+            "kms": {k: v / n_prof for k, v in kms.items()}}
+
+
+def load_profile(path):
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return {}
 
 
 def main():
@@ -117,47 +129,56 @@ def main():
     V = L.totvox
     copy_gbs, copy_ms = copy_bandwidth(1 << 30, 20)
 
-    # plain single chain, X'Z fused into the latent kernel (FP32 FMA)
-    wall, lat_ms, span_ms, kms = time_mode(L, d, 2, a.steps, a.warmup)
-    # plain single chain, separate FP32 X'Z kernel reading Z (the layout of the reference sweep)
-    wall_sep, lat_ms_sep, span_sep, kms_sep = time_mode(L, d, 0, a.steps, a.warmup)
+    head = time_mode(L, d, 2, 2, a.steps, a.warmup)  # fused X'Z, Z not kept
+    forms = {"fused_xtz_z_stored": (2, 0), "separate_xtz_fp32": (0, 0), "separate_xtz_mfma_bf16x2": (1, 0)}
+    others = {k: time_mode(L, d, m, z, a.steps, a.warmup) for k, (m, z) in forms.items()}
 
-    value = a.steps / wall
-    gpu_ms = span_ms / a.steps
-    lat_bytes = N * V * (4 + 1 / 8) + 8 * P * V
-    lat_s = lat_ms / a.steps / 1e3
-    achieved = lat_bytes / lat_s / 1e9
-    traffic = None
-    if os.path.exists(a.traffic):
-        with open(a.traffic) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
-    sweep_bytes_fused = N * V * (4 + 1 / 8) + V * (24 * P + 5)
-    sweep_bytes_sep = N * V * (4 + 4 + 1 / 8) + V * (24 * P + 5)
-    gpu_ms_sep = span_sep / a.steps
+    value = a.steps / head["wall"]
+    sweep_bytes = N * V * (8 + 1 / 8) + V * (24 * P + 5)  # algorithmic bytes of one sweep
+    # k_latent_vl moves y bits, b_j, WM_j in and m_j out: N V / 8 + 8 P V + 4 V
+    vl_bytes = N * V / 8 + 8 * P * V + 4 * V
+    lat_s = head["lat_ms"] / 1e3
+    achieved = vl_bytes / lat_s / 1e9
+    prof = load_profile(a.profile)
+    traffic = prof.get("bytes_per_launch")
+    valu = None
+    if prof.get("valu_wave_instructions_per_launch"):
+        w = prof["valu_wave_instructions_per_launch"]
+        # wave64 VALU issue takes 4 cycles on a 16-lane SIMD; 1024 SIMDs at 2.4 GHz
+        valu = {"wave_instructions_per_launch": w, "per_draw": w * 64 / (N * V),
+                "frac": w * 4 / (1024 * 2.4e9 * lat_s), "source": prof.get("source")}
+
+    def form_line(r, z_written, fused):
+        lat_b = N * V * (4 + 1 / 8) + 4 * P * V + (4 * P * V if fused else 0)
+        line = {"value": a.steps / r["wall"], "unit": "sweeps/s", "ms_per_step": 1e3 * r["wall"] / a.steps,
+                "latent_kernel_ms": r["lat_ms"], "latent_gbs": lat_b / (r["lat_ms"] / 1e3) / 1e9,
+                "kernel_ms_serial_breakdown": r["kms"],
+                "sweep_hbm_frac": sweep_bytes / (r["span_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
+        if not fused:
+            line["xtz_gbs"] = (4 * N * V + 4 * P * V) / (r["kms"]["xtz"] / 1e3) / 1e9
+        return line
+
     out = {
         "metric": METRIC, "value": value, "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
+        "warmup": a.warmup, "ms_per_step": 1e3 * head["wall"] / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{a.config}: grid {nx}x{ny}x{nz} ellipsoidal-shell mask, plain single chain, "
-                               f"FP32 state, FP32-FMA X'Z fused into the latent kernel",
+                               f"FP32 state, FP32-FMA X'Z fused into the latent kernel, Z not kept",
                    "V": V, "N": N, "P": P, "chains": 1},
         "latent_draws_per_s": value * N * V,
-        "latent_kernel_ms": lat_ms / a.steps,
-        "kernel_ms_serial_breakdown": kms,
-        "sweep_gpu_ms": gpu_ms,
-        "sweep_hbm_frac": sweep_bytes_fused / (gpu_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "latent_kernel_ms": head["lat_ms"],
+        "kernel_ms_serial_breakdown": head["kms"],
+        "sweep_gpu_ms": head["span_ms"],
+        "sweep_roofline": {"bound": "hbm", "algorithmic_bytes": sweep_bytes,
+                           "achieved": sweep_bytes / (head["span_ms"] / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s",
+                           "frac": sweep_bytes / (head["span_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS},
         "copy_kernel_gbs": copy_gbs,
-        "roofline": {"kernel": "k_latent (X'Z fused)", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes_per_launch": lat_bytes,
-                     "frac_of_copy": achieved / copy_gbs},
-        "plain_separate_xtz": {
-            "value": a.steps / wall_sep, "unit": "sweeps/s", "ms_per_step": 1e3 * wall_sep / a.steps,
-            "kernel_ms_serial_breakdown": kms_sep,
-            "sweep_gpu_ms": gpu_ms_sep,
-            "sweep_hbm_frac": sweep_bytes_sep / (gpu_ms_sep / 1e3) / 1e9 / HBM_PEAK_GBS,
-            "latent_gbs": (N * V * (4 + 1 / 8) + 4 * P * V) / (lat_ms_sep / a.steps / 1e3) / 1e9,
-            "xtz_gbs": (4 * N * V + 4 * P * V) / (kms_sep["xtz"] / 1e3) / 1e9},
+        "roofline": {"kernel": "k_latent_vl (latent draw + X'Z, Z not kept)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": vl_bytes, "valu_issue": valu},
+        "forms": {k: form_line(r, True, forms[k][0] == 2) for k, r in others.items()},
         "cpu_baseline": None,
     }
     if not a.no_cpu:

@@ -34,8 +34,8 @@ struct bsg_ctx {
   bsg::DevProblem pb{};
   std::vector<void *> allocs;
   hipEvent_t ev[7] = {};
-  hipEvent_t e_cf1 = nullptr, e_wish = nullptr;
-  bool wish_pending = false;
+  // e_c0: colour-1 update done (stream); e_c1: Wishart draw done (stream2)
+  hipEvent_t e_c0 = nullptr, e_c1 = nullptr;
   std::vector<hipEvent_t> lat_ev;  // timing pool for bsg_sweep_timed
   bsg::Accum acc;                  // posterior accumulators (bsg_accum_reset)
   std::vector<void *> acc_allocs;
@@ -62,9 +62,10 @@ struct bsg_ctx {
     }
     for (auto &e : lat_ev) hipEventDestroy(e);
     lat_ev.clear();
-    if (e_cf1) hipEventDestroy(e_cf1);
-    if (e_wish) hipEventDestroy(e_wish);
-    e_cf1 = e_wish = nullptr;
+    for (hipEvent_t *e : {&e_c0, &e_c1}) {
+      if (*e) hipEventDestroy(*e);
+      *e = nullptr;
+    }
     if (stream2) hipStreamDestroy(stream2);
     if (stream) hipStreamDestroy(stream);
     stream = stream2 = nullptr;
@@ -257,8 +258,8 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   // ---- device state ----
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&ctx->e_cf1, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&ctx->e_wish, hipEventDisableTiming));
+  for (hipEvent_t *e : {&ctx->e_c0, &ctx->e_c1})
+    HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   bsg::DevProblem &pb = ctx->pb;
   pb.V = V;
   pb.N = N;
@@ -268,6 +269,8 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.P = P;
   pb.nchains = C;
   pb.n0 = L.n[0];
+  pb.vb = 0;
+  pb.ve = V;
   pb.key0 = (uint32_t)(o.seed & 0xffffffffu);
   pb.key1 = (uint32_t)(o.seed >> 32);
   pb.df = (int)df;
@@ -351,14 +354,15 @@ int bsg_destroy(bsg_ctx *ctx)
 int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream));
+  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream, 0,
+                            ctx->L.totvox));
   return BSG_OK;
 }
 
 int bsg_compute_xtz(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, ctx->stream));
+  HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, ctx->stream, 0, ctx->L.totvox));
   return BSG_OK;
 }
 
@@ -382,59 +386,57 @@ int bsg_update_car_prec(bsg_ctx *ctx)
   return BSG_OK;
 }
 
-// Full sweeps. The CAR-precision draw of sweep t (final pair-sum reduction and
-// Wishart, side stream) only has to finish before the colour-0 update of
-// sweep t+1, so it overlaps the latent draw of sweep t+1 on the main stream.
-// lat_ev (2 per sweep, optional) brackets each latent kernel for timing.
-static int accum_launch(bsg_ctx *ctx)
+static int accum_launch(bsg_ctx *ctx, hipStream_t st)
 {
   if (!ctx->acc.sum) {
     set_error("posterior accumulation: call bsg_accum_reset first");
     return BSG_ERR_ARG;
   }
   const int batch = std::min(ctx->acc_count / ctx->acc_batch_len, ctx->acc.nbatches - 1);
-  HIPCHK(bsg::launch_accum(ctx->pb, ctx->acc, batch, ctx->stream));
+  HIPCHK(bsg::launch_accum(ctx->pb, ctx->acc, batch, st));
   ctx->acc_count++;
   return BSG_OK;
 }
 
+// Full sweeps in the reference order (mcmc.cpp:311-394): latent draw (+ X'Z),
+// colour-0 then colour-1 coefficients, CAR precision. The CAR-precision draw
+// of sweep t (final pair-sum reduction and Wishart) runs on stream2 and only
+// has to finish before the colour-0 update of sweep t+1, so it overlaps the
+// latent draw of sweep t+1. Each call ends joined on `stream`.
+// lat_ev (2 per sweep, optional) brackets each latent launch for timing.
 static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_every = 0)
 {
-  hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
+  hipStream_t sa = ctx->stream, sb = ctx->stream2;
   const bool fused = ctx->opts.xtz_mode == 2;
-  const bool one_pass = ctx->L.n[1] > 0;
+  const int V = ctx->L.totvox;
+  const bool two_colours = ctx->L.n[1] > 0;
+  bool wish_pending = false;
   for (int i = 0; i < nsweeps; i++) {
     const uint32_t sw = ctx->sweep;
-    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i], s1));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, s1));
-    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i + 1], s1));
-    if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, s1));
-    if (ctx->wish_pending) {
-      HIPCHK(hipStreamWaitEvent(s1, ctx->e_wish, 0));
-      ctx->wish_pending = false;
-    }
-    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, s1));
-    if (one_pass) {
-      HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, s1));  // colour 1 + pair partials
-      HIPCHK(hipEventRecord(ctx->e_cf1, s1));
-      HIPCHK(hipStreamWaitEvent(s2, ctx->e_cf1, 0));
-      HIPCHK(bsg::launch_wishart(ctx->pb, true, sw, s2));
-      HIPCHK(hipEventRecord(ctx->e_wish, s2));
-      ctx->wish_pending = true;
+    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i], sa));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, sa, 0, V));
+    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i + 1], sa));
+    if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, sa, 0, V));
+    if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, sa));
+    if (two_colours) {
+      HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, sa));  // colour 1 + pair partials
+      HIPCHK(hipEventRecord(ctx->e_c0, sa));
+      HIPCHK(hipStreamWaitEvent(sb, ctx->e_c0, 0));
+      HIPCHK(bsg::launch_wishart(ctx->pb, true, sw, sb));
+      HIPCHK(hipEventRecord(ctx->e_c1, sb));
+      wish_pending = true;
     } else {
-      HIPCHK(bsg::launch_pairs(ctx->pb, s1));
-      HIPCHK(bsg::launch_wishart(ctx->pb, false, sw, s1));
+      HIPCHK(bsg::launch_pairs(ctx->pb, sa));
+      HIPCHK(bsg::launch_wishart(ctx->pb, false, sw, sa));
     }
     ctx->sweep = sw + 1;
     if (accum_every > 0 && (i + 1) % accum_every == 0) {
-      const int rc = accum_launch(ctx);  // reads only the coefficients: main stream
+      const int rc = accum_launch(ctx, sa);  // reads only the coefficients
       if (rc != BSG_OK) return rc;
     }
   }
-  if (ctx->wish_pending) {  // join: the caller's stream sees the finished sweep
-    HIPCHK(hipStreamWaitEvent(s1, ctx->e_wish, 0));
-    ctx->wish_pending = false;
-  }
+  if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // join
   return BSG_OK;
 }
 
@@ -535,7 +537,7 @@ int bsg_accum_reset(bsg_ctx *ctx, int ntypes, int nbatches, int batch_len)
 int bsg_accum_add(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  return accum_launch(ctx);
+  return accum_launch(ctx, ctx->stream);
 }
 
 int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *var, double *prb,
@@ -623,9 +625,9 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
     const bool fused = ctx->opts.xtz_mode == 2;
     const uint32_t sw = ctx->sweep;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, st));
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, st, 0, ctx->L.totvox));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st));
+    if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st, 0, ctx->L.totvox));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
     HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[3], st));

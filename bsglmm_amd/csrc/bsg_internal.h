@@ -59,6 +59,7 @@ struct DevProblem {
   int max_parts;           // partial slots per chain
   float *bwm;              // [chain]  WM coefficient b
   uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
+  int vb, ve;              // voxel range [vb, ve) of a latent / X'Z launch
 };
 
 // Posterior accumulators (device, per chain): see k_accum.
@@ -72,8 +73,9 @@ struct Accum {
 };
 
 // kernel launchers (bsg_kernels.hip)
-hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, int zstore, uint32_t sweep, hipStream_t st);
-hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st);
+hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, int zstore, uint32_t sweep, hipStream_t st,
+                         int v0, int v1);
+hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st, int v0, int v1);
 hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);

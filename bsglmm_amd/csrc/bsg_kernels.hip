@@ -286,8 +286,8 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
   }
   // per-voxel inputs of the next iteration are loaded one iteration ahead
   const float *betc = pb.beta + (size_t)chain * pb.V * P;
-  auto voxel_of = [&](int it) { return (wave * viter + it) * vpw + sub; };
-  int sn = min(voxel_of(0), pb.V - 1);
+  auto voxel_of = [&](int it) { return pb.vb + (wave * viter + it) * vpw + sub; };
+  int sn = min(voxel_of(0), pb.ve - 1);
   float bs_n[P];
 #pragma unroll
   for (int c = 0; c < P; c++) bs_n[c] = betc[(size_t)sn * P + c];
@@ -295,17 +295,17 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
   uint32_t y_n = XREG ? pb.ybits[(size_t)sn * pb.NW + (ql >> 3)] : 0u;
   for (int it = 0; it < viter; it++) {
     const int grp = wave * viter + it;
-    if (grp * vpw >= pb.V) break;  // wave-uniform
-    const int s = grp * vpw + sub;
-    const bool valid = s < pb.V;
-    const int sv = valid ? s : pb.V - 1;
+    if (pb.vb + grp * vpw >= pb.ve) break;  // wave-uniform
+    const int s = pb.vb + grp * vpw + sub;
+    const bool valid = s < pb.ve;
+    const int sv = valid ? s : pb.ve - 1;
     float bs[P];
 #pragma unroll
     for (int c = 0; c < P; c++) bs[c] = bs_n[c];
     const float bw = b * wm_n;
     const uint32_t y0 = y_n;
     if (it + 1 < viter) {
-      sn = min(voxel_of(it + 1), pb.V - 1);
+      sn = min(voxel_of(it + 1), pb.ve - 1);
 #pragma unroll
       for (int c = 0; c < P; c++) bs_n[c] = betc[(size_t)sn * P + c];
       wm_n = pb.wm[sn];
@@ -380,6 +380,93 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
 }
 
 // ---------------------------------------------------------------------------
+// k_latent_vl: fused latent draw + X'Z with voxels across lanes, for the form
+// that does not keep Z. A wave holds 64/G voxels; the G lanes of a voxel take
+// G contiguous ranges of its subject quads. X is staged once per block in LDS
+// as [q][c] float4 and read at addresses shared by all lanes of a subject
+// group (broadcast), so the per-voxel inputs live in registers, the covariate
+// products need no cross-lane traffic, and the G partial sums of m_j meet in
+// log2(G) lane exchanges once per voxel. Counters and draw arithmetic are
+// those of k_latent, so the draws are bit-identical; only the summation order
+// of m_j differs. Blocks walk voxel tiles of 4 * 64/G voxels, grid-strided.
+// ---------------------------------------------------------------------------
+template <int P, int G>
+__global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep)
+{
+  extern __shared__ float4 xs[];  // [NQ][P]
+  constexpr int VPW = 64 / G;
+  const int chain = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int vl = lane % VPW;
+  const int g = lane / VPW;
+  for (int i = threadIdx.x; i < pb.NQ * P; i += blockDim.x) {
+    const int q = i / P, c = i - q * P;
+    xs[i] = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + 4 * q);
+  }
+  __syncthreads();
+  const float b = pb.bwm[chain];
+  const float eps = pb.trunc_eps;
+  float xsum[P];
+#pragma unroll
+  for (int c = 0; c < P; c++) xsum[c] = pb.xsum[c];
+  const int R = (pb.NQ + G - 1) / G;
+  const int q0 = g * R;
+  const int q1 = min(q0 + R, pb.NQ);
+  const int tiles = (pb.ve - pb.vb + 4 * VPW - 1) / (4 * VPW);
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const int s = pb.vb + (t * 4 + (threadIdx.x >> 6)) * VPW + vl;
+    const bool valid = s < pb.ve;
+    const int sv = valid ? s : pb.ve - 1;
+    const float bw = b * pb.wm[sv];
+    const float *bp = pb.beta + ((size_t)chain * pb.V + sv) * P;
+    float bs[P];
+#pragma unroll
+    for (int c = 0; c < P; c++) bs[c] = bp[c];
+    const uint32_t *yrow = pb.ybits + (size_t)sv * pb.NW;
+    const uint32_t cbase = (uint32_t)sv * (uint32_t)pb.NQ;
+    float acc[P];
+#pragma unroll
+    for (int c = 0; c < P; c++) acc[c] = 0.0f;
+    for (int q = q0; q < q1; q++) {
+      float4 x[P];
+#pragma unroll
+      for (int c = 0; c < P; c++) x[c] = xs[q * P + c];
+      float mu[4] = {bw, bw, bw, bw};
+#pragma unroll
+      for (int c = 0; c < P; c++) {
+        mu[0] = fmaf(bs[c], x[c].x, mu[0]);
+        mu[1] = fmaf(bs[c], x[c].y, mu[1]);
+        mu[2] = fmaf(bs[c], x[c].z, mu[2]);
+        mu[3] = fmaf(bs[c], x[c].w, mu[3]);
+      }
+      const uint32_t yw = yrow[q >> 3] >> ((q & 7) * 4);
+      uint32_t rr[4];
+      philox10(cbase + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain, pb.key0, pb.key1, rr);
+      float z[4];
+      latent_quad(mu, yw, rr, eps, z);
+      // X is zero past N, so padding subjects add nothing to m_j
+#pragma unroll
+      for (int c = 0; c < P; c++) {
+        acc[c] = fmaf(x[c].x, z[0], acc[c]);
+        acc[c] = fmaf(x[c].y, z[1], acc[c]);
+        acc[c] = fmaf(x[c].z, z[2], acc[c]);
+        acc[c] = fmaf(x[c].w, z[3], acc[c]);
+      }
+    }
+#pragma unroll
+    for (int off = VPW; off < 64; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
+    }
+    if (valid && g == 0) {
+      float *m = pb.M + ((size_t)chain * pb.V + s) * P;
+#pragma unroll
+      for (int c = 0; c < P; c++) m[c] = fmaf(-bw, xsum[c], acc[c]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_xtz_f32: m_j = X'Z_j - b WM_j X'1 in FP32 FMA, reading Z once. Same lane
 // grouping and voxel walk as k_latent (covarGPU.cu:327-352 contraction).
 // ---------------------------------------------------------------------------
@@ -401,9 +488,9 @@ __global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int 
   }
   for (int it = 0; it < viter; it++) {
     const int grp = wave * viter + it;
-    if (grp * vpw >= pb.V) break;  // wave-uniform
-    const int s = grp * vpw + sub;
-    const bool valid = s < pb.V;
+    if (pb.vb + grp * vpw >= pb.ve) break;  // wave-uniform
+    const int s = pb.vb + grp * vpw + sub;
+    const bool valid = s < pb.ve;
     float acc[P];
 #pragma unroll
     for (int c = 0; c < P; c++) acc[c] = 0.0f;
@@ -478,10 +565,10 @@ __global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
   const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int s0 = tile * 16;
-  if (s0 >= pb.V) return;  // wave-uniform
+  const int s0 = pb.vb + tile * 16;
+  if (s0 >= pb.ve) return;  // wave-uniform
   const int r = lane & 15, g = lane >> 4;
-  const int srow = min(s0 + r, pb.V - 1);
+  const int srow = min(s0 + r, pb.ve - 1);
   const float *zrow = pb.Z + ((size_t)chain * pb.V + srow) * pb.NS;
   const int col = r;  // B column of this lane
   const float *xrow = pb.X + (size_t)(col < P ? col : (col < 2 * P ? col - P : 0)) * pb.NS;
@@ -514,7 +601,7 @@ __global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
   for (int i = 0; i < 4; i++) {
     const float other = __shfl_down(acc[i], P, 16);  // D[row][col + P]
     const int s = s0 + 4 * g + i;
-    if (col < P && s < pb.V) {
+    if (col < P && s < pb.ve) {
       const float bw = b * pb.wm[s];
       pb.M[((size_t)chain * pb.V + s) * P + col] = fmaf(-bw, pb.xsum[col], acc[i] + other);
     }
@@ -526,7 +613,8 @@ __global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
 // order, forms S = (T+I)^-1 and L = chol(S) in FP64 (covarGPU.cu:1086-1101),
 // draws the Bartlett factor A (A_ii^2 ~ chi2(df-i) by Marsaglia-Tsang gamma,
 // A_ij ~ N(0,1)) and writes Lambda = (L A)(L A)' (cholesky.cpp:259-281).
-// Advances the chain's sweep index. Runs on the first wave of a block.
+// The sweep index (Philox counter word) is passed in by the host. Runs on the
+// first wave of a block.
 // ---------------------------------------------------------------------------
 __device__ double gamma_mt(double a, uint32_t row, uint32_t sweep, uint32_t chain, uint32_t k0,
                            uint32_t k1)
@@ -985,12 +1073,59 @@ static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
     default: return hipErrorInvalidValue; \
   }
 
-hipError_t launch_latent(const DevProblem &pb, bool fused, int zstore, uint32_t sweep, hipStream_t st)
+// Subject groups per voxel for k_latent_vl: enough lanes per voxel to give
+// the chip many waves, few enough that every lane keeps several quads.
+int latent_vl_groups(const DevProblem &pb)
 {
+  if (pb.NQ <= 4) return 1;
+  if (pb.NQ <= 16) return 2;
+  if (pb.NQ <= 128) return 4;
+  return 8;
+}
+
+static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st)
+{
+  const size_t lds = (size_t)pb.NQ * pb.P * sizeof(float4);
+  const int vpb = 4 * (64 / G);  // voxels per block tile
+  const long long tiles = ((long long)(pb.ve - pb.vb) + vpb - 1) / vpb;
+  const long long per = (tiles + 4095) / 4096;  // tiles per block, equal shares
+  dim3 grid((unsigned)((tiles + per - 1) / per), (unsigned)pb.nchains);
+#define VK(PP, GG) hipLaunchKernelGGL((k_latent_vl<PP, GG>), grid, dim3(256), lds, st, pb, sweep)
+#define V_(PP)                         \
+  do {                                 \
+    if (G == 1) VK(PP, 1);             \
+    else if (G == 2) VK(PP, 2);        \
+    else if (G == 4) VK(PP, 4);        \
+    else if (G == 8) VK(PP, 8);        \
+    else return hipErrorInvalidValue;  \
+  } while (0)
+  BSG_DISPATCH_P(pb.P, V_)
+#undef V_
+#undef VK
+  return hipGetLastError();
+}
+
+// Voxel ranges: latent and X'Z launches cover the sorted voxels [v0, v1), so a
+// sweep can draw one colour while the other colour's coefficients update.
+static bool set_range(DevProblem &pb, int v0, int v1)
+{
+  if (v0 < 0 || v1 > pb.V || v0 > v1) return false;
+  pb.vb = v0;
+  pb.ve = v1;
+  return true;
+}
+
+hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t sweep, hipStream_t st, int v0, int v1)
+{
+  DevProblem pb = pb0;
+  if (!set_range(pb, v0, v1)) return hipErrorInvalidValue;
+  if (v0 == v1) return hipSuccess;
+  if (fused && zstore == 2 && (size_t)pb.NQ * pb.P * sizeof(float4) <= 65536)
+    return launch_latent_vl(pb, latent_vl_groups(pb), sweep, st);
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
   const int vpw = 64 >> lshift;
-  const long long groups = ((long long)pb.V + vpw - 1) / vpw;
+  const long long groups = ((long long)(v1 - v0) + vpw - 1) / vpw;
   const int viter = 8;
   const long long waves = (groups + viter - 1) / viter;
   dim3 grid((unsigned)((waves + 3) / 4), (unsigned)pb.nchains);
@@ -1026,10 +1161,13 @@ hipError_t launch_latent(const DevProblem &pb, bool fused, int zstore, uint32_t 
   return hipGetLastError();
 }
 
-hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st)
+hipError_t launch_xtz(const DevProblem &pb0, int mode, hipStream_t st, int v0, int v1)
 {
+  DevProblem pb = pb0;
+  if (!set_range(pb, v0, v1)) return hipErrorInvalidValue;
+  if (v0 == v1) return hipSuccess;
   if (mode == 1) {
-    const long long tiles = ((long long)pb.V + 15) / 16;
+    const long long tiles = ((long long)(v1 - v0) + 15) / 16;
     dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)pb.nchains);
 #define M_(PP) hipLaunchKernelGGL(k_xtz_mfma<PP>, grid, dim3(256), 0, st, pb)
     BSG_DISPATCH_P(pb.P, M_)
@@ -1039,7 +1177,7 @@ hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st)
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
   const int vpw = 64 >> lshift;
-  const long long groups = ((long long)pb.V + vpw - 1) / vpw;
+  const long long groups = ((long long)(v1 - v0) + vpw - 1) / vpw;
   const int viter = 8;
   const long long waves = (groups + viter - 1) / viter;
   dim3 grid((unsigned)((waves + 3) / 4), (unsigned)pb.nchains);

@@ -134,28 +134,60 @@ def test_full_sweep_matches_oracle(nsubs, ncovar, xtz_mode):
     assert np.max(np.abs(t2 - O.pair_sums(S.get_spat_coef())) / np.abs(t2).max()) < 1e-12
 
 
-@pytest.mark.parametrize("z_store", [1, 2])
-def test_z_store_variants(z_store):
-    """Streaming stores (1) must give the same Z; without Z (2) the fused X'Z
-    must match X' times the oracle's own draws."""
-    print(f"case c1 z_store={z_store}", flush=True)
+def test_z_store_streaming():
+    """Streaming stores (z_store 1) give the same Z and X'Z as plain stores."""
+    print("case c1 z_store=1", flush=True)
     d, lay, L, S, O, sc0 = problem("c1", xtz_mode=2)
-    S2 = Sampler(L, d["covar"], d["data"], seed=SEED, xtz_mode=2, z_store=z_store)
+    S2 = Sampler(L, d["covar"], d["data"], seed=SEED, xtz_mode=2, z_store=1)
     for T in (S, S2):
         T.set_spat_coef(sc0)
         T.set_sweep_index(3)
         T.update_latent()
-    if z_store == 1:
-        assert np.array_equal(S.get_z(), S2.get_z())
-        assert np.array_equal(S.get_xtz(), S2.get_xtz())
-    else:
-        z_orc = O.update_z(sc0, 3)
-        m_ref = O.xtz(z_orc.astype(np.float32))
+    assert np.array_equal(S.get_z(), S2.get_z())
+    assert np.array_equal(S.get_xtz(), S2.get_xtz())
+
+
+# (N, P) cases span every lane split of the voxel-per-lane kernel:
+# N=5 one lane per voxel, N=37/50 two, N=300 four, N=600 eight
+@pytest.mark.parametrize("nsubs,ncovar,nchains", [(5, 2, 1), (37, 8, 1), (None, None, 2), (300, 4, 1),
+                                                  (600, 3, 1), (140, 1, 1)])
+def test_z_free_fused_sweep(nsubs, ncovar, nchains):
+    """Without Z (z_store 2) X'Z is formed from draws held in registers: it
+    must match X' times the oracle's own draws (same Philox counters), and the
+    coefficient updates that consume it must match the oracle's."""
+    print(f"case c1 z_store=2 N={nsubs} P={ncovar} chains={nchains}", flush=True)
+    d = synth.generate("c1", nsubs=nsubs, ncovar=ncovar)
+    nx, ny, nz = d["grid"]
+    lay = oracle.layout(d["mask"], nx, ny, nz)
+    L = Layout(d["mask"], nx, ny, nz)
+    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2)
+    O = oracle.Oracle(lay, d["covar"], d["data"], seed=SEED)
+    P = S.P
+    rng = np.random.default_rng(11)
+    sweep = 3
+    prec0 = (np.eye(P) * 1.5 + 0.2).astype(np.float32)
+    S.set_sweep_index(sweep)
+    scs = []
+    for c in range(nchains):
+        beta = d["beta_true"] + 0.1 * rng.standard_normal(d["beta_true"].shape).astype(np.float32)
+        scs.append(spat_coef_from_ranks(beta, lay))
+        S.set_spat_coef(scs[c], chain=c)
+        S.set_prec(prec0, chain=c)
+    S.sweep(1)
+    S.synchronize()
+    for c in range(nchains):
+        O.set_chain(c)
+        z_orc = O.update_z(scs[c], sweep)
+        m_gpu = S.get_xtz(c)
         scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_orc)
-        assert np.max(np.abs(S2.get_xtz() - m_ref) / scale) < 2e-5
-        assert np.array_equal(S.get_xtz(), S2.get_xtz())
-        with pytest.raises(Exception):
-            S2.get_z()
+        # the GPU draws are FP32 (1e-5 of the oracle's) and summed in FP32
+        assert np.max(np.abs(m_gpu - O.xtz(z_orc.astype(np.float32))) / scale) < 2e-5
+        ref = O.update_coef(0, m_gpu.astype(np.float64), prec0, scs[c].astype(np.float64), sweep)
+        ref = O.update_coef(1, m_gpu.astype(np.float64), prec0, ref, sweep)
+        assert rel_err(S.get_spat_coef(c), ref) < 1e-5
+    assert S.fallback_count() == 0
+    with pytest.raises(Exception):
+        S.get_z()
 
 
 def test_full_size_invariants_and_determinism():

@@ -6,6 +6,7 @@ like the reference's update functions (bsglmm/covarGPU.cu):
 
     Layout(mask, nrow, ncol, ndepth)          main.cu:189-394 layout
     Sampler.update_latent()                   updateZGPU          covarGPU.cu:1625
+    Sampler.update_wm_coef()                  updateBetaGPU       covarGPU.cu:686
     Sampler.compute_xtz() / update_coef(c)    updateSpatCoefGPU   covarGPU.cu:535
     Sampler.update_car_prec()                 updateSpatCoefPrecGPU covarGPU.cu:1063
     Sampler.sweep(n)                          mcmc.cpp:311-394 loop body
@@ -60,6 +61,9 @@ def lib():
         "bsg_sweep": (C.c_int, [vp, C.c_int]),
         "bsg_update_latent": (C.c_int, [vp]),
         "bsg_compute_xtz": (C.c_int, [vp]),
+        "bsg_update_wm_coef": (C.c_int, [vp]),
+        "bsg_get_wm_coef": (C.c_int, [vp, C.c_int, pf]),
+        "bsg_set_wm_coef": (C.c_int, [vp, C.c_int, C.c_float]),
         "bsg_update_coef": (C.c_int, [vp, C.c_int]),
         "bsg_update_car_prec": (C.c_int, [vp]),
         "bsg_synchronize": (C.c_int, [vp]),
@@ -215,6 +219,9 @@ class Sampler:
     def compute_xtz(self):
         _check(lib().bsg_compute_xtz(self._h))
 
+    def update_wm_coef(self):
+        _check(lib().bsg_update_wm_coef(self._h))
+
     def update_coef(self, colour):
         _check(lib().bsg_update_coef(self._h, int(colour)))
 
@@ -248,6 +255,14 @@ class Sampler:
     def set_xtz(self, m, chain=0):
         m = np.ascontiguousarray(m, dtype=np.float32)
         _check(lib().bsg_set_xtz(self._h, chain, _ptr(m, C.c_float)))
+
+    def get_wm_coef(self, chain=0):
+        v = C.c_float()
+        _check(lib().bsg_get_wm_coef(self._h, chain, C.byref(v)))
+        return v.value
+
+    def set_wm_coef(self, b, chain=0):
+        _check(lib().bsg_set_wm_coef(self._h, chain, C.c_float(b)))
 
     def get_prec(self, chain=0):
         out = np.empty((self.P, self.P), np.float32)

@@ -40,6 +40,7 @@ struct bsg_ctx {
   bsg::Accum acc;                  // posterior accumulators (bsg_accum_reset)
   std::vector<void *> acc_allocs;
   int acc_count = 0, acc_batch_len = 1;
+  int bstat_parts = 0;             // WM-statistic partials written by the last latent launch
 
   void free_accum()
   {
@@ -233,6 +234,13 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   }
 
   // ---- host-side preparation in sorted order ----
+  // y indexes the two truncation intervals (covarGPU.cu:1414-1420), so only 0 and 1 are valid
+  for (size_t k = 0; k < (size_t)N * V; k++)
+    if (data[k] > 1) {
+      set_error("bsg_create: lesion data must be 0 or 1 (value " + std::to_string((int)data[k]) +
+                " at subject " + std::to_string(k / V) + ", mask rank " + std::to_string(k % V) + ")");
+      return BSG_ERR_ARG;
+    }
   std::vector<uint32_t> ybits((size_t)V * NW, 0u);
   for (int s = 0; s < V; s++) {
     const int rank = L.rank_of_sorted[s];
@@ -271,6 +279,17 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.n0 = L.n[0];
   pb.vb = 0;
   pb.ve = V;
+  // b is drawn each sweep only when a WM image is given (mcmc.cpp:344-345, UPDATE_WM)
+  pb.wm_update = wm ? 1 : 0;
+  {
+    double w2 = 0.0;
+    for (int s = 0; s < V; s++) w2 += (double)wms[s] * (double)wms[s];
+    pb.wm2 = (double)N * w2;
+  }
+  if (pb.wm_update && !(pb.wm2 > 0.0)) {
+    set_error("bsg_create: the WM image is zero over the mask; b has no information");
+    return BSG_ERR_ARG;
+  }
   pb.key0 = (uint32_t)(o.seed & 0xffffffffu);
   pb.key1 = (uint32_t)(o.seed >> 32);
   pb.df = (int)df;
@@ -299,6 +318,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(pb.partials, (size_t)C * pb.max_parts * NE);
 
   ALLOC(pb.bwm, (size_t)C);
+  ALLOC(pb.bstat, (size_t)C * bsg::BSTAT_SLOTS);
 
   ALLOC(pb.flags, (size_t)4);
 #undef ALLOC
@@ -355,7 +375,20 @@ int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
   HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream, 0,
-                            ctx->L.totvox));
+                            ctx->L.totvox, &ctx->bstat_parts));
+  return BSG_OK;
+}
+
+int bsg_update_wm_coef(bsg_ctx *ctx)
+{
+  CTXCHK(ctx);
+  if (!ctx->pb.wm_update) return BSG_OK;  // no WM image: b stays 0 (mcmc.cpp:344)
+  if (ctx->bstat_parts == 0 && ctx->opts.z_store == 2) {
+    set_error("bsg_update_wm_coef: Z is not kept; call bsg_update_latent first in this sweep");
+    return BSG_ERR_ARG;
+  }
+  HIPCHK(bsg::launch_bwm(ctx->pb, ctx->bstat_parts, ctx->sweep, ctx->stream));
+  ctx->bstat_parts = 0;
   return BSG_OK;
 }
 
@@ -414,9 +447,11 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
   for (int i = 0; i < nsweeps; i++) {
     const uint32_t sw = ctx->sweep;
     if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i], sa));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, sa, 0, V));
+    int parts = 0;
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, sa, 0, V, &parts));
     if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i + 1], sa));
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, sa, 0, V));
+    HIPCHK(bsg::launch_bwm(ctx->pb, parts, sw, sa));  // WM coefficient (mcmc.cpp:344-345)
     if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
     HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, sa));
     if (two_colours) {
@@ -625,9 +660,11 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
     const bool fused = ctx->opts.xtz_mode == 2;
     const uint32_t sw = ctx->sweep;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, st, 0, ctx->L.totvox));
+    int parts = 0;
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, st, 0, ctx->L.totvox, &parts));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st, 0, ctx->L.totvox));
+    HIPCHK(bsg::launch_bwm(ctx->pb, parts, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
     HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[3], st));
@@ -726,6 +763,26 @@ int bsg_set_xtz(bsg_ctx *ctx, int chain, const float *m)
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipMemcpy(ctx->pb.M + (size_t)chain * V * P, mm.data(), sizeof(float) * mm.size(),
                    hipMemcpyHostToDevice));
+  return BSG_OK;
+}
+
+int bsg_get_wm_coef(bsg_ctx *ctx, int chain, float *b)
+{
+  CHAINCHK(ctx, chain);
+  if (!b) {
+    set_error("bsg_get_wm_coef: NULL output");
+    return BSG_ERR_ARG;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(b, ctx->pb.bwm + chain, sizeof(float), hipMemcpyDeviceToHost));
+  return BSG_OK;
+}
+
+int bsg_set_wm_coef(bsg_ctx *ctx, int chain, float b)
+{
+  CHAINCHK(ctx, chain);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(ctx->pb.bwm + chain, &b, sizeof(float), hipMemcpyHostToDevice));
   return BSG_OK;
 }
 

@@ -16,8 +16,10 @@ constexpr uint32_t TAG_LATENT = 1;  // index = voxel * NQ + subject quad
 constexpr uint32_t TAG_COEF = 2;    // index = voxel, sub = group of 4 normals
 constexpr uint32_t TAG_GAMMA = 3;   // index = Bartlett row, sub = trial
 constexpr uint32_t TAG_WNORM = 4;   // index = row * P + column
+constexpr uint32_t TAG_BWM = 5;     // index = 0: normal of the WM coefficient draw
 
 constexpr int PAIR_BLOCKS = 512;    // fixed partial count per chain: deterministic sums
+constexpr int BSTAT_SLOTS = 4096;   // per-chain partial slots of the WM-coefficient statistic
 constexpr int MAX_P = 8;
 
 // Host copy of the reference layout (main.cu:189-394) plus the colour-sorted
@@ -60,6 +62,10 @@ struct DevProblem {
   float *bwm;              // [chain]  WM coefficient b
   uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
   int vb, ve;              // voxel range [vb, ve) of a latent / X'Z launch
+  // WM coefficient b (covarGPU.cu:588-724): drawn each sweep when wm_update
+  int wm_update;
+  double wm2;              // N sum_j WM_j^2
+  double *bstat;           // [chain][BSTAT_SLOTS] block partials of sum_j WM_j (sum_i Z_ij - X'1 b_j)
 };
 
 // Posterior accumulators (device, per chain): see k_accum.
@@ -73,8 +79,13 @@ struct Accum {
 };
 
 // kernel launchers (bsg_kernels.hip)
+// Returns, in *bstat_parts, how many WM-statistic partials the launch wrote
+// (0 when the kernel used does not form them; see launch_bwm).
 hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, int zstore, uint32_t sweep, hipStream_t st,
-                         int v0, int v1);
+                         int v0, int v1, int *bstat_parts = nullptr);
+// WM coefficient draw. parts > 0: the latent kernel already wrote that many
+// partials; parts == 0: they are formed here from the stored Z first.
+hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st, int v0, int v1);
 hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);

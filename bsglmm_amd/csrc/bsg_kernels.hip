@@ -18,6 +18,8 @@
 // (index, sweep, tag | sub << 4, chain); no per-thread generator state.
 #include "bsg_internal.h"
 
+#include <algorithm>
+
 namespace bsg {
 
 // ---------------------------------------------------------------------------
@@ -58,6 +60,53 @@ __device__ __forceinline__ float u01f(uint32_t x)
 __device__ __forceinline__ double u01d(uint32_t x)
 {
   return (double)(((x >> 9) << 1) | 1u) * 5.9604644775390625e-8;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic block reductions in FP64 (fixed order, so sums repeat bit for bit).
+// ---------------------------------------------------------------------------
+// Sum of the per-block partials in a fixed order: thread t adds blocks
+// t, t+256, ... of every entry, then a fixed LDS tree over the 256 threads.
+template <int NE>
+__device__ void sum_partials(int nparts, const double *part, double *T)
+{
+  __shared__ double red[256];
+  double acc[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) acc[e] = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
+  }
+  for (int e = 0; e < NE; e++) {
+    red[threadIdx.x] = acc[e];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) T[e] = red[0];
+    __syncthreads();
+  }
+}
+
+// Block sum of NE doubles in fixed order; thread e < NE of the block gets entry e.
+template <int NE>
+__device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst)
+{
+  __shared__ double red[4][NE];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    double v = acc[e];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wv][e] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NE) {
+    const int e = threadIdx.x;
+    dst[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -363,7 +412,7 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
       if (lshift == 6) {
         const float tot = wave_reduce_scatter<P>(acc, lane);
         const int c = red_cov<P>(lane);
-        if (c >= 0) pb.M[((size_t)chain * pb.V + s) * P + c] = fmaf(-bw, pb.xsum[c], tot);
+        if (c >= 0) pb.M[((size_t)chain * pb.V + s) * P + c] = tot;
       } else {
         for (int off = (1 << lshift) >> 1; off > 0; off >>= 1) {
 #pragma unroll
@@ -372,7 +421,7 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
         if (valid && ql == 0) {
           float *m = pb.M + ((size_t)chain * pb.V + s) * P;
 #pragma unroll
-          for (int c = 0; c < P; c++) m[c] = fmaf(-bw, pb.xsum[c], acc[c]);
+          for (int c = 0; c < P; c++) m[c] = acc[c];
         }
       }
     }
@@ -390,7 +439,7 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
 // those of k_latent, so the draws are bit-identical; only the summation order
 // of m_j differs. Blocks walk voxel tiles of 4 * 64/G voxels, grid-strided.
 // ---------------------------------------------------------------------------
-template <int P, int G>
+template <int P, int G, bool WMU>
 __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep)
 {
   extern __shared__ float4 xs[];  // [NQ][P]
@@ -413,11 +462,13 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
   const int q0 = g * R;
   const int q1 = min(q0 + R, pb.NQ);
   const int tiles = (pb.ve - pb.vb + 4 * VPW - 1) / (4 * VPW);
+  double bpart = 0.0;  // WMU: this thread's share of sum_j WM_j (sum_i Z_ij - X'1 b_j)
   for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
     const int s = pb.vb + (t * 4 + (threadIdx.x >> 6)) * VPW + vl;
     const bool valid = s < pb.ve;
     const int sv = valid ? s : pb.ve - 1;
-    const float bw = b * pb.wm[sv];
+    const float wmv = pb.wm[sv];
+    const float bw = b * wmv;
     const float *bp = pb.beta + ((size_t)chain * pb.V + sv) * P;
     float bs[P];
 #pragma unroll
@@ -427,6 +478,7 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
     float acc[P];
 #pragma unroll
     for (int c = 0; c < P; c++) acc[c] = 0.0f;
+    float zs = 0.0f;
     for (int q = q0; q < q1; q++) {
       float4 x[P];
 #pragma unroll
@@ -452,22 +504,86 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
         acc[c] = fmaf(x[c].z, z[2], acc[c]);
         acc[c] = fmaf(x[c].w, z[3], acc[c]);
       }
+      if (WMU) {
+        const int i0 = 4 * q;
+        zs += z[0] + ((i0 + 1 < pb.N) ? z[1] : 0.0f) + ((i0 + 2 < pb.N) ? z[2] : 0.0f) +
+              ((i0 + 3 < pb.N) ? z[3] : 0.0f);
+      }
     }
 #pragma unroll
     for (int off = VPW; off < 64; off <<= 1) {
 #pragma unroll
       for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
+      if (WMU) zs += __shfl_xor(zs, off, 64);
     }
     if (valid && g == 0) {
       float *m = pb.M + ((size_t)chain * pb.V + s) * P;
 #pragma unroll
-      for (int c = 0; c < P; c++) m[c] = fmaf(-bw, xsum[c], acc[c]);
+      for (int c = 0; c < P; c++) m[c] = acc[c];
+      if (WMU) {
+        double r = (double)zs;
+#pragma unroll
+        for (int c = 0; c < P; c++) r -= (double)xsum[c] * (double)bs[c];
+        bpart += (double)wmv * r;
+      }
     }
+  }
+  if (WMU) {
+    double bp[1] = {bpart};
+    block_sum_store<1>(bp, pb.bstat + (size_t)chain * BSTAT_SLOTS + blockIdx.x);
   }
 }
 
 // ---------------------------------------------------------------------------
-// k_xtz_f32: m_j = X'Z_j - b WM_j X'1 in FP32 FMA, reading Z once. Same lane
+// WM coefficient b (covarGPU.cu:588-724, GPU semantics: flat prior).
+//   S = sum_j WM_j sum_i (Z_ij - x_i'b_j)   (betaMGPU, X'1 . b_j per voxel)
+//   W = N sum_j WM_j^2                      (betaVGPU, host FP64 at set-up)
+//   b ~ N(S / W, 1 / W)
+// k_bstat forms block partials of S from the stored Z (one thread per voxel,
+// FP64 accumulation); the Z-free latent kernel forms the same partials from
+// its draws. k_bwm sums the partials in slot order and draws b.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(256) void k_bstat(DevProblem pb)
+{
+  const int chain = blockIdx.y;
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  double part = 0.0;
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < pb.V; s += gridDim.x * 256) {
+    const float4 *zrow = reinterpret_cast<const float4 *>(pb.Z + ((size_t)chain * pb.V + s) * pb.NS);
+    float zs = 0.0f;
+    for (int q = 0; q < pb.NQ; q++) {
+      const float4 z = zrow[q];
+      const int i0 = 4 * q;
+      zs += z.x + ((i0 + 1 < pb.N) ? z.y : 0.0f) + ((i0 + 2 < pb.N) ? z.z : 0.0f) +
+            ((i0 + 3 < pb.N) ? z.w : 0.0f);
+    }
+    double r = (double)zs;
+#pragma unroll
+    for (int c = 0; c < P; c++) r -= (double)pb.xsum[c] * (double)betc[(size_t)s * P + c];
+    part += (double)pb.wm[s] * r;
+  }
+  double pp[1] = {part};
+  block_sum_store<1>(pp, pb.bstat + (size_t)chain * BSTAT_SLOTS + blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_bwm(DevProblem pb, int nparts, uint32_t sweep)
+{
+  const int chain = blockIdx.y;
+  __shared__ double S[1];
+  sum_partials<1>(nparts, pb.bstat + (size_t)chain * BSTAT_SLOTS, S);
+  if (threadIdx.x == 0) {
+    uint32_t rr[4];
+    philox10(0u, sweep, TAG_BWM, (uint32_t)chain, pb.key0, pb.key1, rr);
+    const double n = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
+    const double var = 1.0 / pb.wm2;
+    pb.bwm[chain] = (float)(S[0] * var + sqrt(var) * n);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_xtz_f32: m_j = X'Z_j in FP32 FMA, reading Z once (b WM_j X'1 is taken off
+// by the coefficient kernel, with the b of the sweep). Same lane
 // grouping and voxel walk as k_latent (covarGPU.cu:327-352 contraction).
 // ---------------------------------------------------------------------------
 template <int P, bool XREG>
@@ -479,7 +595,6 @@ __global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int 
   const int vpw = 64 >> lshift;
   const int sub = lane >> lshift;
   const int ql = lane & ((1 << lshift) - 1);
-  const float b = pb.bwm[chain];
   float4 xr[P];
   if (XREG) {
     const int i0 = 4 * min(ql, pb.NQ - 1);
@@ -516,10 +631,9 @@ __global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int 
       for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
     }
     if (valid && ql == 0) {
-      const float bw = b * pb.wm[s];
       float *m = pb.M + ((size_t)chain * pb.V + s) * P;
 #pragma unroll
-      for (int c = 0; c < P; c++) m[c] = fmaf(-bw, pb.xsum[c], acc[c]);
+      for (int c = 0; c < P; c++) m[c] = acc[c];
     }
   }
 }
@@ -596,15 +710,11 @@ __global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zh, xb, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zl, xb, acc, 0, 0, 0);
   }
-  const float b = pb.bwm[chain];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const float other = __shfl_down(acc[i], P, 16);  // D[row][col + P]
     const int s = s0 + 4 * g + i;
-    if (col < P && s < pb.ve) {
-      const float bw = b * pb.wm[s];
-      pb.M[((size_t)chain * pb.V + s) * P + col] = fmaf(-bw, pb.xsum[col], acc[i] + other);
-    }
+    if (col < P && s < pb.ve) pb.M[((size_t)chain * pb.V + s) * P + col] = acc[i] + other;
   }
 }
 
@@ -647,31 +757,6 @@ __device__ void chol_lower(double (&a)[P][P])
       a[i][j] = s / d;
     }
     for (int i = 0; i < j; i++) a[i][j] = 0.0;
-  }
-}
-
-// Sum of the per-block partials in a fixed order: thread t adds blocks
-// t, t+256, ... of every entry, then a fixed LDS tree over the 256 threads.
-template <int NE>
-__device__ void sum_partials(int nparts, const double *part, double *T)
-{
-  __shared__ double red[256];
-  double acc[NE];
-#pragma unroll
-  for (int e = 0; e < NE; e++) acc[e] = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += 256) {
-#pragma unroll
-    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
-  }
-  for (int e = 0; e < NE; e++) {
-    red[threadIdx.x] = acc[e];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) T[e] = red[0];
-    __syncthreads();
   }
 }
 
@@ -781,25 +866,6 @@ __device__ __forceinline__ void add_pair(const float (&bj)[P], const float (&bk)
   }
 }
 
-// Block sum of NE doubles in fixed order; thread e < NE of the block gets entry e.
-template <int NE>
-__device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst)
-{
-  __shared__ double red[4][NE];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int e = 0; e < NE; e++) {
-    double v = acc[e];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wv][e] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < NE) {
-    const int e = threadIdx.x;
-    dst[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Coefficient update of one colour. With Q = X'X + n_j Lambda and
 // r = m + Lambda sumV, the reference draws b = Q^-1 r + chol_lower(Q^-1) z
@@ -894,7 +960,12 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
       for (int c = 0; c < P; c++) bk[d][c] = 0.0f;
     }
   }
-  const float *m = pb.M + ((size_t)chain * pb.V + s) * P;
+  // m_j = X'(Z_j - b WM_j) = X'Z_j - b WM_j X'1, with the b of this sweep
+  const float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
+  const float bw = pb.bwm[chain] * pb.wm[s];
+  float m[P];
+#pragma unroll
+  for (int i = 0; i < P; i++) m[i] = fmaf(-bw, pb.xsum[i], mz[i]);
   float Q[P][P], r[P];
 #pragma unroll
   for (int i = 0; i < P; i++) {
@@ -1083,14 +1154,19 @@ int latent_vl_groups(const DevProblem &pb)
   return 8;
 }
 
-static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st)
+static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st, int *parts)
 {
   const size_t lds = (size_t)pb.NQ * pb.P * sizeof(float4);
   const int vpb = 4 * (64 / G);  // voxels per block tile
   const long long tiles = ((long long)(pb.ve - pb.vb) + vpb - 1) / vpb;
-  const long long per = (tiles + 4095) / 4096;  // tiles per block, equal shares
+  const long long per = (tiles + BSTAT_SLOTS - 1) / BSTAT_SLOTS;  // tiles per block, equal shares
   dim3 grid((unsigned)((tiles + per - 1) / per), (unsigned)pb.nchains);
-#define VK(PP, GG) hipLaunchKernelGGL((k_latent_vl<PP, GG>), grid, dim3(256), lds, st, pb, sweep)
+  const bool wmu = pb.wm_update != 0;
+#define VK(PP, GG)                                                                                    \
+  do {                                                                                                \
+    if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, GG, true>), grid, dim3(256), lds, st, pb, sweep);  \
+    else hipLaunchKernelGGL((k_latent_vl<PP, GG, false>), grid, dim3(256), lds, st, pb, sweep);     \
+  } while (0)
 #define V_(PP)                         \
   do {                                 \
     if (G == 1) VK(PP, 1);             \
@@ -1102,6 +1178,7 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
   BSG_DISPATCH_P(pb.P, V_)
 #undef V_
 #undef VK
+  if (parts) *parts = wmu ? (int)grid.x : 0;
   return hipGetLastError();
 }
 
@@ -1115,13 +1192,19 @@ static bool set_range(DevProblem &pb, int v0, int v1)
   return true;
 }
 
-hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t sweep, hipStream_t st, int v0, int v1)
+hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t sweep, hipStream_t st, int v0, int v1,
+                         int *bstat_parts)
 {
   DevProblem pb = pb0;
+  if (bstat_parts) *bstat_parts = 0;
   if (!set_range(pb, v0, v1)) return hipErrorInvalidValue;
   if (v0 == v1) return hipSuccess;
-  if (fused && zstore == 2 && (size_t)pb.NQ * pb.P * sizeof(float4) <= 65536)
-    return launch_latent_vl(pb, latent_vl_groups(pb), sweep, st);
+  if (fused && zstore == 2) {
+    // the Z-free form needs X in LDS; the WM statistic needs the whole voxel range
+    if ((size_t)pb.NQ * pb.P * sizeof(float4) > 65536) return hipErrorInvalidValue;
+    if (pb.wm_update && (v0 != 0 || v1 != pb.V)) return hipErrorInvalidValue;
+    return launch_latent_vl(pb, latent_vl_groups(pb), sweep, st, bstat_parts);
+  }
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
   const int vpw = 64 >> lshift;
@@ -1158,6 +1241,21 @@ hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
 #undef LK
+  return hipGetLastError();
+}
+
+hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st)
+{
+  if (!pb.wm_update) return hipSuccess;
+  if (parts <= 0) {
+    if (pb.Z == nullptr || pb.NQ <= 0) return hipErrorInvalidValue;
+    parts = std::min(BSTAT_SLOTS, (pb.V + 255) / 256);
+    dim3 grid((unsigned)parts, (unsigned)pb.nchains);
+#define B_(PP) hipLaunchKernelGGL(k_bstat<PP>, grid, dim3(256), 0, st, pb)
+    BSG_DISPATCH_P(pb.P, B_)
+#undef B_
+  }
+  hipLaunchKernelGGL(k_bwm, dim3(1, (unsigned)pb.nchains), dim3(256), 0, st, pb, parts, sweep);
   return hipGetLastError();
 }
 

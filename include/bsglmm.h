@@ -82,17 +82,24 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
                const unsigned char *data, const float *wm, const bsg_opts *opts, bsg_ctx **out);
 int bsg_destroy(bsg_ctx *ctx);
 
-/* ---- the sweep (mcmc.cpp:311-394, GPU branch, probit, no WM image) ---- */
-/* nsweeps full sweeps: latent, X'Z, coefficients colour 0 then 1, CAR
- * precision. The CAR-precision draw of sweep t runs on a second internal
+/* ---- the sweep (mcmc.cpp:311-394, GPU branch, probit) -------------------- */
+/* nsweeps full sweeps: latent, X'Z, WM coefficient b (when a WM image was
+ * given), coefficients colour 0 then 1, CAR precision. The CAR-precision draw of sweep t runs on a second internal
  * stream beside the latent draw of sweep t+1; the call returns with all work
  * ordered on the context's stream. */
 int bsg_sweep(bsg_ctx *ctx, int nsweeps);
 /* updateZGPU (covarGPU.cu:1625-1669) */
 int bsg_update_latent(bsg_ctx *ctx);
-/* X'(Z - b WM), the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352),
- * recomputed from Z (xtz_mode 2 already forms it inside bsg_update_latent) */
+/* X'Z, the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352),
+ * recomputed from Z (xtz_mode 2 already forms it inside bsg_update_latent).
+ * The b WM_j X'1 term of X'(Z - b WM) is applied by bsg_update_coef with the
+ * b of the sweep. */
 int bsg_compute_xtz(bsg_ctx *ctx);
+/* updateBetaGPU (covarGPU.cu:686-724): the WM coefficient
+ * b ~ N(S/W, 1/W), S = sum_j WM_j sum_i (Z_ij - x_i'b_j), W = N sum_j WM_j^2
+ * (flat prior, as the GPU path). A no-op when the context has no WM image.
+ * With z_store 2 it must follow bsg_update_latent of the same sweep. */
+int bsg_update_wm_coef(bsg_ctx *ctx);
 /* Spat_Coef_Probit for one colour (covarGPU.cu:547-560); reads X'Z from bsg_compute_xtz */
 int bsg_update_coef(bsg_ctx *ctx, int colour);
 /* updateSpatCoefPrecGPU (covarGPU.cu:1063-1112): FP64 pair reduction and a
@@ -132,8 +139,10 @@ int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *va
 int bsg_get_spat_coef(bsg_ctx *ctx, int chain, float *spat_coef);
 int bsg_set_spat_coef(bsg_ctx *ctx, int chain, const float *spat_coef);
 int bsg_get_z(bsg_ctx *ctx, int chain, float *z);
-int bsg_get_xtz(bsg_ctx *ctx, int chain, float *m);           /* float[P][TOTVOX] */
+int bsg_get_xtz(bsg_ctx *ctx, int chain, float *m);           /* X'Z, float[P][TOTVOX] */
 int bsg_set_xtz(bsg_ctx *ctx, int chain, const float *m);
+int bsg_get_wm_coef(bsg_ctx *ctx, int chain, float *b);      /* WM coefficient b */
+int bsg_set_wm_coef(bsg_ctx *ctx, int chain, float b);
 int bsg_get_prec(bsg_ctx *ctx, int chain, float *prec);       /* float[P][P] */
 int bsg_set_prec(bsg_ctx *ctx, int chain, const float *prec);
 int bsg_get_pair_sums(bsg_ctx *ctx, int chain, double *t);    /* double[P][P], last update */

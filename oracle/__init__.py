@@ -61,6 +61,8 @@ def orc():
             "orc_xtz": (None, [C.POINTER(Problem), pf, C.c_float, pd]),
             "orc_update_coef": (None, [C.POINTER(Problem), C.c_int, pd, pf, pd, C.c_uint32, pd]),
             "orc_pair_sums": (None, [C.POINTER(Problem), pf, pd]),
+            "orc_wm_stat": (None, [C.POINTER(Problem), pf, pd, pd, pd]),
+            "orc_wm_draw": (C.c_double, [C.c_double, C.c_double, pu32, C.c_uint32, C.c_uint32]),
             "orc_wishart": (None, [C.c_int, pd, C.c_int, pu32, C.c_uint32, C.c_uint32, pd, pd, pd]),
             "orc_gamma": (C.c_double, [C.c_double, pu32, C.c_uint32, C.c_uint32, C.c_uint32]),
             "orc_chol": (C.c_int, [pd, C.c_int]),
@@ -202,6 +204,19 @@ class Oracle:
         orc().orc_update_coef(C.byref(self.pb), colour, _p(m, C.c_double), _p(prec, C.c_float),
                               _p(sc, C.c_double), sweep, zp)
         return sc
+
+    def wm_stat(self, spat_coef, z):
+        """(S, W) of the WM coefficient draw from z[N][TOTVOX] (covarGPU.cu:588-683)."""
+        sc = np.ascontiguousarray(spat_coef, dtype=np.float32)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        S, W = C.c_double(), C.c_double()
+        orc().orc_wm_stat(C.byref(self.pb), _p(sc, C.c_float), _p(z, C.c_double), C.byref(S), C.byref(W))
+        return S.value, W.value
+
+    def wm_draw(self, S, W, sweep):
+        """b ~ N(S/W, 1/W) from the Philox normal of (0, sweep, BWM, chain) (covarGPU.cu:711-717)."""
+        k = (C.c_uint32 * 2)(int(self.key[0]), int(self.key[1]))
+        return orc().orc_wm_draw(C.c_double(S), C.c_double(W), k, sweep, int(self.pb.chain))
 
     def pair_sums(self, spat_coef):
         sc = np.ascontiguousarray(spat_coef, dtype=np.float32)

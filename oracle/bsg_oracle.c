@@ -332,6 +332,41 @@ void orc_xtz(const orc_problem *pb, const float *z, float b, double *m_out)
   }
 }
 
+/* covarGPU.cu:588-683 (betaMGPU, betaVGPU): the sufficient statistics of the
+ * WM coefficient, over the mask in raster order, in FP64. */
+void orc_wm_stat(const orc_problem *pb, const float *spat_coef, const double *z, double *S, double *W)
+{
+  int N = pb->nsubs, P = pb->ncovar, V = pb->totvox;
+  double s = 0.0, w = 0.0;
+  for (int c = 0; c < 2; c++) {
+    for (int q = 0; q < pb->n_col[c]; q++) {
+      int vox = pb->vox[c][q];
+      int id = pb->host_idx[vox], idsc = pb->host_idx_sc[vox];
+      double wm = (double)pb->wm[id], r = 0.0;
+      for (int i = 0; i < N; i++) {
+        r += z[(size_t)i * V + id];
+        for (int k = 0; k < P; k++)
+          r -= (double)spat_coef[(size_t)k * pb->totvoxp + idsc] * (double)pb->covar[(size_t)k * N + i];
+      }
+      s += wm * r;
+      w += wm * wm;
+    }
+  }
+  *S = s;
+  *W = (double)N * w;
+}
+
+/* covarGPU.cu:711-717: var = 1/W, mean = S var, b = mean + sqrt(var) n */
+double orc_wm_draw(double S, double W, const uint32_t key[2], uint32_t sweep, uint32_t chain)
+{
+  uint32_t r[4];
+  double n0, n1;
+  orc_draw(key, 0u, sweep, ORC_TAG_BWM, 0, chain, r);
+  orc_box_muller(orc_u01(r[0]), orc_u01(r[1]), &n0, &n1);
+  double var = 1.0 / W;
+  return S * var + sqrt(var) * n0;
+}
+
 /* covarGPU.cu:354-392 (probit coefficient update of one colour):
  *   sumV = sum of the 6 dilated-mask neighbours' beta (out-of-mask ones are 0)
  *   mean = m + Prec sumV,  var = X'X + n_j Prec

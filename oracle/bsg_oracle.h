@@ -37,7 +37,8 @@ enum {
   ORC_TAG_LATENT = 1,
   ORC_TAG_COEF = 2,
   ORC_TAG_GAMMA = 3,
-  ORC_TAG_WNORM = 4
+  ORC_TAG_WNORM = 4,
+  ORC_TAG_BWM = 5
 };
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -110,6 +111,11 @@ void orc_pair_sums(const orc_problem *pb, const float *spat_coef, double *t_out)
 void orc_wishart(int p, const double *t, int df, const uint32_t key[2], uint32_t sweep,
                  uint32_t chain, const double *chi2_inj, const double *norm_inj,
                  double *lambda_out);
+/* WM coefficient (covarGPU.cu:588-724, GPU semantics: flat prior):
+ *   S = sum_j WM_j sum_i (Z_ij - x_i'b_j),  W = N sum_j WM_j^2  (z as orc_update_z) */
+void orc_wm_stat(const orc_problem *pb, const float *spat_coef, const double *z, double *S, double *W);
+/* b = S/W + sqrt(1/W) n, n from Philox (0, sweep, ORC_TAG_BWM, chain) by Box-Muller */
+double orc_wm_draw(double S, double W, const uint32_t key[2], uint32_t sweep, uint32_t chain);
 /* Marsaglia-Tsang gamma(a,1), a >= 1, from Philox stream (row, sweep, chain) */
 double orc_gamma(double a, const uint32_t key[2], uint32_t row, uint32_t sweep, uint32_t chain);
 

@@ -126,7 +126,7 @@ def test_full_sweep_matches_oracle(nsubs, ncovar, xtz_mode):
     lam_gpu = S.get_prec()
     assert rel_err(lam_gpu, lam_orc, floor=np.abs(lam_orc).max()) < 1e-6
     assert S.sweep_index() == sweep + 1
-    # a second sweep reuses the completion counter: it must still elect one block
+    # a second sweep: the side-stream Wishart of the first must be ordered before it
     S.sweep(1)
     S.synchronize()
     assert S.sweep_index() == sweep + 2
@@ -222,3 +222,47 @@ def test_chains_are_independent_streams():
     S2.update_latent()
     O.set_chain(2)
     assert rel_err(S2.get_z(chain=2), O.update_z(sc0, 0)) < 1e-5
+
+
+@pytest.mark.parametrize("z_store,xtz_mode", [(0, 2), (0, 0), (2, 2)])
+def test_wm_coefficient_sweep(z_store, xtz_mode):
+    """With a WM image the sweep draws b between the latent draw and the
+    coefficients (mcmc.cpp:344-356): Z uses the old b, the coefficients use
+    X'(Z - b_new WM). Each step is checked against the oracle."""
+    print(f"case c1 WM z_store={z_store} xtz_mode={xtz_mode}", flush=True)
+    d = synth.generate("c1")
+    nx, ny, nz = d["grid"]
+    lay = oracle.layout(d["mask"], nx, ny, nz)
+    L = Layout(d["mask"], nx, ny, nz)
+    rng = np.random.default_rng(5)
+    wm = rng.uniform(0.0, 1.0, lay["totvox"]).astype(np.float32)
+    S = Sampler(L, d["covar"], d["data"], wm=wm, seed=SEED, xtz_mode=xtz_mode, z_store=z_store)
+    O = oracle.Oracle(lay, d["covar"], d["data"], wm=wm, seed=SEED)
+    P = S.P
+    sweep, b0 = 7, 0.3
+    sc0 = spat_coef_from_ranks(d["beta_true"], lay)
+    prec0 = (np.eye(P) * 1.5 + 0.2).astype(np.float32)
+    S.set_spat_coef(sc0)
+    S.set_prec(prec0)
+    S.set_wm_coef(b0)
+    S.set_sweep_index(sweep)
+    S.sweep(1)
+    S.synchronize()
+    z_orc = O.update_z(sc0, sweep, b=b0)
+    if z_store == 0:
+        z_gpu = S.get_z()
+        assert rel_err(z_gpu, z_orc) < 1e-5
+        z_used = z_gpu.astype(np.float64)
+    else:
+        z_used = z_orc  # Z is not kept; the oracle's draws carry the same counters
+    s_orc, w_orc = O.wm_stat(sc0, z_used)
+    b_orc = O.wm_draw(s_orc, w_orc, sweep)
+    b_gpu = S.get_wm_coef()
+    # FP32 draws and per-voxel sums against FP64: the mean S/W moves by < 1e-6
+    assert abs(b_gpu - b_orc) < 1e-6 + 1e-5 * abs(b_orc), (b_gpu, b_orc)
+    m_ref = O.xtz(z_used.astype(np.float32), b_gpu)
+    ref = O.update_coef(0, m_ref, prec0, sc0.astype(np.float64), sweep)
+    ref = O.update_coef(1, m_ref, prec0, ref, sweep)
+    assert rel_err(S.get_spat_coef(), ref) < 2e-5
+    # the statistic itself: b ~ N(S/W, 1/W) about the oracle's mean
+    assert abs(b_gpu - s_orc / w_orc) < 6.0 / np.sqrt(w_orc)

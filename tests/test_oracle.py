@@ -243,3 +243,31 @@ def test_kiss_matches_reference_stream():
     s = oracle.seed_array(f["seed"])
     assert o.orc_runiform_long_n(C.c_ulong(2 ** 64 - 1), s) == f["devseed"]
     assert list(s) == f["state_after_devseed"]
+
+
+def test_wm_stat_closed_form_and_draw_law():
+    """orc_wm_stat/orc_wm_draw (covarGPU.cu:588-724): with Z = x'b_j + c WM_j
+    exactly, S/W = c (the residual is c WM_j, so S = c N sum WM^2 = c W);
+    with a flipped sign or a wrong covariate index S/W would miss c. The draws
+    over many sweeps follow N(S/W, 1/W)."""
+    from bsglmm_amd import synth
+    d = synth.generate("c1", nsubs=12, ncovar=3)
+    nx, ny, nz = d["grid"]
+    lay = oracle.layout(d["mask"], nx, ny, nz)
+    rng = np.random.default_rng(3)
+    V, N = lay["totvox"], 12
+    wm = rng.uniform(0.1, 1.0, V).astype(np.float32)
+    O = oracle.Oracle(lay, d["covar"], d["data"], wm=wm, seed=99)
+    beta = rng.standard_normal((3, V)).astype(np.float32)
+    hi, hs = lay["host_idx"], lay["host_idx_sc"]
+    ids = np.flatnonzero(hi >= 0)
+    sc = np.zeros((3, lay["totvoxp"]), np.float32)
+    sc[:, hs[ids]] = beta[:, hi[ids]]
+    c = -0.37
+    z = (d["covar"].astype(np.float64).T @ beta.astype(np.float64)) + c * wm.astype(np.float64)[None, :]
+    S, W = O.wm_stat(sc, z)
+    assert W == pytest.approx(N * np.sum(wm.astype(np.float64) ** 2), rel=1e-12)
+    assert S / W == pytest.approx(c, rel=1e-9)
+    draws = np.array([O.wm_draw(S, W, sw) for sw in range(4000)])
+    zs = (draws - S / W) * np.sqrt(W)
+    assert st.kstest(zs, "norm").pvalue > 1e-3

@@ -86,6 +86,13 @@ def lib():
         "bsg_get_fallback_count": (C.c_int, [vp, C.POINTER(C.c_uint32)]),
         "bsg_measure_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, pd, pd]),
         "bsg_measure_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, pd, pd]),
+        "bsg_coef_means": (C.c_int, [vp, C.c_int, pd]),
+        "bsg_loglik": (C.c_int, [vp, C.c_int, C.c_int, pd]),
+        "bsg_nifti_resolve": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
+        "bsg_nifti_info": (C.c_int, [C.c_char_p, pi, pi]),
+        "bsg_nifti_read": (C.c_int, [C.c_char_p, pd, C.c_size_t]),
+        "bsg_nifti_write_f64": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, pd]),
+        "bsg_design_read": (C.c_int, [C.c_char_p, C.c_int, pi, pf, C.c_char_p, C.c_char_p]),
         "bsg_kiss": (C.c_double, [C.POINTER(C.c_ulong)]),
         "bsg_devseed": (C.c_uint64, [C.POINTER(C.c_ulong)]),
     }
@@ -256,6 +263,18 @@ class Sampler:
         m = np.ascontiguousarray(m, dtype=np.float32)
         _check(lib().bsg_set_xtz(self._h, chain, _ptr(m, C.c_float)))
 
+    def coef_means(self, chain=0):
+        """Plane means of the coefficients over the mask (alphaMean)."""
+        out = np.empty(self.P, np.float64)
+        _check(lib().bsg_coef_means(self._h, chain, _ptr(out, C.c_double)))
+        return out
+
+    def loglik(self, ntypes=1, chain=0):
+        """Per-subject log-likelihood under each subtype, float64[N][ntypes]."""
+        out = np.empty((self.N, ntypes), np.float64)
+        _check(lib().bsg_loglik(self._h, chain, int(ntypes), _ptr(out, C.c_double)))
+        return out
+
     def get_wm_coef(self, chain=0):
         v = C.c_float()
         _check(lib().bsg_get_wm_coef(self._h, chain, C.byref(v)))
@@ -332,3 +351,46 @@ def devseed(seed):
     v = lib().bsg_devseed(s)
     seed[:] = list(s)
     return v
+
+
+# ---- file formats of the reference program (include/bsglmm_io.h) ----
+def nifti_resolve(path):
+    buf = C.create_string_buffer(4096)
+    _check(lib().bsg_nifti_resolve(os.fsencode(path), buf, 4096))
+    return buf.value.decode()
+
+
+def nifti_info(path):
+    """(dims (nx, ny, nz), NIfTI datatype code) of volume 1."""
+    dims = (C.c_int * 3)()
+    dt = C.c_int()
+    _check(lib().bsg_nifti_info(os.fsencode(path), dims, C.byref(dt)))
+    return tuple(dims), dt.value
+
+
+def nifti_read(path):
+    """Volume 1 as float64[nz][ny][nx] (x fastest in memory)."""
+    (nx, ny, nz), _ = nifti_info(path)
+    out = np.empty((nz, ny, nx), np.float64)
+    _check(lib().bsg_nifti_read(os.fsencode(path), _ptr(out, C.c_double), out.size))
+    return out
+
+
+def nifti_write(path, vol):
+    """Writes float64[nz][ny][nx] as a FLOAT64 NIfTI-1 (.nii, or .nii.gz compressed)."""
+    v = np.ascontiguousarray(vol, dtype=np.float64)
+    nz, ny, nx = v.shape
+    _check(lib().bsg_nifti_write_f64(os.fsencode(path), nx, ny, nz, _ptr(v, C.c_double)))
+
+
+def design_read(path, ncovar):
+    """(covar float32[ncovar][N], covariate names, image paths) of a design file."""
+    n = C.c_int()
+    _check(lib().bsg_design_read(os.fsencode(path), int(ncovar), C.byref(n), None, None, None))
+    covar = np.empty((ncovar, n.value), np.float32)
+    names = C.create_string_buffer(64 * ncovar)
+    imgs = C.create_string_buffer(512 * n.value)
+    _check(lib().bsg_design_read(os.fsencode(path), int(ncovar), C.byref(n), _ptr(covar, C.c_float), names, imgs))
+    nm = [names.raw[64 * c:64 * (c + 1)].split(b"\0")[0].decode() for c in range(ncovar)]
+    im = [imgs.raw[512 * i:512 * (i + 1)].split(b"\0")[0].decode() for i in range(n.value)]
+    return covar, nm, im

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsglmm.so")
-SOURCES = ["bsg_layout.cpp", "bsg_ctx.cpp", "bsg_kernels.hip", "bsg_probe.hip"]
+SOURCES = ["bsg_layout.cpp", "bsg_ctx.cpp", "bsg_io.cpp", "bsg_cli.cpp", "bsg_kernels.hip", "bsg_probe.hip"]
+EXE = os.path.join(HERE, "BinCar")
 
 # Packed FP32 instructions are disabled for all device code: some of their
 # forms have given wrong results on these machines.
@@ -61,9 +62,15 @@ def build(verbose=False, jobs=None):
     if failed:
         raise RuntimeError("hipcc failed on: " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the BinCar executable: a main() over the library, found next to it at run time
+    tmp = EXE + ".tmp"
+    cmd = ["g++", "-O2", "-o", tmp, os.path.join(CSRC, "bincar_exe.cpp"), "-L", HERE, "-lbsglmm",
+           "-Wl,-rpath,$ORIGIN"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, EXE)
     return LIB
 
 

@@ -41,6 +41,8 @@ struct bsg_ctx {
   std::vector<void *> acc_allocs;
   int acc_count = 0, acc_batch_len = 1;
   int bstat_parts = 0;             // WM-statistic partials written by the last latent launch
+  double *cpart = nullptr;         // [chain][CM_BLOCKS][P] plane-mean partials
+  double *llpart = nullptr, *llout = nullptr;  // log-likelihood partials / results (lazy)
 
   void free_accum()
   {
@@ -376,6 +378,64 @@ int bsg_update_latent(bsg_ctx *ctx)
   CTXCHK(ctx);
   HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream, 0,
                             ctx->L.totvox, &ctx->bstat_parts));
+  return BSG_OK;
+}
+
+static constexpr int CM_BLOCKS = 256;
+
+int bsg_coef_means(bsg_ctx *ctx, int chain, double *mean)
+{
+  CTXCHK(ctx);
+  if (chain < 0 || chain >= ctx->C || !mean) {
+    set_error("bsg_coef_means: bad chain or NULL output");
+    return BSG_ERR_ARG;
+  }
+  const int P = ctx->P;
+  if (!ctx->cpart) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (hipMalloc((void **)&ctx->cpart, sizeof(double) * ctx->C * CM_BLOCKS * P) != hipSuccess) {
+      ctx->cpart = nullptr;
+      set_error("bsg_coef_means: hipMalloc failed");
+      return BSG_ERR_NOMEM;
+    }
+    ctx->allocs.push_back(ctx->cpart);
+  }
+  HIPCHK(bsg::launch_cmeans(ctx->pb, ctx->cpart, CM_BLOCKS, ctx->stream));
+  std::vector<double> h((size_t)CM_BLOCKS * P);
+  HIPCHK(hipMemcpyAsync(h.data(), ctx->cpart + (size_t)chain * CM_BLOCKS * P, sizeof(double) * h.size(),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < P; c++) {
+    double acc = 0.0;
+    for (int b = 0; b < CM_BLOCKS; b++) acc += h[(size_t)b * P + c];
+    mean[c] = acc / (double)ctx->L.totvox;
+  }
+  return BSG_OK;
+}
+
+int bsg_loglik(bsg_ctx *ctx, int chain, int ntypes, double *ll)
+{
+  CTXCHK(ctx);
+  if (chain < 0 || chain >= ctx->C || !ll || ntypes < 1 || ntypes > ctx->P) {
+    set_error("bsg_loglik: need a valid chain, 1 <= ntypes <= P and an output array");
+    return BSG_ERR_ARG;
+  }
+  const int P = ctx->P, N = ctx->N, C = ctx->C;
+  if (!ctx->llpart) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t tiles = (size_t)bsg::loglik_tiles(ctx->pb);
+    if (hipMalloc((void **)&ctx->llpart, sizeof(double) * C * tiles * N * P) != hipSuccess ||
+        hipMalloc((void **)&ctx->llout, sizeof(double) * C * N * P) != hipSuccess) {
+      set_error("bsg_loglik: hipMalloc failed");
+      return BSG_ERR_NOMEM;
+    }
+    ctx->allocs.push_back(ctx->llpart);
+    ctx->allocs.push_back(ctx->llout);
+  }
+  HIPCHK(bsg::launch_loglik(ctx->pb, ntypes, ctx->llpart, ctx->llout, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ll, ctx->llout + (size_t)chain * N * ntypes, sizeof(double) * N * ntypes,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return BSG_OK;
 }
 

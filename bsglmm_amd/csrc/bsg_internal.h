@@ -93,6 +93,11 @@ hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipSt
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_probe(int kind, const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st);
+// plane means: part[chain][blocks][P] FP64 block partials
+hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st);
+// per-subject log-likelihood: part [chain][tiles][N][T], out [chain][N][T]
+int loglik_tiles(const DevProblem &pb);
+hipError_t launch_loglik(const DevProblem &pb, int T, double *part, double *out, hipStream_t st);
 
 void set_error(const std::string &msg);
 

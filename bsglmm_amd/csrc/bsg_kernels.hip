@@ -1117,6 +1117,94 @@ __global__ __launch_bounds__(256) void k_accum(DevProblem pb, Accum ac, int batc
 }
 
 // ---------------------------------------------------------------------------
+// Plane means of the coefficients over the mask (alphaMean, covarGPU.cu:563-583):
+// FP64 block partials in fixed order, summed on the host.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(256) void k_cmeans(DevProblem pb, double *part)
+{
+  const int chain = blockIdx.y;
+  double acc[P];
+#pragma unroll
+  for (int c = 0; c < P; c++) acc[c] = 0.0;
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < pb.V; s += gridDim.x * 256)
+#pragma unroll
+    for (int c = 0; c < P; c++) acc[c] += (double)betc[(size_t)s * P + c];
+  block_sum_store<P>(acc, part + ((size_t)chain * gridDim.x + blockIdx.x) * P);
+}
+
+// ---------------------------------------------------------------------------
+// Per-subject log-likelihood under each subtype (compute_predictGPU with
+// ProbSDNormGPU_prediction and reduce_for_Mpredict, covarGPU.cu:1783-1812,
+// 1826-1957): for subject i and subtype t,
+//   p = Phi(b_t,j + b WM_j + sum_{c >= T} b_c,j x_c,i)  (dProbSDNormGPU, A&S 26.2.17, FP32)
+//   ll_it = sum_j y_ij log(p + 1e-35) + (1 - y_ij) log(1 - p + 1e-35)
+// A block owns a tile of 256 voxels and its threads own subjects (blocks of
+// 256); each thread keeps T FP64 sums over the tile. k_llreduce adds the tile
+// partials in tile order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float phi_as_f(float x)
+{
+  const float sg = copysignf(1.0f, x), y = fabsf(x);
+  const float t = 1.0f / fmaf(0.2316419f, y, 1.0f);
+  const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.330274429f, -1.821255978f), 1.781477937f),
+                                       -0.356563782f), 0.319381530f);
+  return (x >= 0.0f ? 1.0f : 0.0f) - sg * 0.39894228f * expf(-0.5f * y * y) * poly;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_loglik(DevProblem pb, int T, double *part)
+{
+  const int chain = blockIdx.z;
+  const int tile = blockIdx.x;
+  const int i = blockIdx.y * 256 + threadIdx.x;
+  const bool active = i < pb.N;
+  const int ii = active ? i : 0;
+  float xi[P];
+#pragma unroll
+  for (int c = 0; c < P; c++) xi[c] = pb.X[(size_t)c * pb.NS + ii];
+  const float b = pb.bwm[chain];
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  double acc[MAX_P];
+#pragma unroll
+  for (int t = 0; t < MAX_P; t++) acc[t] = 0.0;
+  const int s1 = min(pb.V, (tile + 1) * 256);
+  for (int s = tile * 256; s < s1; s++) {
+    float bs[P];
+#pragma unroll
+    for (int c = 0; c < P; c++) bs[c] = betc[(size_t)s * P + c];
+    float rest = b * pb.wm[s];
+#pragma unroll
+    for (int c = 0; c < P; c++)
+      if (c >= T) rest = fmaf(bs[c], xi[c], rest);
+    const bool y = (pb.ybits[(size_t)s * pb.NW + (ii >> 5)] >> (ii & 31)) & 1u;
+#pragma unroll
+    for (int t = 0; t < P; t++) {
+      if (t < T) {
+        const float p = phi_as_f(bs[t] + rest);
+        acc[t] += (double)(y ? logf(p + 1e-35f) : logf(1.0f - p + 1e-35f));
+      }
+    }
+  }
+  if (active) {
+    double *dst = part + (((size_t)chain * gridDim.x + tile) * pb.N + i) * T;
+    for (int t = 0; t < T; t++) dst[t] = acc[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_llreduce(int tiles, int NT, const double *part, double *out)
+{
+  const int chain = blockIdx.y;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= NT) return;
+  const double *src = part + (size_t)chain * tiles * NT + k;
+  double acc = 0.0;
+  for (int t = 0; t < tiles; t++) acc += src[(size_t)t * NT];
+  out[(size_t)chain * NT + k] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
@@ -1329,6 +1417,31 @@ hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipSt
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st)
+{
+  dim3 grid((unsigned)blocks, (unsigned)pb.nchains);
+#define L_(PP) hipLaunchKernelGGL(k_cmeans<PP>, grid, dim3(256), 0, st, pb, part)
+  BSG_DISPATCH_P(pb.P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+int loglik_tiles(const DevProblem &pb) { return (pb.V + 255) / 256; }
+
+hipError_t launch_loglik(const DevProblem &pb, int T, double *part, double *out, hipStream_t st)
+{
+  if (T < 1 || T > pb.P) return hipErrorInvalidValue;
+  const int tiles = loglik_tiles(pb);
+  dim3 grid((unsigned)tiles, (unsigned)((pb.N + 255) / 256), (unsigned)pb.nchains);
+#define L_(PP) hipLaunchKernelGGL(k_loglik<PP>, grid, dim3(256), 0, st, pb, T, part)
+  BSG_DISPATCH_P(pb.P, L_)
+#undef L_
+  const int NT = pb.N * T;
+  hipLaunchKernelGGL(k_llreduce, dim3((unsigned)((NT + 255) / 256), (unsigned)pb.nchains), dim3(256), 0, st, tiles,
+                     NT, (const double *)part, out);
   return hipGetLastError();
 }
 

@@ -135,6 +135,18 @@ int bsg_sweep_accumulate(bsg_ctx *ctx, int nsweeps, int save_every);
 int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *var, double *prb,
                   double *bwm_mean, double *mcse);
 
+/* ---- per-save summaries of the current draw ----------------------------- */
+/* alphaMean (covarGPU.cu:563-583): mean of each coefficient plane over the
+ * mask, double[P] (FP64 sums). Synchronous. */
+int bsg_coef_means(bsg_ctx *ctx, int chain, double *mean);
+/* compute_predictGPU (covarGPU.cu:1826-1957, MODEL 1): for every subject i and
+ * subtype t < ntypes, ll[i*ntypes + t] = sum_j y_ij log(p + 1e-35) +
+ * (1 - y_ij) log(1 - p + 1e-35) with p = Phi(b_t,j + b WM_j +
+ * sum_{c >= ntypes} b_c,j x_c,i), Phi by A&S 26.2.17 in FP32 as
+ * dProbSDNormGPU (covarGPU.cu:1672-1693); sums in FP64. The host forms DIC
+ * and Qhat from it (covarGPU.cu:1927-1953, mcmc.cpp:719-786). Synchronous. */
+int bsg_loglik(bsg_ctx *ctx, int chain, int ntypes, double *ll);
+
 /* ---- state exchange in the reference layouts (chain index first) ------ */
 int bsg_get_spat_coef(bsg_ctx *ctx, int chain, float *spat_coef);
 int bsg_set_spat_coef(bsg_ctx *ctx, int chain, const float *spat_coef);

@@ -1,4 +1,4 @@
-"""The C ABI library loads, exports every entry point include/bsglmm.h
+"""The C ABI library loads, exports every entry point include/*.h
 declares, and its host-only parts work without a GPU."""
 import ctypes as C
 import json
@@ -18,9 +18,13 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 
 
 def header_functions():
-    text = open(os.path.join(ROOT, "include", "bsglmm.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(bsg_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(bsg_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_function():

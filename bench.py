@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-sweeps", type=int, default=2, help="timed reference CPU sweeps")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-chains", type=int, default=32,
+                    help="chains of the batched form (0 skips the ESS comparison)")
+    ap.add_argument("--batched-iters", type=int, default=5000, help="iterations per chain, burn-in included")
+    ap.add_argument("--batched-burnin", type=int, default=1000)
     ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "k_latent_vl_c2.json"),
                     help="measured traffic and VALU counts of the dominant kernel (rocprofv3 summary)")
     return ap.parse_args()
@@ -104,6 +108,77 @@ def time_mode(L, d, mode, zstore, steps, warmup):
     S.close()
     return {"wall": wall, "lat_ms": lat_ms / steps, "span_ms": span_ms / steps,
             "kms": {k: v / n_prof for k, v in kms.items()}}
+
+
+def chain_summaries(L, d, chains, iters, burnin, nbatches=40):
+    """`chains` chains of the Z-free fused sweep in one context (every kernel
+    launch covers all chains); after burn-in, every draw goes to the device
+    accumulators. Returns the sampling wall time (burn-in excluded), the
+    per-chain batch-means ESS of every coefficient map (var / mcse^2)
+    and the per-voxel R-hat across chains (Gelman-Rubin, chain means and
+    variances). Throughput counts chain-sweeps."""
+    import numpy as np
+
+    from bsglmm_amd import Sampler
+    S = Sampler(L, d["covar"], d["data"], seed=0x5EED5EED12345678, nchains=chains, xtz_mode=2, z_store=2)
+    t0 = time.perf_counter()
+    S.sweep(burnin)
+    S.synchronize()
+    t_burn = time.perf_counter() - t0
+    n = iters - burnin
+    S.accum_reset(ntypes=1, nbatches=nbatches, batch_len=n // nbatches)
+    t0 = time.perf_counter()
+    S.sweep_accumulate(n, 1)
+    S.synchronize()
+    wall = time.perf_counter() - t0
+    P = S.P
+    means, vars_, ess = [], [], []
+    for c in range(chains):
+        o = S.accum_get(c)
+        means.append(o["mean"])
+        vars_.append(o["var"])
+        ess.append(o["var"] / np.maximum(o["mcse"][:P] ** 2, 1e-300))  # var / Var(mean)
+    S.close()
+    means, vars_, ess = np.array(means), np.array(vars_), np.array(ess)  # chains x P x V
+    rhat = None
+    if chains > 1:
+        W = vars_.mean(0)
+        B = n * means.var(0, ddof=1)
+        rhat = np.sqrt(((n - 1) / n * W + B / n) / W)
+    return {"wall": wall, "burnin_wall": t_burn, "ess": ess, "rhat": rhat, "draws": n}
+
+
+def batched_vs_plain(L, d, chains, iters, burnin):
+    import numpy as np
+    plain = chain_summaries(L, d, 1, iters, burnin)
+    many = chain_summaries(L, d, chains, iters, burnin)
+    P = plain["ess"].shape[1]
+    out = {"chains": chains, "iterations": iters, "burnin": burnin, "draws_per_chain": many["draws"],
+           "estimator": "batch-means ESS per voxel (40 batches), summed over chains; ESS/s over the sampling "
+                        "phase wall time (accumulation included, burn-in excluded)",
+           "plain_chain_sweeps_per_s": plain["draws"] / plain["wall"],
+           "batched_chain_sweeps_per_s": chains * many["draws"] / many["wall"], "maps": []}
+    ratios = []
+    for c in range(P):
+        ess_plain = plain["ess"][0, c]
+        ess_many = many["ess"][:, c].sum(0)
+        r_plain = np.median(ess_plain) / plain["wall"]
+        r_many = np.median(ess_many) / many["wall"]
+        ratios.append(r_many / r_plain)
+        m = {"coef": c, "plain_ess_per_s_median": r_plain, "batched_ess_per_s_median": r_many,
+             "ratio": r_many / r_plain,
+             "plain_ess_median": float(np.median(ess_plain)), "plain_ess_p05": float(np.percentile(ess_plain, 5)),
+             "batched_ess_per_chain_median": float(np.median(many["ess"][:, c]))}
+        if many["rhat"] is not None:
+            rh = many["rhat"][c]
+            # across-chain check of the batch-means ESS: Var(chain mean) = B / n
+            m["ess_per_chain_from_between_chain_variance_median"] = float(
+                np.median(1.0 / np.maximum((rh ** 2 - 1.0 + 1.0 / many["draws"]), 1e-12)))
+            m.update({"rhat_median": float(np.median(rh)), "rhat_p99": float(np.percentile(rh, 99)),
+                      "rhat_max": float(rh.max()), "rhat_frac_above_1_01": float(np.mean(rh > 1.01))})
+        out["maps"].append(m)
+    out["ess_per_s_ratio_min_over_maps"] = float(min(ratios))
+    return out
 
 
 def load_profile(path):
@@ -181,6 +256,8 @@ def main():
         "forms": {k: form_line(r, True, forms[k][0] == 2) for k, r in others.items()},
         "cpu_baseline": None,
     }
+    if a.batched_chains > 0:
+        out["batched"] = batched_vs_plain(L, d, a.batched_chains, a.batched_iters, a.batched_burnin)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d, a.cpu_sweeps)
     print(json.dumps(out), flush=True)

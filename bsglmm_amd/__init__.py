@@ -59,6 +59,7 @@ def lib():
         "bsg_create": (C.c_int, [vp, C.c_int, C.c_int, pf, pu8, pf, C.POINTER(_Opts), C.POINTER(vp)]),
         "bsg_destroy": (C.c_int, [vp]),
         "bsg_sweep": (C.c_int, [vp, C.c_int]),
+        "bsg_sweep_graph": (C.c_int, [vp, C.c_int, C.c_int]),
         "bsg_update_latent": (C.c_int, [vp]),
         "bsg_compute_xtz": (C.c_int, [vp]),
         "bsg_update_wm_coef": (C.c_int, [vp]),
@@ -183,6 +184,10 @@ class Sampler:
     # ---- sweep steps ----
     def sweep(self, n=1):
         _check(lib().bsg_sweep(self._h, int(n)))
+
+    def sweep_graph(self, n, per_graph=8):
+        """n sweeps replayed from a HIP graph of per_graph sweeps (same results as sweep)."""
+        _check(lib().bsg_sweep_graph(self._h, int(n), int(per_graph)))
 
     def sweep_profiled(self, n):
         ms = np.zeros(6, dtype=np.float64)

@@ -43,6 +43,11 @@ struct bsg_ctx {
   int bstat_parts = 0;             // WM-statistic partials written by the last latent launch
   double *cpart = nullptr;         // [chain][CM_BLOCKS][P] plane-mean partials
   double *llpart = nullptr, *llout = nullptr;  // log-likelihood partials / results (lazy)
+  uint32_t *sweep_words = nullptr;  // [0] = 0 (direct launches), [1] = base of a captured group
+  uint32_t sweep_stage = 0;         // host source of the base upload before graph replays
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_k = 0;
 
   void free_accum()
   {
@@ -56,6 +61,10 @@ struct bsg_ctx {
   {
     if (stream) hipStreamSynchronize(stream);
     if (stream2) hipStreamSynchronize(stream2);
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    if (graph) hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
     free_accum();
     for (void *p : allocs) hipFree(p);
     allocs.clear();
@@ -323,6 +332,8 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(pb.bstat, (size_t)C * bsg::BSTAT_SLOTS);
 
   ALLOC(pb.flags, (size_t)4);
+  ALLOC(ctx->sweep_words, (size_t)2);
+  pb.sweep_base = ctx->sweep_words;
 #undef ALLOC
   pb.nbr6 = nbr6;
   pb.ybits = yb;
@@ -345,6 +356,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemsetAsync(pb.bwm, 0, sizeof(float) * C, st));
 
   HIPCHK(hipMemsetAsync(pb.flags, 0, sizeof(uint32_t) * 4, st));
+  HIPCHK(hipMemsetAsync(ctx->sweep_words, 0, sizeof(uint32_t) * 2, st));
 
   std::vector<float> lam((size_t)C * P * P, 0.0f);
   std::vector<double> lam64((size_t)C * P * P, 0.0);
@@ -497,42 +509,105 @@ static int accum_launch(bsg_ctx *ctx, hipStream_t st)
 // has to finish before the colour-0 update of sweep t+1, so it overlaps the
 // latent draw of sweep t+1. Each call ends joined on `stream`.
 // lat_ev (2 per sweep, optional) brackets each latent launch for timing.
-static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_every = 0)
+// One sweep of problem pb, whose Philox sweep word is *pb.sweep_base + off.
+// lat_ev2 (optional) brackets the latent launch.
+static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, hipEvent_t *lat_ev2,
+                         bool &wish_pending)
 {
   hipStream_t sa = ctx->stream, sb = ctx->stream2;
   const bool fused = ctx->opts.xtz_mode == 2;
   const int V = ctx->L.totvox;
-  const bool two_colours = ctx->L.n[1] > 0;
+  if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[0], sa));
+  int parts = 0;
+  HIPCHK(bsg::launch_latent(pb, fused, ctx->opts.z_store, off, sa, 0, V, &parts));
+  if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[1], sa));
+  if (!fused) HIPCHK(bsg::launch_xtz(pb, ctx->opts.xtz_mode, sa, 0, V));
+  HIPCHK(bsg::launch_bwm(pb, parts, off, sa));  // WM coefficient (mcmc.cpp:344-345)
+  if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
+  HIPCHK(bsg::launch_coef(pb, 0, false, off, sa));
+  if (ctx->L.n[1] > 0) {
+    HIPCHK(bsg::launch_coef(pb, 1, true, off, sa));  // colour 1 + pair partials
+    HIPCHK(hipEventRecord(ctx->e_c0, sa));
+    HIPCHK(hipStreamWaitEvent(sb, ctx->e_c0, 0));
+    HIPCHK(bsg::launch_wishart(pb, true, off, sb));
+    HIPCHK(hipEventRecord(ctx->e_c1, sb));
+    wish_pending = true;
+  } else {
+    HIPCHK(bsg::launch_pairs(pb, sa));
+    HIPCHK(bsg::launch_wishart(pb, false, off, sa));
+  }
+  return BSG_OK;
+}
+
+static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_every = 0)
+{
+  hipStream_t sa = ctx->stream;
   bool wish_pending = false;
   for (int i = 0; i < nsweeps; i++) {
-    const uint32_t sw = ctx->sweep;
-    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i], sa));
-    int parts = 0;
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, sa, 0, V, &parts));
-    if (lat_ev) HIPCHK(hipEventRecord(lat_ev[2 * i + 1], sa));
-    if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, sa, 0, V));
-    HIPCHK(bsg::launch_bwm(ctx->pb, parts, sw, sa));  // WM coefficient (mcmc.cpp:344-345)
-    if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
-    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, sa));
-    if (two_colours) {
-      HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, sa));  // colour 1 + pair partials
-      HIPCHK(hipEventRecord(ctx->e_c0, sa));
-      HIPCHK(hipStreamWaitEvent(sb, ctx->e_c0, 0));
-      HIPCHK(bsg::launch_wishart(ctx->pb, true, sw, sb));
-      HIPCHK(hipEventRecord(ctx->e_c1, sb));
-      wish_pending = true;
-    } else {
-      HIPCHK(bsg::launch_pairs(ctx->pb, sa));
-      HIPCHK(bsg::launch_wishart(ctx->pb, false, sw, sa));
-    }
-    ctx->sweep = sw + 1;
+    const int rc = enqueue_sweep(ctx, ctx->pb, ctx->sweep, lat_ev ? lat_ev + 2 * i : nullptr, wish_pending);
+    if (rc != BSG_OK) return rc;
+    ctx->sweep++;
     if (accum_every > 0 && (i + 1) % accum_every == 0) {
-      const int rc = accum_launch(ctx, sa);  // reads only the coefficients
-      if (rc != BSG_OK) return rc;
+      const int rc2 = accum_launch(ctx, sa);  // reads only the coefficients
+      if (rc2 != BSG_OK) return rc2;
     }
   }
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // join
   return BSG_OK;
+}
+
+// A HIP graph of k sweeps, captured once per k: the kernels read their sweep
+// word as base + 0..k-1 from a device word that the graph's last node
+// advances by k, so one executable graph serves every group of k sweeps.
+static int build_graph(bsg_ctx *ctx, int k)
+{
+  if (ctx->graph_exec && ctx->graph_k == k) return BSG_OK;
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+  hipStream_t sa = ctx->stream;
+  bsg::DevProblem pg = ctx->pb;
+  pg.sweep_base = ctx->sweep_words + 1;
+  HIPCHK(hipStreamBeginCapture(sa, hipStreamCaptureModeThreadLocal));
+  bool wish_pending = false;
+  int rc = BSG_OK;
+  for (int i = 0; i < k && rc == BSG_OK; i++) rc = enqueue_sweep(ctx, pg, (uint32_t)i, nullptr, wish_pending);
+  if (rc == BSG_OK && wish_pending && hipStreamWaitEvent(sa, ctx->e_c1, 0) != hipSuccess) rc = BSG_ERR_HIP;
+  if (rc == BSG_OK && bsg::launch_tick(ctx->sweep_words + 1, (uint32_t)k, sa) != hipSuccess) rc = BSG_ERR_HIP;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(sa, &g);
+  if (rc != BSG_OK || ec != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    if (rc == BSG_OK) set_error(std::string("graph capture: ") + hipGetErrorString(ec));
+    return rc == BSG_OK ? BSG_ERR_HIP : rc;
+  }
+  ctx->graph = g;
+  HIPCHK(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+  ctx->graph_k = k;
+  return BSG_OK;
+}
+
+int bsg_sweep_graph(bsg_ctx *ctx, int nsweeps, int sweeps_per_graph)
+{
+  CTXCHK(ctx);
+  if (nsweeps < 0 || sweeps_per_graph < 1 || sweeps_per_graph > 256) {
+    set_error("bsg_sweep_graph: need nsweeps >= 0 and 1 <= sweeps_per_graph <= 256");
+    return BSG_ERR_ARG;
+  }
+  const int k = sweeps_per_graph, q = nsweeps / k;
+  if (q > 0) {
+    int rc = build_graph(ctx, k);
+    if (rc != BSG_OK) return rc;
+    ctx->sweep_stage = ctx->sweep;
+    HIPCHK(hipMemcpyAsync(ctx->sweep_words + 1, &ctx->sweep_stage, sizeof(uint32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    for (int j = 0; j < q; j++) HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    ctx->sweep += (uint32_t)(q * k);
+    // the staging word is read by the copy above: keep it stable until then
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  return sweep_run(ctx, nsweeps - q * k, nullptr);
 }
 
 int bsg_sweep(bsg_ctx *ctx, int nsweeps)

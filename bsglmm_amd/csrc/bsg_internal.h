@@ -62,6 +62,10 @@ struct DevProblem {
   float *bwm;              // [chain]  WM coefficient b
   uint32_t *flags;         // [0] FP64 refactors, [1] non-SPD voxels
   int vb, ve;              // voxel range [vb, ve) of a latent / X'Z launch
+  // Philox sweep word = *sweep_base + the kernel's sweep argument. Direct
+  // launches point at a word holding 0 and pass the absolute index; a
+  // captured graph of K sweeps passes 0..K-1 and ends with k_tick(+K).
+  const uint32_t *sweep_base;
   // WM coefficient b (covarGPU.cu:588-724): drawn each sweep when wm_update
   int wm_update;
   double wm2;              // N sum_j WM_j^2
@@ -93,6 +97,7 @@ hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipSt
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_probe(int kind, const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st);
+hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st);
 // plane means: part[chain][blocks][P] FP64 block partials
 hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st);
 // per-subject log-likelihood: part [chain][tiles][N][T], out [chain][N][T]

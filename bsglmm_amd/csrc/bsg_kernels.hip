@@ -318,6 +318,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 template <int P, bool XREG, bool FUSED, int ZS>
 __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps, int viter, uint32_t sweep)
 {
+  sweep += *pb.sweep_base;
   static_assert(ZS != 2 || FUSED, "Z can only be dropped when X'Z is fused");
   const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
@@ -442,6 +443,7 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
 template <int P, int G, bool WMU>
 __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep)
 {
+  sweep += *pb.sweep_base;
   extern __shared__ float4 xs[];  // [NQ][P]
   constexpr int VPW = 64 / G;
   const int chain = blockIdx.y;
@@ -569,6 +571,7 @@ __global__ __launch_bounds__(256) void k_bstat(DevProblem pb)
 
 __global__ __launch_bounds__(256) void k_bwm(DevProblem pb, int nparts, uint32_t sweep)
 {
+  sweep += *pb.sweep_base;
   const int chain = blockIdx.y;
   __shared__ double S[1];
   sum_partials<1>(nparts, pb.bstat + (size_t)chain * BSTAT_SLOTS, S);
@@ -839,6 +842,7 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
 template <int P, bool SIDE>
 __global__ __launch_bounds__(256) void k_wishart(DevProblem pb, uint32_t sweep)
 {
+  sweep += *pb.sweep_base;
   constexpr int NE = P * (P + 1) / 2;
   const int chain = blockIdx.x;
   const int nparts = SIDE ? (pb.V - pb.n0 + 255) / 256 : PAIR_BLOCKS;
@@ -923,6 +927,7 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
 template <int P, bool PREC>
 __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep)
 {
+  sweep += *pb.sweep_base;
   constexpr int NE = P * (P + 1) / 2;
   const int chain = blockIdx.y;
   const int q = blockIdx.x * 256 + threadIdx.x;
@@ -1442,6 +1447,18 @@ hipError_t launch_loglik(const DevProblem &pb, int T, double *part, double *out,
   const int NT = pb.N * T;
   hipLaunchKernelGGL(k_llreduce, dim3((unsigned)((NT + 255) / 256), (unsigned)pb.nchains), dim3(256), 0, st, tiles,
                      NT, (const double *)part, out);
+  return hipGetLastError();
+}
+
+// Advances the device sweep base at the end of a captured group of sweeps.
+__global__ void k_tick(uint32_t *base, uint32_t k)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *base += k;
+}
+
+hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, base, k);
   return hipGetLastError();
 }
 

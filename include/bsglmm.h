@@ -88,6 +88,11 @@ int bsg_destroy(bsg_ctx *ctx);
  * stream beside the latent draw of sweep t+1; the call returns with all work
  * ordered on the context's stream. */
 int bsg_sweep(bsg_ctx *ctx, int nsweeps);
+/* bsg_sweep through a HIP graph that holds sweeps_per_graph sweeps (captured
+ * on first use, replayed for every full group; the remainder runs directly).
+ * Same updates and random counters as bsg_sweep, so the same results.
+ * Returns after the replays drained. */
+int bsg_sweep_graph(bsg_ctx *ctx, int nsweeps, int sweeps_per_graph);
 /* updateZGPU (covarGPU.cu:1625-1669) */
 int bsg_update_latent(bsg_ctx *ctx);
 /* X'Z, the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352),

@@ -266,3 +266,30 @@ def test_wm_coefficient_sweep(z_store, xtz_mode):
     assert rel_err(S.get_spat_coef(), ref) < 2e-5
     # the statistic itself: b ~ N(S/W, 1/W) about the oracle's mean
     assert abs(b_gpu - s_orc / w_orc) < 6.0 / np.sqrt(w_orc)
+
+
+@pytest.mark.parametrize("z_store,xtz_mode,wm", [(2, 2, False), (0, 1, True)])
+def test_graph_replays_equal_direct_sweeps(z_store, xtz_mode, wm):
+    """Sweeps replayed from a captured HIP graph (device sweep base advanced by
+    the graph's last node) give bit-identical state to direct launches, over
+    several replays with direct sweeps and host reads between them."""
+    print(f"case c1 graph z_store={z_store} xtz_mode={xtz_mode} wm={wm}", flush=True)
+    d = synth.generate("c1")
+    nx, ny, nz = d["grid"]
+    L = Layout(d["mask"], nx, ny, nz)
+    w = np.random.default_rng(1).uniform(0, 1, L.totvox).astype(np.float32) if wm else None
+    A = Sampler(L, d["covar"], d["data"], wm=w, seed=SEED, nchains=2, xtz_mode=xtz_mode, z_store=z_store)
+    G = Sampler(L, d["covar"], d["data"], wm=w, seed=SEED, nchains=2, xtz_mode=xtz_mode, z_store=z_store)
+    for step in (11, 3, 8, 16):  # 11 = one group of 8 + 3 direct; then a direct batch, then replays again
+        A.sweep(step)
+        if step == 3:
+            G.sweep(step)
+        else:
+            G.sweep_graph(step, 8)
+        A.synchronize()
+        G.synchronize()
+        assert G.sweep_index() == A.sweep_index()
+        for c in range(2):
+            assert np.array_equal(G.get_spat_coef(c), A.get_spat_coef(c))
+            assert np.array_equal(G.get_prec(c), A.get_prec(c))
+            assert G.get_wm_coef(c) == A.get_wm_coef(c)

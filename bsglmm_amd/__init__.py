@@ -32,7 +32,7 @@ class BsgError(RuntimeError):
 class _Opts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("nchains", C.c_int), ("xtz_mode", C.c_int),
                 ("df_offset", C.c_int), ("trunc_eps", C.c_float), ("pivot_min", C.c_float),
-                ("z_store", C.c_int)]
+                ("z_store", C.c_int), ("latent_method", C.c_int)]
 
 
 def _ptr(a, ctype):
@@ -160,7 +160,7 @@ class Sampler:
     KERNELS = ("latent", "xtz", "coef0", "coef1", "pairs", "wishart")
 
     def __init__(self, layout, covar, data, wm=None, seed=0, nchains=1, xtz_mode=0,
-                 df_offset=19, trunc_eps=1e-6, pivot_min=1e-6, z_store=0):
+                 df_offset=19, trunc_eps=1e-6, pivot_min=1e-6, z_store=0, latent_method=0):
         covar = np.ascontiguousarray(covar, dtype=np.float32)
         data = np.ascontiguousarray(data, dtype=np.uint8)
         self.P, self.N = covar.shape
@@ -172,6 +172,7 @@ class Sampler:
         o.seed, o.nchains, o.xtz_mode = int(seed), int(nchains), int(xtz_mode)
         o.df_offset, o.trunc_eps, o.pivot_min = int(df_offset), float(trunc_eps), float(pivot_min)
         o.z_store = int(z_store)
+        o.latent_method = int(latent_method)
         self.nchains = int(nchains)
         wmp = None
         if wm is not None:

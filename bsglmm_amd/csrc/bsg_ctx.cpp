@@ -175,6 +175,7 @@ void bsg_opts_default(bsg_opts *o)
   o->trunc_eps = 1e-6f;
   o->pivot_min = 1e-6f;
   o->z_store = 0;
+  o->latent_method = 0;
 }
 
 int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *covar,
@@ -210,6 +211,10 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   }
   if (o.z_store < 0 || o.z_store > 2 || (o.z_store == 2 && o.xtz_mode != 2)) {
     set_error("bsg_create: z_store must be 0, 1, or 2 with xtz_mode 2");
+    return BSG_ERR_ARG;
+  }
+  if (o.latent_method < 0 || o.latent_method > 1 || (o.latent_method == 1 && o.z_store != 2)) {
+    set_error("bsg_create: latent_method must be 0, or 1 with z_store 2");
     return BSG_ERR_ARG;
   }
   int ndev = 0;
@@ -292,6 +297,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.ve = V;
   // b is drawn each sweep only when a WM image is given (mcmc.cpp:344-345, UPDATE_WM)
   pb.wm_update = wm ? 1 : 0;
+  pb.latent_method = o.latent_method;
   {
     double w2 = 0.0;
     for (int s = 0; s < V; s++) w2 += (double)wms[s] * (double)wms[s];

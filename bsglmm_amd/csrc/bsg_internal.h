@@ -17,6 +17,7 @@ constexpr uint32_t TAG_COEF = 2;    // index = voxel, sub = group of 4 normals
 constexpr uint32_t TAG_GAMMA = 3;   // index = Bartlett row, sub = trial
 constexpr uint32_t TAG_WNORM = 4;   // index = row * P + column
 constexpr uint32_t TAG_BWM = 5;     // index = 0: normal of the WM coefficient draw
+constexpr uint32_t TAG_FALLBACK = 6; // index = voxel * NQ + quad, sub = subject in quad: exact redraw
 
 constexpr int PAIR_BLOCKS = 512;    // fixed partial count per chain: deterministic sums
 constexpr int BSTAT_SLOTS = 4096;   // per-chain partial slots of the WM-coefficient statistic
@@ -66,6 +67,7 @@ struct DevProblem {
   // launches point at a word holding 0 and pass the absolute index; a
   // captured graph of K sweeps passes 0..K-1 and ends with k_tick(+K).
   const uint32_t *sweep_base;
+  int latent_method;       // Z-free form: 0 inversion for every draw, 1 Box-Muller candidate + exact fallback
   // WM coefficient b (covarGPU.cu:588-724): drawn each sweep when wm_update
   int wm_update;
   double wm2;              // N sum_j WM_j^2

@@ -430,21 +430,60 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
 }
 
 // ---------------------------------------------------------------------------
+// Exact single draw by inversion (the arithmetic of latent_quad for one
+// subject): W ~ N(0,1) truncated below at a = eps - s mu, Z = mu + s W.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float latent_one(float mu, bool y, float u, float eps)
+{
+  const float s = y ? 1.0f : -1.0f;
+  const float al = fmaf(-s, mu, eps);
+  const float qt = q_tail(fabsf(al));
+  const float uq = u * qt;
+  const float pn = u - uq;
+  const bool neg = al < 0.0f;
+  const bool flip = neg && pn > 0.5f;
+  const float arg = neg ? (flip ? (1.0f - u) + uq : pn) : uq;
+  float w = q_inv(arg);
+  w = flip ? -w : w;
+  w = (w < al) ? al : w;
+  if (al >= 8.0f) w = fmaxf(tail_w(al, u), al);
+  return fmaf(s, w, mu);
+}
+
+// ---------------------------------------------------------------------------
 // k_latent_vl: fused latent draw + X'Z with voxels across lanes, for the form
 // that does not keep Z. A wave holds 64/G voxels; the G lanes of a voxel take
 // G contiguous ranges of its subject quads. X is staged once per block in LDS
 // as [q][c] float4 and read at addresses shared by all lanes of a subject
 // group (broadcast), so the per-voxel inputs live in registers, the covariate
 // products need no cross-lane traffic, and the G partial sums of m_j meet in
-// log2(G) lane exchanges once per voxel. Counters and draw arithmetic are
-// those of k_latent, so the draws are bit-identical; only the summation order
-// of m_j differs. Blocks walk voxel tiles of 4 * 64/G voxels, grid-strided.
+// log2(G) lane exchanges once per voxel. Blocks walk voxel tiles of
+// 4 * 64/G voxels, grid-strided.
+//
+// Draws: BMF = false draws every subject by inversion (latent_quad, the
+// draws of k_latent bit for bit). BMF = true is accept-or-fallback, also
+// exact in distribution. The four uniforms of the
+// quad's Philox block give two Box-Muller pairs (R = sqrt(-2 ln u0),
+// W0 = R cos 2 pi u1, W1 = R sin 2 pi u1; likewise u2, u3), hardware
+// v_log/v_sin/v_cos (1e-7 relative / absolute). Z = mu + W is kept when it
+// falls in the interval of y (eps, 500) or (-500, -eps). Otherwise the draw
+// is queued in the lane's own LDS list and later redrawn exactly, by
+// inversion, from an independent Philox block (tag FALLBACK, sub = subject
+// in quad). An accepted candidate is a TN draw and a rejected one is
+// replaced by an independent TN draw, so every Z is TN. About 93 % of draws
+// take the cheap path (y = 0 draws reject when mu + W > -eps); the lists are
+// drained in a wave-uniform loop, each lane its own queue in subject order,
+// so the sums are deterministic.
 // ---------------------------------------------------------------------------
-template <int P, int G, bool WMU>
+constexpr int VL_QUEUE = 32;  // fallback slots per lane (uint16 subject index)
+
+template <int P, int G, bool WMU, bool BMF>
 __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep)
 {
   sweep += *pb.sweep_base;
-  extern __shared__ float4 xs[];  // [NQ][P]
+  extern __shared__ float4 xs[];  // [NQ][P], then (BMF) the fallback queues
+  uint16_t *queue = reinterpret_cast<uint16_t *>(xs + (size_t)pb.NQ * P) + threadIdx.x * VL_QUEUE;
+  const float *xsf = reinterpret_cast<const float *>(xs);  // x_c,i = xsf[(q P + c) 4 + l]
   constexpr int VPW = 64 / G;
   const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
@@ -481,7 +520,34 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
 #pragma unroll
     for (int c = 0; c < P; c++) acc[c] = 0.0f;
     float zs = 0.0f;
+    int nq = 0;  // queued fallback draws of this lane
+    // exact redraw of every queued subject, lanes in step, each its own queue
+    auto drain = [&]() {
+      for (int k = 0; __ballot(k < nq) != 0; k++) {
+        if (k < nq) {
+          const int i = queue[k];
+          const int q = i >> 2, l = i & 3;
+          float xv[P];
+          float mu = bw;
+#pragma unroll
+          for (int c = 0; c < P; c++) {
+            xv[c] = xsf[(q * P + c) * 4 + l];
+            mu = fmaf(bs[c], xv[c], mu);
+          }
+          const bool y = (yrow[i >> 5] >> (i & 31)) & 1u;
+          uint32_t rr[4];
+          philox10(cbase + (uint32_t)q, sweep, TAG_FALLBACK | ((uint32_t)l << 4), (uint32_t)chain, pb.key0,
+                   pb.key1, rr);
+          const float z = latent_one(mu, y, u01f(rr[0]), eps);
+#pragma unroll
+          for (int c = 0; c < P; c++) acc[c] = fmaf(xv[c], z, acc[c]);
+          if (WMU) zs += z;
+        }
+      }
+      nq = 0;
+    };
     for (int q = q0; q < q1; q++) {
+      if (BMF && __ballot(nq > VL_QUEUE - 4) != 0) drain();  // room for this quad's four
       float4 x[P];
 #pragma unroll
       for (int c = 0; c < P; c++) x[c] = xs[q * P + c];
@@ -497,8 +563,34 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
       uint32_t rr[4];
       philox10(cbase + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain, pb.key0, pb.key1, rr);
       float z[4];
-      latent_quad(mu, yw, rr, eps, z);
-      // X is zero past N, so padding subjects add nothing to m_j
+      const int i0 = 4 * q;
+      if (!BMF) {
+        latent_quad(mu, yw, rr, eps, z);  // X is zero past N: padding subjects add nothing to m_j
+        if (WMU && i0 + 3 >= pb.N) {
+#pragma unroll
+          for (int l = 1; l < 4; l++) z[l] = (i0 + l < pb.N) ? z[l] : 0.0f;
+        }
+      } else {
+      float w[4];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        // R = sqrt(-2 ln u) = sqrt(-2 ln2 log2 u); angle 2 pi u in revolutions for v_sin/v_cos
+        const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01f(rr[2 * h])));
+        const float ang = u01f(rr[2 * h + 1]);
+        w[2 * h] = rad * __builtin_amdgcn_cosf(ang);
+        w[2 * h + 1] = rad * __builtin_amdgcn_sinf(ang);
+      }
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        const float zc = mu[l] + w[l];
+        const bool yl = (yw >> l) & 1u;
+        const bool ok = yl ? (zc > eps && zc < 500.0f) : (zc < -eps && zc > -500.0f);
+        const bool live = i0 + l < pb.N;  // padding subjects are neither kept nor queued
+        z[l] = (ok && live) ? zc : 0.0f;
+        queue[nq] = (uint16_t)(i0 + l);  // kept only if the count advances
+        nq += (!ok && live) ? 1 : 0;
+      }
+      }
 #pragma unroll
       for (int c = 0; c < P; c++) {
         acc[c] = fmaf(x[c].x, z[0], acc[c]);
@@ -506,12 +598,9 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
         acc[c] = fmaf(x[c].z, z[2], acc[c]);
         acc[c] = fmaf(x[c].w, z[3], acc[c]);
       }
-      if (WMU) {
-        const int i0 = 4 * q;
-        zs += z[0] + ((i0 + 1 < pb.N) ? z[1] : 0.0f) + ((i0 + 2 < pb.N) ? z[2] : 0.0f) +
-              ((i0 + 3 < pb.N) ? z[3] : 0.0f);
-      }
+      if (WMU) zs += (z[0] + z[1]) + (z[2] + z[3]);
     }
+    if (BMF) drain();
 #pragma unroll
     for (int off = VPW; off < 64; off <<= 1) {
 #pragma unroll
@@ -1247,18 +1336,28 @@ int latent_vl_groups(const DevProblem &pb)
   return 8;
 }
 
+static size_t latent_vl_lds(const DevProblem &pb)
+{
+  return (size_t)pb.NQ * pb.P * sizeof(float4) + (pb.latent_method == 1 ? 256 * VL_QUEUE * sizeof(uint16_t) : 0);
+}
+
 static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st, int *parts)
 {
-  const size_t lds = (size_t)pb.NQ * pb.P * sizeof(float4);
+  const size_t lds = latent_vl_lds(pb);
   const int vpb = 4 * (64 / G);  // voxels per block tile
   const long long tiles = ((long long)(pb.ve - pb.vb) + vpb - 1) / vpb;
   const long long per = (tiles + BSTAT_SLOTS - 1) / BSTAT_SLOTS;  // tiles per block, equal shares
   dim3 grid((unsigned)((tiles + per - 1) / per), (unsigned)pb.nchains);
-  const bool wmu = pb.wm_update != 0;
-#define VK(PP, GG)                                                                                    \
-  do {                                                                                                \
-    if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, GG, true>), grid, dim3(256), lds, st, pb, sweep);  \
-    else hipLaunchKernelGGL((k_latent_vl<PP, GG, false>), grid, dim3(256), lds, st, pb, sweep);     \
+  const bool wmu = pb.wm_update != 0, bmf = pb.latent_method == 1;
+#define VK(PP, GG)                                                                                       \
+  do {                                                                                                   \
+    if (bmf) {                                                                                           \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, GG, true, true>), grid, dim3(256), lds, st, pb, sweep);  \
+      else hipLaunchKernelGGL((k_latent_vl<PP, GG, false, true>), grid, dim3(256), lds, st, pb, sweep);     \
+    } else {                                                                                             \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, GG, true, false>), grid, dim3(256), lds, st, pb, sweep); \
+      else hipLaunchKernelGGL((k_latent_vl<PP, GG, false, false>), grid, dim3(256), lds, st, pb, sweep);    \
+    }                                                                                                    \
   } while (0)
 #define V_(PP)                         \
   do {                                 \
@@ -1294,7 +1393,7 @@ hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t
   if (v0 == v1) return hipSuccess;
   if (fused && zstore == 2) {
     // the Z-free form needs X in LDS; the WM statistic needs the whole voxel range
-    if ((size_t)pb.NQ * pb.P * sizeof(float4) > 65536) return hipErrorInvalidValue;
+    if (latent_vl_lds(pb) > 65536) return hipErrorInvalidValue;
     if (pb.wm_update && (v0 != 0 || v1 != pb.V)) return hipErrorInvalidValue;
     return launch_latent_vl(pb, latent_vl_groups(pb), sweep, st, bstat_parts);
   }

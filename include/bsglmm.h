@@ -71,6 +71,9 @@ typedef struct {
   float pivot_min;      /* FP32 Cholesky pivot below this -> FP64 refactor of that voxel */
   int z_store;          /* Z in HBM: 0 stored, 1 stored with streaming (nontemporal) stores,
                            2 not kept (xtz_mode 2 only; the probit sweep needs Z only for X'Z) */
+  int latent_method;    /* z_store 2 only: 0 every draw by inversion (the draws of z_store 0);
+                           1 Box-Muller candidate kept inside the interval of y, exact
+                           inversion redraw otherwise (an exact sampler with other draws) */
 } bsg_opts;
 
 void bsg_opts_default(bsg_opts *opts);

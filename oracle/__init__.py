@@ -57,7 +57,7 @@ def orc():
             "orc_latent_draw": (C.c_double, [C.c_double, C.c_int, C.c_double, C.c_double]),
             "orc_layout_counts": (None, [pu8, C.c_int, C.c_int, C.c_int, pi]),
             "orc_layout_fill": (None, [pu8, C.c_int, C.c_int, C.c_int, pi, pi, pi, pi, pu8, pu8, pi]),
-            "orc_update_z": (None, [C.POINTER(Problem), pf, C.c_float, C.c_double, C.c_uint32, pd]),
+            "orc_update_z": (None, [C.POINTER(Problem), pf, C.c_float, C.c_double, C.c_uint32, C.c_int, pd]),
             "orc_xtz": (None, [C.POINTER(Problem), pf, C.c_float, pd]),
             "orc_update_coef": (None, [C.POINTER(Problem), C.c_int, pd, pf, pd, C.c_uint32, pd]),
             "orc_pair_sums": (None, [C.POINTER(Problem), pf, pd]),
@@ -181,10 +181,12 @@ class Oracle:
     def set_chain(self, chain):
         self.pb.chain = chain
 
-    def update_z(self, spat_coef, sweep, b=0.0, eps=1e-6):
+    def update_z(self, spat_coef, sweep, b=0.0, eps=1e-6, method=0):
+        """method 0: inversion for every draw (k_latent); 1: Box-Muller candidate
+        with an exact fallback (k_latent_vl, the form without Z)."""
         sc = np.ascontiguousarray(spat_coef, dtype=np.float32)
         z = np.empty((self.N, self.V), np.float64)
-        orc().orc_update_z(C.byref(self.pb), _p(sc, C.c_float), b, eps, sweep, _p(z, C.c_double))
+        orc().orc_update_z(C.byref(self.pb), _p(sc, C.c_float), b, eps, sweep, int(method), _p(z, C.c_double))
         return z
 
     def xtz(self, z, b=0.0):

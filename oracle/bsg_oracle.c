@@ -290,7 +290,7 @@ void orc_chol_invert(int n, double *h)
 /* One sweep in the reference layout.                                         */
 /* ------------------------------------------------------------------------- */
 void orc_update_z(const orc_problem *pb, const float *spat_coef, float b, double eps,
-                  uint32_t sweep, double *z_out)
+                  uint32_t sweep, int method, double *z_out)
 {
   int N = pb->nsubs, P = pb->ncovar, nq = (N + 3) / 4;
   for (int c = 0; c < 2; c++) {
@@ -302,6 +302,11 @@ void orc_update_z(const orc_problem *pb, const float *spat_coef, float b, double
       for (int g = 0; g < nq; g++) {
         uint32_t r[4];
         orc_draw(pb->key, s * (uint32_t)nq + (uint32_t)g, sweep, ORC_TAG_LATENT, 0, pb->chain, r);
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        if (method == 1) {  /* Box-Muller pairs (u0, u1), (u2, u3) */
+          orc_box_muller(orc_u01(r[0]), orc_u01(r[1]), &w[0], &w[1]);
+          orc_box_muller(orc_u01(r[2]), orc_u01(r[3]), &w[2], &w[3]);
+        }
         for (int l = 0; l < 4; l++) {
           int i = 4 * g + l;
           if (i >= N)
@@ -310,7 +315,23 @@ void orc_update_z(const orc_problem *pb, const float *spat_coef, float b, double
           for (int k = 0; k < P; k++)
             mu += (double)spat_coef[(size_t)k * pb->totvoxp + idsc] * (double)pb->covar[(size_t)k * N + i];
           int y = pb->data[(size_t)i * pb->totvox + id] == 1;
-          z_out[(size_t)i * pb->totvox + id] = orc_latent_draw(mu, y, orc_u01(r[l]), eps);
+          double z;
+          if (method == 1) {
+            /* keep the candidate inside the interval of y, else redraw exactly
+               from the independent block (FALLBACK, sub = l) */
+            double zc = mu + w[l];
+            int ok = y ? (zc > eps && zc < 500.0) : (zc < -eps && zc > -500.0);
+            if (ok) {
+              z = zc;
+            } else {
+              uint32_t f[4];
+              orc_draw(pb->key, s * (uint32_t)nq + (uint32_t)g, sweep, ORC_TAG_FALLBACK, (uint32_t)l, pb->chain, f);
+              z = orc_latent_draw(mu, y, orc_u01(f[0]), eps);
+            }
+          } else {
+            z = orc_latent_draw(mu, y, orc_u01(r[l]), eps);
+          }
+          z_out[(size_t)i * pb->totvox + id] = z;
         }
       }
     }

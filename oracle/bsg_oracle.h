@@ -38,7 +38,8 @@ enum {
   ORC_TAG_COEF = 2,
   ORC_TAG_GAMMA = 3,
   ORC_TAG_WNORM = 4,
-  ORC_TAG_BWM = 5
+  ORC_TAG_BWM = 5,
+  ORC_TAG_FALLBACK = 6
 };
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -92,9 +93,13 @@ typedef struct {
   uint32_t chain;
 } orc_problem;
 
-/* latent update; Z_out[i*TOTVOX + hostIdx] (double); b = WM coefficient */
+/* latent update; Z_out[i*TOTVOX + hostIdx] (double); b = WM coefficient.
+ * method 0: every draw by inversion from uniform l of the quad's block.
+ * method 1: Box-Muller candidates from the quad's block, (u0,u1) and (u2,u3);
+ * a candidate outside the interval of y is redrawn by inversion from
+ * block (FALLBACK, sub = l). Both are exact truncated-normal samplers. */
 void orc_update_z(const orc_problem *pb, const float *spat_coef, float b, double eps,
-                  uint32_t sweep, double *z_out);
+                  uint32_t sweep, int method, double *z_out);
 /* M[c*TOTVOX + hostIdx] = sum_i covar[c,i] (Z[i,idx] - b WM[idx]) */
 void orc_xtz(const orc_problem *pb, const float *z, float b, double *m_out);
 /* coefficient update of one colour, reference math (chol, invert, chol).

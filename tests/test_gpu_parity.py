@@ -149,18 +149,20 @@ def test_z_store_streaming():
 
 # (N, P) cases span every lane split of the voxel-per-lane kernel:
 # N=5 one lane per voxel, N=37/50 two, N=300 four, N=600 eight
+@pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("nsubs,ncovar,nchains", [(5, 2, 1), (37, 8, 1), (None, None, 2), (300, 4, 1),
                                                   (600, 3, 1), (140, 1, 1)])
-def test_z_free_fused_sweep(nsubs, ncovar, nchains):
+def test_z_free_fused_sweep(nsubs, ncovar, nchains, method):
     """Without Z (z_store 2) X'Z is formed from draws held in registers: it
     must match X' times the oracle's own draws (same Philox counters), and the
     coefficient updates that consume it must match the oracle's."""
-    print(f"case c1 z_store=2 N={nsubs} P={ncovar} chains={nchains}", flush=True)
+    print(f"case c1 z_store=2 method={method} N={nsubs} P={ncovar} chains={nchains}", flush=True)
     d = synth.generate("c1", nsubs=nsubs, ncovar=ncovar)
     nx, ny, nz = d["grid"]
     lay = oracle.layout(d["mask"], nx, ny, nz)
     L = Layout(d["mask"], nx, ny, nz)
-    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2)
+    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2,
+                latent_method=method)
     O = oracle.Oracle(lay, d["covar"], d["data"], seed=SEED)
     P = S.P
     rng = np.random.default_rng(11)
@@ -177,7 +179,7 @@ def test_z_free_fused_sweep(nsubs, ncovar, nchains):
     S.synchronize()
     for c in range(nchains):
         O.set_chain(c)
-        z_orc = O.update_z(scs[c], sweep)
+        z_orc = O.update_z(scs[c], sweep, method=method)
         m_gpu = S.get_xtz(c)
         scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_orc)
         # the GPU draws are FP32 (1e-5 of the oracle's) and summed in FP32
@@ -224,8 +226,8 @@ def test_chains_are_independent_streams():
     assert rel_err(S2.get_z(chain=2), O.update_z(sc0, 0)) < 1e-5
 
 
-@pytest.mark.parametrize("z_store,xtz_mode", [(0, 2), (0, 0), (2, 2)])
-def test_wm_coefficient_sweep(z_store, xtz_mode):
+@pytest.mark.parametrize("z_store,xtz_mode,method", [(0, 2, 0), (0, 0, 0), (2, 2, 0), (2, 2, 1)])
+def test_wm_coefficient_sweep(z_store, xtz_mode, method):
     """With a WM image the sweep draws b between the latent draw and the
     coefficients (mcmc.cpp:344-356): Z uses the old b, the coefficients use
     X'(Z - b_new WM). Each step is checked against the oracle."""
@@ -236,7 +238,8 @@ def test_wm_coefficient_sweep(z_store, xtz_mode):
     L = Layout(d["mask"], nx, ny, nz)
     rng = np.random.default_rng(5)
     wm = rng.uniform(0.0, 1.0, lay["totvox"]).astype(np.float32)
-    S = Sampler(L, d["covar"], d["data"], wm=wm, seed=SEED, xtz_mode=xtz_mode, z_store=z_store)
+    S = Sampler(L, d["covar"], d["data"], wm=wm, seed=SEED, xtz_mode=xtz_mode, z_store=z_store,
+                latent_method=method)
     O = oracle.Oracle(lay, d["covar"], d["data"], wm=wm, seed=SEED)
     P = S.P
     sweep, b0 = 7, 0.3
@@ -248,7 +251,7 @@ def test_wm_coefficient_sweep(z_store, xtz_mode):
     S.set_sweep_index(sweep)
     S.sweep(1)
     S.synchronize()
-    z_orc = O.update_z(sc0, sweep, b=b0)
+    z_orc = O.update_z(sc0, sweep, b=b0, method=method)
     if z_store == 0:
         z_gpu = S.get_z()
         assert rel_err(z_gpu, z_orc) < 1e-5

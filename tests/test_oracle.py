@@ -271,3 +271,27 @@ def test_wm_stat_closed_form_and_draw_law():
     draws = np.array([O.wm_draw(S, W, sw) for sw in range(4000)])
     zs = (draws - S / W) * np.sqrt(W)
     assert st.kstest(zs, "norm").pvalue > 1e-3
+
+
+@pytest.mark.parametrize("mu,y", [(-2.0, 0), (-2.0, 1), (0.3, 0), (0.3, 1), (3.5, 0), (-0.5, 1)])
+def test_accept_or_fallback_sampler_is_the_truncated_normal(mu, y):
+    """orc_update_z method 1 (Box-Muller candidate kept inside the interval
+    of y, exact inversion from an independent block otherwise) against the
+    exact truncated-normal CDF: one voxel-major problem with an intercept-only
+    design, every subject at the same mean, many sweeps. A sampler that kept
+    rejected candidates, clipped them, or reused the candidate's uniform for
+    the redraw fails this."""
+    nsub = 400
+    mask = np.zeros((4, 4, 5), np.uint8)
+    mask[1:3, 1:3, 1:4] = 1  # 2x2x3 voxels
+    lay = oracle.layout(mask.reshape(-1), 3, 2, 2)
+    V = lay["totvox"]
+    covar = np.ones((1, nsub), np.float32)
+    data = np.full((nsub, V), y, np.uint8)
+    O = oracle.Oracle(lay, covar, data, seed=1234)
+    sc = np.full((1, lay["totvoxp"]), mu, np.float32)
+    z = np.concatenate([O.update_z(sc, sw, method=1).ravel() for sw in range(6)])
+    lo, hi = (1e-6, 500.0) if y else (-500.0, -1e-6)
+    assert np.all((z > lo) & (z < hi))
+    p = st.kstest(z, lambda t: st.truncnorm.cdf(t, lo - mu, hi - mu, loc=mu)).pvalue
+    assert p > 1e-3, p

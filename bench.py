@@ -42,6 +42,8 @@ def parse():
                     help="chains of the batched form (0 skips the ESS comparison)")
     ap.add_argument("--batched-iters", type=int, default=5000, help="iterations per chain, burn-in included")
     ap.add_argument("--batched-burnin", type=int, default=1000)
+    ap.add_argument("--uneven-steps", type=int, default=50,
+                    help="batch sweeps of the 12-analysis uneven batch (0 skips it)")
     ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "k_latent_vl_c2.json"),
                     help="measured traffic and VALU counts of the dominant kernel (rocprofv3 summary)")
     return ap.parse_args()
@@ -187,6 +189,80 @@ def batched_vs_plain(L, d, chains, iters, burnin):
     return out
 
 
+def uneven_batch(steps):
+    """Twelve independent analyses (ellipsoid masks of 20,031 to 398,905
+    voxels, N from 40 to 1,000, P from 2 to 8), 4 chains each. Each analysis
+    gets its lane split from the autotuner; then `steps` sweeps of the whole
+    batch (one launch per phase over every analysis and chain) are timed
+    against the same sweeps run analysis by analysis."""
+    from bsglmm_amd import Batch, Layout, Sampler, synth
+    rows, samplers = [], []
+    for k in range(12):
+        d = synth.generate(f"c5_{k}")
+        L = Layout(d["mask"], *d["grid"])
+        S = Sampler(L, d["covar"], d["data"], seed=0xC5C5 + k, nchains=4, xtz_mode=2, z_store=2)
+        S.sweep(3)
+        g, ms = S.autotune(reps=3)
+        P, N = d["covar"].shape
+        rows.append({"analysis": k, "V": L.totvox, "N": N, "P": P, "chains": 4, "lane_split_G": g,
+                     "latent_ms_by_G": {str(G): (None if m < 0 else round(float(m), 4)) for G, m in zip((1, 2, 4, 8), ms)}})
+        samplers.append(S)
+        del d
+    draws = sum(4 * r["N"] * r["V"] for r in rows)
+    t0 = time.perf_counter()
+    for S in samplers:
+        S.sweep(steps)
+    for S in samplers:
+        S.synchronize()
+    seq = time.perf_counter() - t0
+    B = Batch(samplers)
+    B.sweep(3)
+    t0 = time.perf_counter()
+    B.sweep(steps)
+    bat = time.perf_counter() - t0
+    B.close()
+    for S in samplers:
+        S.close()
+    small = small_batch(max(steps, 100))
+    return {"small_analyses_batch": small, "analyses": rows, "chains_per_analysis": 4,
+            "latent_draws_per_batch_sweep": draws,
+            "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
+            "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat,
+            "form": "FP32, Z-free fused X'Z (the BF16-split MFMA X'Z needs Z in HBM and is slower, "
+                    "see forms.separate_xtz_mfma_bf16x2)"}
+
+
+def small_batch(steps, count=24):
+    """`count` small analyses (the 32^3 ellipsoid mask, V = 7,056; N from 20 to
+    112, P from 2 to 8; one chain each), where a single analysis cannot fill
+    the GPU: batched launches against the analyses swept one by one."""
+    from bsglmm_amd import Batch, Layout, Sampler, synth
+    samplers, draws = [], 0
+    for k in range(count):
+        n, p = 20 + 4 * k, 2 + k % 7
+        d = synth.generate("c1", nsubs=n, ncovar=p)
+        L = Layout(d["mask"], *d["grid"])
+        samplers.append(Sampler(L, d["covar"], d["data"], seed=0x5A11 + k, xtz_mode=2, z_store=2))
+        draws += n * L.totvox
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for S in samplers:
+            S.sweep(1)
+    for S in samplers:
+        S.synchronize()
+    seq = time.perf_counter() - t0
+    B = Batch(samplers)
+    B.sweep(3)
+    t0 = time.perf_counter()
+    B.sweep(steps)
+    bat = time.perf_counter() - t0
+    B.close()
+    for S in samplers:
+        S.close()
+    return {"analyses": count, "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
+            "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat}
+
+
 def load_profile(path):
     if os.path.exists(path):
         with open(path) as fh:
@@ -263,6 +339,8 @@ def main():
     }
     if a.batched_chains > 0:
         out["batched"] = batched_vs_plain(L, d, a.batched_chains, a.batched_iters, a.batched_burnin)
+    if a.uneven_steps > 0:
+        out["uneven_batch"] = uneven_batch(a.uneven_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d, a.cpu_sweeps)
     print(json.dumps(out), flush=True)

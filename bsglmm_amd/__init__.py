@@ -94,6 +94,11 @@ def lib():
         "bsg_nifti_read": (C.c_int, [C.c_char_p, pd, C.c_size_t]),
         "bsg_nifti_write_f64": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, pd]),
         "bsg_design_read": (C.c_int, [C.c_char_p, C.c_int, pi, pf, C.c_char_p, C.c_char_p]),
+        "bsg_autotune": (C.c_int, [vp, C.c_int, pi, pd]),
+        "bsg_batch_create": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+        "bsg_batch_destroy": (C.c_int, [vp]),
+        "bsg_batch_sweep": (C.c_int, [vp, C.c_int]),
+        "bsg_batch_sweep_accumulate": (C.c_int, [vp, C.c_int, C.c_int]),
         "bsg_kiss": (C.c_double, [C.POINTER(C.c_ulong)]),
         "bsg_devseed": (C.c_uint64, [C.POINTER(C.c_ulong)]),
     }
@@ -269,6 +274,13 @@ class Sampler:
         m = np.ascontiguousarray(m, dtype=np.float32)
         _check(lib().bsg_set_xtz(self._h, chain, _ptr(m, C.c_float)))
 
+    def autotune(self, reps=3):
+        """Lane split G of the Z-free latent kernel chosen by timing; (G, ms for G = 1, 2, 4, 8)."""
+        g = C.c_int()
+        ms = np.empty(4, np.float64)
+        _check(lib().bsg_autotune(self._h, int(reps), C.byref(g), _ptr(ms, C.c_double)))
+        return g.value, ms
+
     def coef_means(self, chain=0):
         """Plane means of the coefficients over the mask (alphaMean)."""
         out = np.empty(self.P, np.float64)
@@ -357,6 +369,34 @@ def devseed(seed):
     v = lib().bsg_devseed(s)
     seed[:] = list(s)
     return v
+
+
+class Batch:
+    """Independent analyses (Samplers in the Z-free form) swept together:
+    each phase of a sweep is one kernel launch over every analysis and chain."""
+
+    def __init__(self, samplers):
+        self.samplers = list(samplers)
+        arr = (C.c_void_p * len(self.samplers))(*[S._h.value for S in self.samplers])
+        self._h = C.c_void_p()
+        _check(lib().bsg_batch_create(len(self.samplers), arr, C.byref(self._h)))
+
+    def sweep(self, n=1):
+        _check(lib().bsg_batch_sweep(self._h, int(n)))
+
+    def sweep_accumulate(self, n, save_every=1):
+        _check(lib().bsg_batch_sweep_accumulate(self._h, int(n), int(save_every)))
+
+    def close(self):
+        if self._h:
+            lib().bsg_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- file formats of the reference program (include/bsglmm_io.h) ----

@@ -1051,3 +1051,226 @@ uint64_t bsg_devseed(unsigned long seed[3])
 }
 
 }  // extern "C"
+
+// ---- autotuning of the Z-free latent kernel ----------------------------
+int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
+{
+  CTXCHK(ctx);
+  if (ctx->opts.z_store != 2 || reps < 1) {
+    set_error("bsg_autotune: tunes the Z-free latent kernel (z_store 2); reps >= 1");
+    return BSG_ERR_ARG;
+  }
+  const int cand[4] = {1, 2, 4, 8};
+  double best = 1e300;
+  int bg = 0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int keep = ctx->pb.lat_g;
+  int rc = BSG_OK;
+  for (int k = 0; k < 4 && rc == BSG_OK; k++) {
+    const int G = cand[k];
+    double ms = -1.0;
+    if (G == 1 || G <= ctx->NQ) {  // more lanes per voxel than subject quads would idle lanes
+      ctx->pb.lat_g = G;
+      // draws at the current sweep index: the state the next sweep reads is not changed
+      if (bsg::launch_latent(ctx->pb, true, 2, ctx->sweep, ctx->stream, 0, ctx->L.totvox) != hipSuccess ||
+          hipEventRecord(e0, ctx->stream) != hipSuccess) {
+        rc = BSG_ERR_HIP;
+        break;
+      }
+      for (int r = 0; r < reps; r++)
+        if (bsg::launch_latent(ctx->pb, true, 2, ctx->sweep, ctx->stream, 0, ctx->L.totvox) != hipSuccess) rc = BSG_ERR_HIP;
+      float f = 0.0f;
+      if (rc == BSG_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&f, e0, e1) != hipSuccess))
+        rc = BSG_ERR_HIP;
+      ms = f / reps;
+      if (rc == BSG_OK && ms < best) {
+        best = ms;
+        bg = G;
+      }
+    }
+    if (ms_per_g) ms_per_g[k] = ms;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != BSG_OK) {
+    ctx->pb.lat_g = keep;
+    set_error("bsg_autotune: latent launch failed");
+    return rc;
+  }
+  ctx->pb.lat_g = bg;
+  if (best_g) *best_g = bg;
+  return BSG_OK;
+}
+
+// ---- batches of independent analyses ------------------------------------
+struct bsg_batch {
+  std::vector<bsg_ctx *> m;
+  hipStream_t sa = nullptr, sb = nullptr;
+  hipEvent_t e_c0 = nullptr, e_c1 = nullptr;
+  bsg::DevProblem *d_pbs = nullptr;
+  bsg::WorkItem *d_items = nullptr;
+  bsg::Accum *d_accs = nullptr;
+  // item ranges per phase (latent, coef 0, coef 1, wishart, accum) and P
+  int off[5][bsg::MAX_P + 1] = {}, cnt[5][bsg::MAX_P + 1] = {};
+  size_t lat_lds = 0;
+  bool bmf = false;
+  std::vector<bsg::Accum> h_accs;
+
+  ~bsg_batch()
+  {
+    if (sa) hipStreamSynchronize(sa);
+    if (sb) hipStreamSynchronize(sb);
+    if (d_pbs) hipFree(d_pbs);
+    if (d_items) hipFree(d_items);
+    if (d_accs) hipFree(d_accs);
+    if (e_c0) hipEventDestroy(e_c0);
+    if (e_c1) hipEventDestroy(e_c1);
+    if (sb) hipStreamDestroy(sb);
+    if (sa) hipStreamDestroy(sa);
+  }
+};
+
+int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
+{
+  if (!out || !members || n < 1 || n > bsg::MAX_BATCH) {
+    set_error("bsg_batch_create: need 1.." + std::to_string(bsg::MAX_BATCH) + " contexts");
+    return BSG_ERR_ARG;
+  }
+  *out = nullptr;
+  std::unique_ptr<bsg_batch> b(new (std::nothrow) bsg_batch);
+  if (!b) {
+    set_error("bsg_batch_create: out of host memory");
+    return BSG_ERR_NOMEM;
+  }
+  std::vector<bsg::DevProblem> pbs;
+  std::vector<bsg::WorkItem> lists[5];
+  for (int a = 0; a < n; a++) {
+    bsg_ctx *c = members[a];
+    if (!c) {
+      set_error("bsg_batch_create: NULL context");
+      return BSG_ERR_ARG;
+    }
+    if (c->opts.z_store != 2 || c->opts.xtz_mode != 2 || c->pb.wm_update || c->L.n[1] == 0) {
+      set_error("bsg_batch_create: members must use the Z-free form (xtz_mode 2, z_store 2), have no WM "
+                "image and two colours");
+      return BSG_ERR_ARG;
+    }
+    if (c->opts.latent_method != members[0]->opts.latent_method || c->sweep != members[0]->sweep) {
+      set_error("bsg_batch_create: members must share the latent method and the sweep index");
+      return BSG_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    bsg::DevProblem pb = c->pb;
+    if (pb.lat_g <= 0) pb.lat_g = bsg::latent_vl_groups(pb);
+    pbs.push_back(pb);
+    b->lat_lds = std::max(b->lat_lds, bsg::latent_vl_lds_bytes(pb));
+    const int tiles = bsg::latent_vl_tiles(pb, pb.lat_g);
+    for (int ch = 0; ch < c->C; ch++) {
+      for (int t = 0; t < tiles; t++) lists[0].push_back({a, ch, t});
+      for (int k = 0; k < (c->L.n[0] + 255) / 256; k++) lists[1].push_back({a, ch, k});
+      for (int k = 0; k < (c->L.n[1] + 255) / 256; k++) lists[2].push_back({a, ch, k});
+      lists[3].push_back({a, ch, 0});
+      for (int k = 0; k < (c->L.totvox + 255) / 256; k++) lists[4].push_back({a, ch, k});
+    }
+    b->m.push_back(c);
+  }
+  b->bmf = members[0]->opts.latent_method == 1;
+  std::vector<bsg::WorkItem> all;
+  for (int k = 0; k < 5; k++)
+    for (int P = 1; P <= bsg::MAX_P; P++) {
+      b->off[k][P] = (int)all.size();
+      for (const bsg::WorkItem &w : lists[k])
+        if (pbs[w.a].P == P) all.push_back(w);
+      b->cnt[k][P] = (int)all.size() - b->off[k][P];
+    }
+  if (hipMalloc((void **)&b->d_pbs, sizeof(bsg::DevProblem) * n) != hipSuccess ||
+      hipMalloc((void **)&b->d_items, sizeof(bsg::WorkItem) * all.size()) != hipSuccess ||
+      hipMalloc((void **)&b->d_accs, sizeof(bsg::Accum) * n) != hipSuccess) {
+    set_error("bsg_batch_create: hipMalloc failed");
+    return BSG_ERR_NOMEM;
+  }
+  HIPCHK(hipStreamCreateWithFlags(&b->sa, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&b->sb, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&b->e_c0, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&b->e_c1, hipEventDisableTiming));
+  HIPCHK(hipMemcpy(b->d_pbs, pbs.data(), sizeof(bsg::DevProblem) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_items, all.data(), sizeof(bsg::WorkItem) * all.size(), hipMemcpyHostToDevice));
+  *out = b.release();
+  return BSG_OK;
+}
+
+int bsg_batch_destroy(bsg_batch *b)
+{
+  delete b;
+  return BSG_OK;
+}
+
+// Sweeps of every analysis and chain, one launch per phase: latent draws and
+// X'Z, colour 0, colour 1 (+ pair partials), Wishart on the second stream.
+static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
+{
+  hipStream_t sa = b->sa, sb = b->sb;
+  const bsg::WorkItem *it = b->d_items;
+  bool wish_pending = false;
+  bsg::BatchIndex bidx{};
+  for (int i = 0; i < nsweeps; i++) {
+    const uint32_t sw = b->m[0]->sweep;
+    for (int P = 1; P <= bsg::MAX_P; P++)
+      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds, b->bmf, sw, sa));
+    if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, b->e_c1, 0));
+    for (int P = 1; P <= bsg::MAX_P; P++)
+      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, sa));
+    for (int P = 1; P <= bsg::MAX_P; P++)
+      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, sa));
+    HIPCHK(hipEventRecord(b->e_c0, sa));
+    HIPCHK(hipStreamWaitEvent(sb, b->e_c0, 0));
+    for (int P = 1; P <= bsg::MAX_P; P++)
+      HIPCHK(bsg::launch_wishart_multi(P, b->d_pbs, it + b->off[3][P], b->cnt[3][P], sw, sb));
+    HIPCHK(hipEventRecord(b->e_c1, sb));
+    wish_pending = true;
+    for (bsg_ctx *c : b->m) c->sweep++;
+    if (accum_every > 0 && (i + 1) % accum_every == 0) {
+      for (size_t a = 0; a < b->m.size(); a++) {
+        bsg_ctx *c = b->m[a];
+        bidx.b[a] = std::min(c->acc_count / c->acc_batch_len, c->acc.nbatches - 1);
+        c->acc_count++;
+      }
+      for (int P = 1; P <= bsg::MAX_P; P++)
+        HIPCHK(bsg::launch_accum_multi(P, b->d_pbs, b->d_accs, bidx, it + b->off[4][P], b->cnt[4][P], sa));
+    }
+  }
+  if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, b->e_c1, 0));
+  HIPCHK(hipStreamSynchronize(sa));
+  return BSG_OK;
+}
+
+int bsg_batch_sweep(bsg_batch *b, int nsweeps)
+{
+  if (!b || nsweeps < 0) {
+    set_error("bsg_batch_sweep: bad argument");
+    return BSG_ERR_ARG;
+  }
+  return batch_run(b, nsweeps, 0);
+}
+
+int bsg_batch_sweep_accumulate(bsg_batch *b, int nsweeps, int save_every)
+{
+  if (!b || nsweeps < 0 || save_every < 1) {
+    set_error("bsg_batch_sweep_accumulate: bad argument");
+    return BSG_ERR_ARG;
+  }
+  b->h_accs.clear();
+  for (bsg_ctx *c : b->m) {
+    if (!c->acc.sum) {
+      set_error("bsg_batch_sweep_accumulate: call bsg_accum_reset on every member first");
+      return BSG_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    b->h_accs.push_back(c->acc);
+  }
+  HIPCHK(hipMemcpy(b->d_accs, b->h_accs.data(), sizeof(bsg::Accum) * b->h_accs.size(), hipMemcpyHostToDevice));
+  return batch_run(b, nsweeps, save_every);
+}

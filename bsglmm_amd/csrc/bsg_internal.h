@@ -68,6 +68,7 @@ struct DevProblem {
   // captured graph of K sweeps passes 0..K-1 and ends with k_tick(+K).
   const uint32_t *sweep_base;
   int latent_method;       // Z-free form: 0 inversion for every draw, 1 Box-Muller candidate + exact fallback
+  int lat_g;               // Z-free form: lanes per voxel (1, 2, 4, 8); 0 = chosen from N
   // WM coefficient b (covarGPU.cu:588-724): drawn each sweep when wm_update
   int wm_update;
   double wm2;              // N sum_j WM_j^2
@@ -82,6 +83,15 @@ struct Accum {
   double *prb = nullptr;   // [chain][V][ntypes]
   double *bwm = nullptr;   // [chain][V]
   float *batch = nullptr;  // [chain][nbatches][V][P + ntypes]
+};
+
+// One block of a batched launch: analysis, chain and tile (or block) index.
+struct WorkItem {
+  int a, chain, blk;
+};
+constexpr int MAX_BATCH = 32;  // analyses per batch
+struct BatchIndex {
+  int b[MAX_BATCH];  // posterior-accumulator batch of each analysis
 };
 
 // kernel launchers (bsg_kernels.hip)
@@ -100,6 +110,19 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 hipError_t launch_probe(int kind, const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st);
 hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st);
+// Z-free latent form: lanes per voxel chosen from N, its tile count and LDS bytes
+int latent_vl_groups(const DevProblem &pb);
+int latent_vl_tiles(const DevProblem &pb, int G);
+size_t latent_vl_lds_bytes(const DevProblem &pb);
+// batched analyses: one launch per phase and covariate count P over the work items
+hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
+                               uint32_t sweep, hipStream_t st);
+hipError_t launch_coef_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, int colour, bool prec,
+                             uint32_t sweep, hipStream_t st);
+hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,
+                                hipStream_t st);
+hipError_t launch_accum_multi(int P, const DevProblem *pbs, const Accum *accs, const BatchIndex &bidx,
+                              const WorkItem *items, int n, hipStream_t st);
 // plane means: part[chain][blocks][P] FP64 block partials
 hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st);
 // per-subject log-likelihood: part [chain][tiles][N][T], out [chain][N][T]

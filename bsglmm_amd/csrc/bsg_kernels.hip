@@ -477,18 +477,20 @@ __device__ __forceinline__ float latent_one(float mu, bool y, float u, float eps
 // ---------------------------------------------------------------------------
 constexpr int VL_QUEUE = 32;  // fallback slots per lane (uint16 subject index)
 
-template <int P, int G, bool WMU, bool BMF>
-__global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep)
+// One block's share: tiles t0, t0 + tstep, ... of chain `chain`; G (1, 2, 4
+// or 8) lanes per voxel; WMU writes its FP64 partial to bstat slot `slot`.
+template <int P, bool WMU, bool BMF>
+__device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint32_t sweep, int chain, int t0,
+                                               int tstep, int slot)
 {
-  sweep += *pb.sweep_base;
   extern __shared__ float4 xs[];  // [NQ][P], then (BMF) the fallback queues
   uint16_t *queue = reinterpret_cast<uint16_t *>(xs + (size_t)pb.NQ * P) + threadIdx.x * VL_QUEUE;
   const float *xsf = reinterpret_cast<const float *>(xs);  // x_c,i = xsf[(q P + c) 4 + l]
-  constexpr int VPW = 64 / G;
-  const int chain = blockIdx.y;
+  const int lgg = G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0;
+  const int VPW = 64 >> lgg;
   const int lane = threadIdx.x & 63;
-  const int vl = lane % VPW;
-  const int g = lane / VPW;
+  const int vl = lane & (VPW - 1);
+  const int g = lane >> (6 - lgg);
   for (int i = threadIdx.x; i < pb.NQ * P; i += blockDim.x) {
     const int q = i / P, c = i - q * P;
     xs[i] = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + 4 * q);
@@ -499,12 +501,12 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
   float xsum[P];
 #pragma unroll
   for (int c = 0; c < P; c++) xsum[c] = pb.xsum[c];
-  const int R = (pb.NQ + G - 1) / G;
+  const int R = (pb.NQ + (1 << lgg) - 1) >> lgg;
   const int q0 = g * R;
   const int q1 = min(q0 + R, pb.NQ);
   const int tiles = (pb.ve - pb.vb + 4 * VPW - 1) / (4 * VPW);
   double bpart = 0.0;  // WMU: this thread's share of sum_j WM_j (sum_i Z_ij - X'1 b_j)
-  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+  for (int t = t0; t < tiles; t += tstep) {
     const int s = pb.vb + (t * 4 + (threadIdx.x >> 6)) * VPW + vl;
     const bool valid = s < pb.ve;
     const int sv = valid ? s : pb.ve - 1;
@@ -621,8 +623,15 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, uint32_t sweep
   }
   if (WMU) {
     double bp[1] = {bpart};
-    block_sum_store<1>(bp, pb.bstat + (size_t)chain * BSTAT_SLOTS + blockIdx.x);
+    block_sum_store<1>(bp, pb.bstat + (size_t)chain * BSTAT_SLOTS + slot);
   }
+}
+
+template <int P, bool WMU, bool BMF>
+__global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, int G, uint32_t sweep)
+{
+  sweep += *pb.sweep_base;
+  latent_vl_body<P, WMU, BMF>(pb, G, sweep, blockIdx.y, blockIdx.x, gridDim.x, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1013,13 +1022,12 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
   return true;
 }
 
+// Block `blk` of colour `colour` (256 voxels of its list) for chain `chain`.
 template <int P, bool PREC>
-__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep)
+__device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint32_t sweep, int chain, int blk)
 {
-  sweep += *pb.sweep_base;
   constexpr int NE = P * (P + 1) / 2;
-  const int chain = blockIdx.y;
-  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int q = blk * 256 + threadIdx.x;
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   const bool active = q < nc;
   const int s = (active ? q : nc - 1) + (colour ? pb.n0 : 0);
@@ -1129,7 +1137,14 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
       if (nb[d] >= 0) add_pair<P>(out, bk[d], (d & 1) != 0, acc);
   }
   double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
-  block_sum_store<NE>(acc, parts + (size_t)blockIdx.x * NE);
+  block_sum_store<NE>(acc, parts + (size_t)blk * NE);
+}
+
+template <int P, bool PREC>
+__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep)
+{
+  sweep += *pb.sweep_base;
+  coef_body<P, PREC>(pb, colour, sweep, blockIdx.y, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1185,10 +1200,9 @@ __device__ __forceinline__ double phi_as(double x)
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void k_accum(DevProblem pb, Accum ac, int batch)
+__device__ __forceinline__ void accum_body(const DevProblem &pb, const Accum &ac, int batch, int chain, int blk)
 {
-  const int chain = blockIdx.y;
-  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int s = blk * 256 + threadIdx.x;
   if (s >= pb.V) return;
   const float *b = pb.beta + ((size_t)chain * pb.V + s) * P;
   const double bw = (double)pb.bwm[chain] * (double)pb.wm[s];
@@ -1208,6 +1222,12 @@ __global__ __launch_bounds__(256) void k_accum(DevProblem pb, Accum ac, int batc
     bat[P + t] += (float)pr;
   }
   ac.bwm[base] += bw;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_accum(DevProblem pb, Accum ac, int batch)
+{
+  accum_body<P>(pb, ac, batch, blockIdx.y, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1299,6 +1319,49 @@ __global__ __launch_bounds__(256) void k_llreduce(int tiles, int NT, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// Batches of independent analyses (their own masks, N and P): one launch per
+// phase and covariate count, so each launch has the register allocation of
+// its own P. Every block reads its work item (analysis, chain, tile or block)
+// and runs the body a single analysis runs. Analyses keep their own device
+// arrays; the problem table and the item lists live in device memory.
+// ---------------------------------------------------------------------------
+template <int P, bool BMF>
+__global__ __launch_bounds__(256) void k_latent_multi(const DevProblem *pbs, const WorkItem *items, uint32_t sweep)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  latent_vl_body<P, false, BMF>(pb, pb.lat_g, sweep + *pb.sweep_base, it.chain, it.blk, 1 << 30, 0);
+}
+
+template <int P, bool PREC>
+__global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const WorkItem *items, int colour,
+                                                    uint32_t sweep)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_wishart_multi(const DevProblem *pbs, const WorkItem *items, uint32_t sweep)
+{
+  constexpr int NE = P * (P + 1) / 2;
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  wishart_block<P>(pb, it.chain, (pb.V - pb.n0 + 255) / 256, pb.partials + (size_t)it.chain * pb.max_parts * NE,
+                   sweep + *pb.sweep_base);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_accum_multi(const DevProblem *pbs, const Accum *accs, BatchIndex bidx,
+                                                     const WorkItem *items)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  accum_body<P>(pb, accs[it.a], bidx.b[it.a], it.chain, it.blk);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
@@ -1349,27 +1412,19 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
   const long long per = (tiles + BSTAT_SLOTS - 1) / BSTAT_SLOTS;  // tiles per block, equal shares
   dim3 grid((unsigned)((tiles + per - 1) / per), (unsigned)pb.nchains);
   const bool wmu = pb.wm_update != 0, bmf = pb.latent_method == 1;
-#define VK(PP, GG)                                                                                       \
-  do {                                                                                                   \
-    if (bmf) {                                                                                           \
-      if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, GG, true, true>), grid, dim3(256), lds, st, pb, sweep);  \
-      else hipLaunchKernelGGL((k_latent_vl<PP, GG, false, true>), grid, dim3(256), lds, st, pb, sweep);     \
-    } else {                                                                                             \
-      if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, GG, true, false>), grid, dim3(256), lds, st, pb, sweep); \
-      else hipLaunchKernelGGL((k_latent_vl<PP, GG, false, false>), grid, dim3(256), lds, st, pb, sweep);    \
-    }                                                                                                    \
-  } while (0)
-#define V_(PP)                         \
-  do {                                 \
-    if (G == 1) VK(PP, 1);             \
-    else if (G == 2) VK(PP, 2);        \
-    else if (G == 4) VK(PP, 4);        \
-    else if (G == 8) VK(PP, 8);        \
-    else return hipErrorInvalidValue;  \
+  if (G != 1 && G != 2 && G != 4 && G != 8) return hipErrorInvalidValue;
+#define V_(PP)                                                                                             \
+  do {                                                                                                     \
+    if (bmf) {                                                                                             \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, true>), grid, dim3(256), lds, st, pb, G, sweep);   \
+      else hipLaunchKernelGGL((k_latent_vl<PP, false, true>), grid, dim3(256), lds, st, pb, G, sweep);      \
+    } else {                                                                                               \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, false>), grid, dim3(256), lds, st, pb, G, sweep);  \
+      else hipLaunchKernelGGL((k_latent_vl<PP, false, false>), grid, dim3(256), lds, st, pb, G, sweep);     \
+    }                                                                                                      \
   } while (0)
   BSG_DISPATCH_P(pb.P, V_)
 #undef V_
-#undef VK
   if (parts) *parts = wmu ? (int)grid.x : 0;
   return hipGetLastError();
 }
@@ -1395,7 +1450,7 @@ hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t
     // the Z-free form needs X in LDS; the WM statistic needs the whole voxel range
     if (latent_vl_lds(pb) > 65536) return hipErrorInvalidValue;
     if (pb.wm_update && (v0 != 0 || v1 != pb.V)) return hipErrorInvalidValue;
-    return launch_latent_vl(pb, latent_vl_groups(pb), sweep, st, bstat_parts);
+    return launch_latent_vl(pb, pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb), sweep, st, bstat_parts);
   }
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
@@ -1560,6 +1615,59 @@ hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st)
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, base, k);
   return hipGetLastError();
 }
+
+hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
+                               uint32_t sweep, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  if (lds > 65536) return hipErrorInvalidValue;
+#define L_(PP)                                                                                                 \
+  do {                                                                                                         \
+    if (bmf) hipLaunchKernelGGL((k_latent_multi<PP, true>), dim3((unsigned)n), dim3(256), lds, st, pbs, items, sweep); \
+    else hipLaunchKernelGGL((k_latent_multi<PP, false>), dim3((unsigned)n), dim3(256), lds, st, pbs, items, sweep);    \
+  } while (0)
+  BSG_DISPATCH_P(P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_coef_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, int colour, bool prec,
+                             uint32_t sweep, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+#define L_(PP)                                                                                                    \
+  do {                                                                                                            \
+    if (prec) hipLaunchKernelGGL((k_coef_multi<PP, true>), dim3((unsigned)n), dim3(256), 0, st, pbs, items, colour, sweep); \
+    else hipLaunchKernelGGL((k_coef_multi<PP, false>), dim3((unsigned)n), dim3(256), 0, st, pbs, items, colour, sweep);    \
+  } while (0)
+  BSG_DISPATCH_P(P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,
+                                hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+#define L_(PP) hipLaunchKernelGGL(k_wishart_multi<PP>, dim3((unsigned)n), dim3(256), 0, st, pbs, items, sweep)
+  BSG_DISPATCH_P(P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_accum_multi(int P, const DevProblem *pbs, const Accum *accs, const BatchIndex &bidx,
+                              const WorkItem *items, int n, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+#define L_(PP) hipLaunchKernelGGL(k_accum_multi<PP>, dim3((unsigned)n), dim3(256), 0, st, pbs, accs, bidx, items)
+  BSG_DISPATCH_P(P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+int latent_vl_tiles(const DevProblem &pb, int G) { return (pb.V + 4 * (64 / G) - 1) / (4 * (64 / G)); }
+
+size_t latent_vl_lds_bytes(const DevProblem &pb) { return latent_vl_lds(pb); }
 
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st)
 {

@@ -37,6 +37,16 @@ CONFIGS = {
                nsubs=1000, ncovar=8, fwhm=8.0, sd=[0.5, 0.3, 0.3, 0.3, 0.2, 0.2, 0.2, 0.2]),
 }
 
+# Uneven batch of 12 analyses: ellipsoid masks on the full grid whose
+# semi-axes run linearly from (16, 20, 15) to (42, 54, 42), with subject
+# counts and covariate counts that differ per analysis (P_k = 2 + k mod 7).
+C5_NSUBS = [40, 80, 120, 250, 400, 600, 800, 1000, 60, 150, 300, 500]
+for _k in range(12):
+    _f = _k / 11.0
+    _axes = tuple(a + _f * (b - a) for a, b in zip((16, 20, 15), (42, 54, 42)))
+    CONFIGS[f"c5_{_k}"] = dict(config_id=50 + _k, grid=(91, 109, 91), mask=("ellipsoid", (45, 54, 45), _axes),
+                               nsubs=C5_NSUBS[_k], ncovar=2 + _k % 7, fwhm=8.0, sd=[0.5, 0.3, 0.3, 0.3])
+
 
 def _ellipsoid(grid, centre, axes):
     nx, ny, nz = grid

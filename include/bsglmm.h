@@ -170,6 +170,28 @@ int bsg_get_sweep_index(bsg_ctx *ctx, int chain, uint32_t *sweep);
 int bsg_set_sweep_index(bsg_ctx *ctx, uint32_t sweep);        /* all chains */
 int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count);     /* FP64 Cholesky refactors */
 
+/* ---- tile shapes and batches of analyses (no counterpart in the reference:
+ * it runs one analysis per process) ------------------------------------- */
+/* Times the Z-free latent kernel (z_store 2) of this context at each lane
+ * split G in {1, 2, 4, 8} lanes per voxel (voxels per workgroup = 256 / G,
+ * subject tile = N / G), reps launches each at the current sweep index (the
+ * chain state is not changed), and keeps the fastest for later sweeps.
+ * best_g receives it; ms_per_g[4] (optional) the mean times, -1 if skipped. */
+int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g);
+typedef struct bsg_batch bsg_batch;
+/* Groups up to 32 contexts (independent analyses: own masks, N, P and chain
+ * counts) so that each phase of a sweep is one kernel launch over all of
+ * them with the same covariate count (one launch per P present, so each has
+ * its own register allocation). Members must use the Z-free form (xtz_mode 2, z_store 2), have no WM
+ * image and share the latent method and sweep index; each keeps its own
+ * device state and lane split, and must outlive the batch. */
+int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out);
+int bsg_batch_destroy(bsg_batch *batch);
+/* nsweeps sweeps of every member; returns when they are done */
+int bsg_batch_sweep(bsg_batch *batch, int nsweeps);
+/* as bsg_sweep_accumulate for every member (bsg_accum_reset each first) */
+int bsg_batch_sweep_accumulate(bsg_batch *batch, int nsweeps, int save_every);
+
 /* ---- measurement utility (no counterpart in the reference) ------------- */
 /* Times reps launches of a float4 stream-copy kernel over `bytes`; gbps is
  * (read + written bytes) / time. Calibrates bandwidth and counter figures. */

@@ -1,0 +1,85 @@
+"""Batches of independent analyses in one launch per phase, and the lane-split
+autotuner.
+
+Every kernel body of a batched launch is the one a single analysis runs, with
+the same Philox counters, so a batch must reproduce each member's own sweeps
+bit for bit (coefficients, CAR precision, posterior accumulators).
+"""
+import numpy as np
+import pytest
+
+from bsglmm_amd import Batch, Layout, Sampler, synth
+
+pytestmark = pytest.mark.gpu
+SEED = 0xBA7C4ED
+
+
+def members(seed_shift=0):
+    """Three analyses with different masks, N, P and chain counts."""
+    out = []
+    for (name, n, p, chains) in (("c1", 50, 3, 2), ("c1", 7, 2, 1), ("c5_0", 120, 5, 3)):
+        d = synth.generate(name, nsubs=n, ncovar=p)
+        L = Layout(d["mask"], *d["grid"])
+        out.append((d, L, chains))
+    return out
+
+
+def make(ms):
+    return [Sampler(L, d["covar"], d["data"], seed=SEED + k, nchains=ch, xtz_mode=2, z_store=2)
+            for k, (d, L, ch) in enumerate(ms)]
+
+
+def test_batch_equals_members_run_alone():
+    print("case batch of 3 analyses (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3)", flush=True)
+    ms = members()
+    solo, grouped = make(ms), make(ms)
+    B = Batch(grouped)
+    for S in solo:
+        S.sweep(7)
+    B.sweep(7)
+    for S in solo:
+        S.accum_reset(ntypes=1, nbatches=4, batch_len=3)
+        S.sweep_accumulate(12, 1)
+    for S in grouped:
+        S.accum_reset(ntypes=1, nbatches=4, batch_len=3)
+    B.sweep_accumulate(12, 1)
+    for S, T in zip(solo, grouped):
+        S.synchronize()
+        T.synchronize()
+        assert S.sweep_index() == T.sweep_index() == 19
+        for c in range(S.nchains):
+            assert np.array_equal(S.get_spat_coef(c), T.get_spat_coef(c))
+            assert np.array_equal(S.get_prec(c), T.get_prec(c))
+            a, b = S.accum_get(c), T.accum_get(c)
+            assert a["ndraws"] == b["ndraws"] == 12
+            for k in ("mean", "var", "prb", "mcse"):
+                assert np.array_equal(a[k], b[k]), k
+    B.close()
+
+
+def test_autotune_keeps_the_chain_state():
+    print("case autotune c5_3 (N=250, P=5)", flush=True)
+    d = synth.generate("c5_3")
+    L = Layout(d["mask"], *d["grid"])
+    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=2, xtz_mode=2, z_store=2)
+    S.sweep(3)
+    before = [S.get_spat_coef(c) for c in range(2)], S.sweep_index()
+    g, ms = S.autotune(reps=2)
+    assert g in (1, 2, 4, 8) and ms[[1, 2, 4, 8].index(g)] == ms[ms > 0].min()
+    assert S.sweep_index() == before[1]
+    for c in range(2):
+        assert np.array_equal(S.get_spat_coef(c), before[0][c])
+    S.sweep(2)  # sweeps run with the tuned split
+    S.synchronize()
+    assert np.all(np.isfinite(S.get_spat_coef(0)))
+
+
+def test_batch_rejects_unsupported_members():
+    d = synth.generate("c1", nsubs=9, ncovar=2)
+    L = Layout(d["mask"], *d["grid"])
+    with pytest.raises(Exception):
+        Batch([Sampler(L, d["covar"], d["data"], seed=1, xtz_mode=2, z_store=0)])
+    A = Sampler(L, d["covar"], d["data"], seed=1, xtz_mode=2, z_store=2)
+    A.sweep(1)
+    with pytest.raises(Exception):  # sweep indices differ
+        Batch([A, Sampler(L, d["covar"], d["data"], seed=2, xtz_mode=2, z_store=2)])

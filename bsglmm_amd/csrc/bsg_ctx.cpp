@@ -298,6 +298,11 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   // b is drawn each sweep only when a WM image is given (mcmc.cpp:344-345, UPDATE_WM)
   pb.wm_update = wm ? 1 : 0;
   pb.latent_method = o.latent_method;
+  if (o.latent_method == 1 && (N > 32768 || bsg::latent_vl_lds_bytes(pb) > 65536)) {
+    set_error("bsg_create: latent_method 1 needs N <= 32768 and ceil(N/4) * P * 16 bytes of X plus the "
+              "fallback queues in 64 KiB of LDS");
+    return BSG_ERR_ARG;
+  }
   {
     double w2 = 0.0;
     for (int s = 0; s < V; s++) w2 += (double)wms[s] * (double)wms[s];

@@ -470,12 +470,23 @@ __device__ __forceinline__ float latent_one(float mu, bool y, float u, float eps
 // is queued in the lane's own LDS list and later redrawn exactly, by
 // inversion, from an independent Philox block (tag FALLBACK, sub = subject
 // in quad). An accepted candidate is a TN draw and a rejected one is
-// replaced by an independent TN draw, so every Z is TN. About 93 % of draws
-// take the cheap path (y = 0 draws reject when mu + W > -eps); the lists are
-// drained in a wave-uniform loop, each lane its own queue in subject order,
-// so the sums are deterministic.
+// replaced by an independent TN draw, so every Z is TN. About 89 % of draws
+// take the cheap path at the benchmark's data (y = 0 draws reject when
+// mu + W > -eps, y = 1 draws mostly). A drain spreads the wave's queued
+// draws over all 64 lanes (each lane publishes its voxel inputs; a lane
+// finds an item's owner by binary search over the wave's queue offsets) and
+// writes each redraw back to the owner's slot; every owner then adds its own
+// redraws in queue order, so the sums are deterministic.
 // ---------------------------------------------------------------------------
-constexpr int VL_QUEUE = 32;  // fallback slots per lane (uint16 subject index)
+constexpr int VL_QUEUE = 16;  // fallback slots per lane: subject index | y << 15
+
+// LDS bytes past X of the accept-or-fallback form: per thread the queue
+// (uint16), the redraws (float), its voxel inputs (P coefficients, b WM_j,
+// counter base) and its queue offset in the wave.
+__host__ __device__ constexpr size_t vl_fallback_lds(int P)
+{
+  return 256 * (VL_QUEUE * sizeof(uint16_t) + VL_QUEUE * sizeof(float) + (P + 2) * sizeof(float) + sizeof(int));
+}
 
 // One block's share: tiles t0, t0 + tstep, ... of chain `chain`; G (1, 2, 4
 // or 8) lanes per voxel; WMU writes its FP64 partial to bstat slot `slot`.
@@ -483,8 +494,13 @@ template <int P, bool WMU, bool BMF>
 __device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint32_t sweep, int chain, int t0,
                                                int tstep, int slot)
 {
-  extern __shared__ float4 xs[];  // [NQ][P], then (BMF) the fallback queues
-  uint16_t *queue = reinterpret_cast<uint16_t *>(xs + (size_t)pb.NQ * P) + threadIdx.x * VL_QUEUE;
+  extern __shared__ float4 xs[];  // [NQ][P], then (BMF) the fallback areas
+  uint16_t *const qall = reinterpret_cast<uint16_t *>(xs + (size_t)pb.NQ * P);  // [256][VL_QUEUE]
+  float *const rall = reinterpret_cast<float *>(qall + 256 * VL_QUEUE);        // [256][VL_QUEUE]
+  float *const info = rall + 256 * VL_QUEUE;                                   // [256][P + 1]
+  uint32_t *const cbs = reinterpret_cast<uint32_t *>(info + 256 * (P + 1));    // [256] counter bases
+  int *const offs = reinterpret_cast<int *>(cbs + 256);                         // [256]
+  uint16_t *const queue = qall + threadIdx.x * VL_QUEUE;
   const float *xsf = reinterpret_cast<const float *>(xs);  // x_c,i = xsf[(q P + c) 4 + l]
   const int lgg = G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0;
   const int VPW = 64 >> lgg;
@@ -524,32 +540,65 @@ __device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint
     float zs = 0.0f;
     int nq = 0;  // queued fallback draws of this lane
     // exact redraw of every queued subject, lanes in step, each its own queue
+    // redraw every queued subject of the wave, all lanes sharing the work
     auto drain = [&]() {
-      for (int k = 0; __ballot(k < nq) != 0; k++) {
-        if (k < nq) {
-          const int i = queue[k];
-          const int q = i >> 2, l = i & 3;
-          float xv[P];
-          float mu = bw;
+      const int wb = threadIdx.x & ~63;
+      float *mine = info + threadIdx.x * (P + 1);
 #pragma unroll
-          for (int c = 0; c < P; c++) {
-            xv[c] = xsf[(q * P + c) * 4 + l];
-            mu = fmaf(bs[c], xv[c], mu);
-          }
-          const bool y = (yrow[i >> 5] >> (i & 31)) & 1u;
+      for (int c = 0; c < P; c++) mine[c] = bs[c];
+      mine[P] = bw;
+      cbs[threadIdx.x] = cbase;
+      int v = nq;  // inclusive scan of the queue lengths over the wave
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+      }
+      const int total = __shfl(v, 63, 64);
+      offs[threadIdx.x] = v - nq;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int base = 0; base < total; base += 64) {
+        const int k = base + lane;
+        if (k < total) {
+          int o = 0;  // owner: the last lane whose offset is <= k
+#pragma unroll
+          for (int st = 32; st >= 1; st >>= 1) o += (offs[wb + o + st] <= k) ? st : 0;
+          const int ow = wb + o, t = k - offs[ow];
+          const uint32_t e = qall[ow * VL_QUEUE + t];
+          const int i = e & 0x7FFF, q = i >> 2, l = i & 3;
+          const float *oi = info + ow * (P + 1);
+          float mu = oi[P];
+#pragma unroll
+          for (int c = 0; c < P; c++) mu = fmaf(oi[c], xsf[(q * P + c) * 4 + l], mu);
           uint32_t rr[4];
-          philox10(cbase + (uint32_t)q, sweep, TAG_FALLBACK | ((uint32_t)l << 4), (uint32_t)chain, pb.key0,
-                   pb.key1, rr);
-          const float z = latent_one(mu, y, u01f(rr[0]), eps);
+          philox10(cbs[ow] + (uint32_t)q, sweep, TAG_FALLBACK | ((uint32_t)l << 4),
+                   (uint32_t)chain, pb.key0, pb.key1, rr);
+          rall[ow * VL_QUEUE + t] = latent_one(mu, (e >> 15) != 0, u01f(rr[0]), eps);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int t = 0; __ballot(t < nq) != 0; t++) {
+        if (t < nq) {
+          const int i = queue[t] & 0x7FFF, q = i >> 2, l = i & 3;
+          const float z = rall[threadIdx.x * VL_QUEUE + t];
 #pragma unroll
-          for (int c = 0; c < P; c++) acc[c] = fmaf(xv[c], z, acc[c]);
+          for (int c = 0; c < P; c++) acc[c] = fmaf(xsf[(q * P + c) * 4 + l], z, acc[c]);
           if (WMU) zs += z;
         }
       }
+      __builtin_amdgcn_wave_barrier();
       nq = 0;
     };
-    for (int q = q0; q < q1; q++) {
+    // the trip count is the same for every lane (R), so a drain runs with the
+    // whole wave; lanes past their own range skip the body
+    for (int r = 0; r < R; r++) {
+      const int q = q0 + r;
       if (BMF && __ballot(nq > VL_QUEUE - 4) != 0) drain();  // room for this quad's four
+      if (q >= q1) continue;
       float4 x[P];
 #pragma unroll
       for (int c = 0; c < P; c++) x[c] = xs[q * P + c];
@@ -589,7 +638,7 @@ __device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint
         const bool ok = yl ? (zc > eps && zc < 500.0f) : (zc < -eps && zc > -500.0f);
         const bool live = i0 + l < pb.N;  // padding subjects are neither kept nor queued
         z[l] = (ok && live) ? zc : 0.0f;
-        queue[nq] = (uint16_t)(i0 + l);  // kept only if the count advances
+        queue[nq] = (uint16_t)((i0 + l) | (yl ? 0x8000 : 0));  // kept only if the count advances
         nq += (!ok && live) ? 1 : 0;
       }
       }
@@ -1401,7 +1450,7 @@ int latent_vl_groups(const DevProblem &pb)
 
 static size_t latent_vl_lds(const DevProblem &pb)
 {
-  return (size_t)pb.NQ * pb.P * sizeof(float4) + (pb.latent_method == 1 ? 256 * VL_QUEUE * sizeof(uint16_t) : 0);
+  return (size_t)pb.NQ * pb.P * sizeof(float4) + (pb.latent_method == 1 ? vl_fallback_lds(pb.P) : 0);
 }
 
 static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st, int *parts)

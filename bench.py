@@ -209,12 +209,7 @@ def uneven_batch(steps):
         samplers.append(S)
         del d
     draws = sum(4 * r["N"] * r["V"] for r in rows)
-    t0 = time.perf_counter()
-    for S in samplers:
-        S.sweep(steps)
-    for S in samplers:
-        S.synchronize()
-    seq = time.perf_counter() - t0
+    seq, conc = baselines(samplers, steps, interleave=False)
     B = Batch(samplers)
     B.sweep(3)
     t0 = time.perf_counter()
@@ -228,8 +223,36 @@ def uneven_batch(steps):
             "latent_draws_per_batch_sweep": draws,
             "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
             "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat,
+            "concurrent_streams_draws_per_s": draws * steps / conc, "batched_over_concurrent_streams": conc / bat,
+            "sum_of_tuned_latent_ms": sum(min(v for v in r["latent_ms_by_G"].values() if v is not None)
+                                          for r in rows),
+            "batch_sweep_ms": 1e3 * bat / steps,
             "form": "FP32, Z-free fused X'Z (the BF16-split MFMA X'Z needs Z in HBM and is slower, "
                     "see forms.separate_xtz_mfma_bf16x2)"}
+
+
+def baselines(samplers, steps, interleave):
+    """Wall seconds of `steps` sweeps of every analysis without a batch:
+    one analysis at a time (each synchronized before the next starts), and
+    every analysis on its own stream with the launches issued together
+    (sweep by sweep when `interleave`, else analysis by analysis), so the
+    GPU overlaps them as it can."""
+    t0 = time.perf_counter()
+    for S in samplers:
+        S.sweep(steps)
+        S.synchronize()
+    seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if interleave:
+        for _ in range(steps):
+            for S in samplers:
+                S.sweep(1)
+    else:
+        for S in samplers:
+            S.sweep(steps)
+    for S in samplers:
+        S.synchronize()
+    return seq, time.perf_counter() - t0
 
 
 def small_batch(steps, count=24):
@@ -244,13 +267,7 @@ def small_batch(steps, count=24):
         L = Layout(d["mask"], *d["grid"])
         samplers.append(Sampler(L, d["covar"], d["data"], seed=0x5A11 + k, xtz_mode=2, z_store=2))
         draws += n * L.totvox
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for S in samplers:
-            S.sweep(1)
-    for S in samplers:
-        S.synchronize()
-    seq = time.perf_counter() - t0
+    seq, conc = baselines(samplers, steps, interleave=True)
     B = Batch(samplers)
     B.sweep(3)
     t0 = time.perf_counter()
@@ -260,7 +277,8 @@ def small_batch(steps, count=24):
     for S in samplers:
         S.close()
     return {"analyses": count, "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
-            "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat}
+            "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat,
+            "concurrent_streams_draws_per_s": draws * steps / conc, "batched_over_concurrent_streams": conc / bat}
 
 
 def load_profile(path):

@@ -95,6 +95,8 @@ def lib():
         "bsg_nifti_write_f64": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, pd]),
         "bsg_design_read": (C.c_int, [C.c_char_p, C.c_int, pi, pf, C.c_char_p, C.c_char_p]),
         "bsg_autotune": (C.c_int, [vp, C.c_int, pi, pd]),
+        "bsg_set_lane_split": (C.c_int, [vp, C.c_int]),
+        "bsg_get_lane_split": (C.c_int, [vp, pi]),
         "bsg_batch_create": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
         "bsg_batch_destroy": (C.c_int, [vp]),
         "bsg_batch_sweep": (C.c_int, [vp, C.c_int]),
@@ -280,6 +282,17 @@ class Sampler:
         ms = np.empty(4, np.float64)
         _check(lib().bsg_autotune(self._h, int(reps), C.byref(g), _ptr(ms, C.c_double)))
         return g.value, ms
+
+    @property
+    def lane_split(self):
+        """Lanes per voxel G of the Z-free latent kernel the next sweep uses."""
+        g = C.c_int()
+        _check(lib().bsg_get_lane_split(self._h, C.byref(g)))
+        return g.value
+
+    @lane_split.setter
+    def lane_split(self, g):
+        _check(lib().bsg_set_lane_split(self._h, int(g)))
 
     def coef_means(self, chain=0):
         """Plane means of the coefficients over the mask (alphaMean)."""

@@ -23,6 +23,8 @@ HIP_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: no AGPR copies around each MFMA
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

@@ -1057,6 +1057,28 @@ uint64_t bsg_devseed(unsigned long seed[3])
 
 }  // extern "C"
 
+int bsg_set_lane_split(bsg_ctx *ctx, int g)
+{
+  CTXCHK(ctx);
+  if (g != 0 && g != 1 && g != 2 && g != 4 && g != 8) {
+    set_error("bsg_set_lane_split: G must be 0, 1, 2, 4 or 8");
+    return BSG_ERR_ARG;
+  }
+  ctx->pb.lat_g = g;
+  return BSG_OK;
+}
+
+int bsg_get_lane_split(bsg_ctx *ctx, int *g)
+{
+  CTXCHK(ctx);
+  if (!g) {
+    set_error("bsg_get_lane_split: NULL output");
+    return BSG_ERR_ARG;
+  }
+  *g = ctx->pb.lat_g > 0 ? ctx->pb.lat_g : bsg::latent_vl_groups(ctx->pb);
+  return BSG_OK;
+}
+
 // ---- autotuning of the Z-free latent kernel ----------------------------
 int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
 {
@@ -1118,9 +1140,10 @@ struct bsg_batch {
   bsg::DevProblem *d_pbs = nullptr;
   bsg::WorkItem *d_items = nullptr;
   bsg::Accum *d_accs = nullptr;
-  // item ranges per phase (latent, coef 0, coef 1, wishart, accum) and P
-  int off[5][bsg::MAX_P + 1] = {}, cnt[5][bsg::MAX_P + 1] = {};
-  size_t lat_lds = 0;
+  // item ranges per phase (latent, coef 0, coef 1, wishart, accum, latent of
+  // the members drawn by inversion at G = 4) and P
+  int off[6][bsg::MAX_P + 1] = {}, cnt[6][bsg::MAX_P + 1] = {};
+  size_t lat_lds[bsg::MAX_P + 1] = {};  // per P: the largest X staging of its members
   bool bmf = false;
   std::vector<bsg::Accum> h_accs;
 
@@ -1151,7 +1174,7 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     return BSG_ERR_NOMEM;
   }
   std::vector<bsg::DevProblem> pbs;
-  std::vector<bsg::WorkItem> lists[5];
+  std::vector<bsg::WorkItem> lists[6];
   for (int a = 0; a < n; a++) {
     bsg_ctx *c = members[a];
     if (!c) {
@@ -1171,10 +1194,11 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     bsg::DevProblem pb = c->pb;
     if (pb.lat_g <= 0) pb.lat_g = bsg::latent_vl_groups(pb);
     pbs.push_back(pb);
-    b->lat_lds = std::max(b->lat_lds, bsg::latent_vl_lds_bytes(pb));
+    b->lat_lds[pb.P] = std::max(b->lat_lds[pb.P], bsg::latent_vl_lds_bytes(pb));
     const int tiles = bsg::latent_vl_tiles(pb, pb.lat_g);
+    const int lat_list = (pb.lat_g == 4 && c->opts.latent_method == 0) ? 5 : 0;
     for (int ch = 0; ch < c->C; ch++) {
-      for (int t = 0; t < tiles; t++) lists[0].push_back({a, ch, t});
+      for (int t = 0; t < tiles; t++) lists[lat_list].push_back({a, ch, t});
       for (int k = 0; k < (c->L.n[0] + 255) / 256; k++) lists[1].push_back({a, ch, k});
       for (int k = 0; k < (c->L.n[1] + 255) / 256; k++) lists[2].push_back({a, ch, k});
       lists[3].push_back({a, ch, 0});
@@ -1184,7 +1208,7 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
   }
   b->bmf = members[0]->opts.latent_method == 1;
   std::vector<bsg::WorkItem> all;
-  for (int k = 0; k < 5; k++)
+  for (int k = 0; k < 6; k++)
     for (int P = 1; P <= bsg::MAX_P; P++) {
       b->off[k][P] = (int)all.size();
       for (const bsg::WorkItem &w : lists[k])
@@ -1223,8 +1247,11 @@ static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
   bsg::BatchIndex bidx{};
   for (int i = 0; i < nsweeps; i++) {
     const uint32_t sw = b->m[0]->sweep;
-    for (int P = 1; P <= bsg::MAX_P; P++)
-      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds, b->bmf, sw, sa));
+    for (int P = 1; P <= bsg::MAX_P; P++) {
+      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds[P], b->bmf, false, sw,
+                                      sa));
+      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[5][P], b->cnt[5][P], b->lat_lds[P], false, true, sw, sa));
+    }
     if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, b->e_c1, 0));
     for (int P = 1; P <= bsg::MAX_P; P++)
       HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, sa));

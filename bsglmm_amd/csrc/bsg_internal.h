@@ -114,9 +114,10 @@ hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st);
 int latent_vl_groups(const DevProblem &pb);
 int latent_vl_tiles(const DevProblem &pb, int G);
 size_t latent_vl_lds_bytes(const DevProblem &pb);
-// batched analyses: one launch per phase and covariate count P over the work items
+// batched analyses: one launch per phase and covariate count P over the work
+// items; latent items of members at G = 4 (inversion) take their own launch (g4)
 hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
-                               uint32_t sweep, hipStream_t st);
+                               bool g4, uint32_t sweep, hipStream_t st);
 hipError_t launch_coef_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, int colour, bool prec,
                              uint32_t sweep, hipStream_t st);
 hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,

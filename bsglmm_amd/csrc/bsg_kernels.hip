@@ -65,31 +65,6 @@ __device__ __forceinline__ double u01d(uint32_t x)
 // ---------------------------------------------------------------------------
 // Deterministic block reductions in FP64 (fixed order, so sums repeat bit for bit).
 // ---------------------------------------------------------------------------
-// Sum of the per-block partials in a fixed order: thread t adds blocks
-// t, t+256, ... of every entry, then a fixed LDS tree over the 256 threads.
-template <int NE>
-__device__ void sum_partials(int nparts, const double *part, double *T)
-{
-  __shared__ double red[256];
-  double acc[NE];
-#pragma unroll
-  for (int e = 0; e < NE; e++) acc[e] = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += 256) {
-#pragma unroll
-    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
-  }
-  for (int e = 0; e < NE; e++) {
-    red[threadIdx.x] = acc[e];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) T[e] = red[0];
-    __syncthreads();
-  }
-}
-
 // Block sum of NE doubles in fixed order; thread e < NE of the block gets entry e.
 template <int NE>
 __device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst)
@@ -107,6 +82,23 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst)
     const int e = threadIdx.x;
     dst[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
   }
+}
+
+// Sum of the per-block partials in a fixed order: thread t adds blocks
+// t, t + 256, ... of every entry, then one block reduction of all NE entries
+// (block_sum_store). T (shared memory) holds the sums for every thread on return.
+template <int NE>
+__device__ void sum_partials(int nparts, const double *part, double *T)
+{
+  double acc[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) acc[e] = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
+  }
+  block_sum_store<NE>(acc, T);
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -676,11 +668,165 @@ __device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint
   }
 }
 
+// ---------------------------------------------------------------------------
+// latent_vl_mfm_body: the inversion path at G = 4 (16 voxels x 4 subject
+// groups per wave) with the covariate means on the matrix cores. Per round
+// lane l (voxel j = l % 16, group g = l / 16) draws one quad of its group;
+// one v_mfma_f32_16x16x4_f32 (FP32 products and sums) forms all 256 means of
+// the wave's round. For that shape lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and receives D[4 (l / 16) + v][l % 16], v = 0..3, so with
+//   A[4g + v][c] = x_c of subject v of group g's quad,  B[c][j] = beta_c,j,
+//   C = b WM_j
+// lane (j, g) receives the four means of its own quad. P = 5..8 takes a second
+// MFMA for covariates 4..7. An FP32 MFMA holds the SIMD for 32 cycles and
+// does not overlap the VALU here, so it pays only when it replaces more FMAs
+// than that: for P >= 3 (P x 4 FMAs of the VALU form); P <= 2 keeps the FMA
+// chain. X'Z stays on FP32 FMAs (the X'Z product has P of the MFMA's 16 rows
+// and would cost more MFMA cycles than the FMAs it replaces). X is staged for
+// 4R quads, zero past NQ, so every lane reads in bounds in every round; the
+// means differ from the FMA chain of latent_vl_body only in FP32 rounding.
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int P, bool WMU>
+__device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_t sweep, int chain, int t0,
+                                                   int tstep, int slot)
+{
+  static_assert(P <= 8, "means take at most two MFMAs");
+  constexpr bool MFM = P >= 3;
+  extern __shared__ float4 xs[];  // [4R][P]
+  const int R = (pb.NQ + 3) >> 2;
+  for (int i = threadIdx.x; i < 4 * R * P; i += blockDim.x) {
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < pb.NQ * P) {
+      const int q = i / P, c = i - q * P;
+      v = *reinterpret_cast<const float4 *>(pb.X + (size_t)c * pb.NS + 4 * q);
+    }
+    xs[i] = v;
+  }
+  __syncthreads();
+  const float *xsf = reinterpret_cast<const float *>(xs);
+  const int lane = threadIdx.x & 63, vl = lane & 15, g = lane >> 4;
+  const float b = pb.bwm[chain];
+  const float eps = pb.trunc_eps;
+  float xsum[P];
+#pragma unroll
+  for (int c = 0; c < P; c++) xsum[c] = pb.xsum[c];
+  // round-0 LDS offsets (floats) of this lane's A entries of the means;
+  // covariates past P read a valid X entry against a zero B
+  const int ga = vl >> 2, ea = vl & 3;
+  const int am0 = ((ga * R) * P + min(g, P - 1)) * 4 + ea;
+  const int am1 = ((ga * R) * P + min(g + 4, P - 1)) * 4 + ea;
+  const int q0 = g * R;
+  const int tiles = (pb.ve - pb.vb + 63) / 64;
+  double bpart = 0.0;
+  for (int t = t0; t < tiles; t += tstep) {
+    const int s = pb.vb + (t * 4 + (threadIdx.x >> 6)) * 16 + vl;
+    const bool valid = s < pb.ve;
+    const int sv = valid ? s : pb.ve - 1;
+    const float wmv = pb.wm[sv];
+    const float bw = b * wmv;
+    const float *bp = pb.beta + ((size_t)chain * pb.V + sv) * P;
+    float bs[P];
+#pragma unroll
+    for (int c = 0; c < P; c++) bs[c] = bp[c];
+    float bm0 = 0.0f, bm1 = 0.0f;  // B of the means: beta_g and beta_(g+4) of this voxel
+#pragma unroll
+    for (int c = 0; c < P; c++) {
+      bm0 = (g == c) ? bs[c] : bm0;
+      if (c >= 4) bm1 = (g + 4 == c) ? bs[c] : bm1;
+    }
+    const f32x4 bw4 = {bw, bw, bw, bw};
+    const uint32_t *yrow = pb.ybits + (size_t)sv * pb.NW;
+    const uint32_t cbase = (uint32_t)sv * (uint32_t)pb.NQ;
+    float acc[P], zs = 0.0f;
+#pragma unroll
+    for (int c = 0; c < P; c++) acc[c] = 0.0f;
+    int am = 0;  // step of the means' A offsets
+    for (int r = 0; r < R; r++, am += 4 * P) {
+      const int q = q0 + r;  // passes NQ only in the last group, where X reads zero
+      float4 x[P];  // read before the draw only for the FMA means: not live through it otherwise
+      if (!MFM) {
+#pragma unroll
+        for (int c = 0; c < P; c++) x[c] = xs[q * P + c];
+      }
+      float mu[4];
+      if (MFM) {
+        f32x4 m4 = __builtin_amdgcn_mfma_f32_16x16x4f32(xsf[am0 + am], bm0, bw4, 0, 0, 0);
+        if (P > 4) m4 = __builtin_amdgcn_mfma_f32_16x16x4f32(xsf[am1 + am], bm1, m4, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) mu[v] = m4[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++) mu[v] = bw;
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+          mu[0] = fmaf(bs[c], x[c].x, mu[0]);
+          mu[1] = fmaf(bs[c], x[c].y, mu[1]);
+          mu[2] = fmaf(bs[c], x[c].z, mu[2]);
+          mu[3] = fmaf(bs[c], x[c].w, mu[3]);
+        }
+      }
+      const int qy = min(q, pb.NQ - 1);
+      const uint32_t yw = yrow[qy >> 3] >> ((qy & 7) * 4);
+      uint32_t rr[4];
+      philox10(cbase + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain, pb.key0, pb.key1, rr);
+      float z[4];
+      latent_quad(mu, yw, rr, eps, z);  // X is zero past N: padding subjects add nothing to m_j
+      if (WMU && 4 * q + 3 >= pb.N) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) z[l] = (4 * q + l < pb.N) ? z[l] : 0.0f;
+      }
+      if (MFM) {
+#pragma unroll
+        for (int c = 0; c < P; c++) x[c] = xs[q * P + c];
+      }
+#pragma unroll
+      for (int c = 0; c < P; c++) {
+        acc[c] = fmaf(x[c].x, z[0], acc[c]);
+        acc[c] = fmaf(x[c].y, z[1], acc[c]);
+        acc[c] = fmaf(x[c].z, z[2], acc[c]);
+        acc[c] = fmaf(x[c].w, z[3], acc[c]);
+      }
+      if (WMU) zs += (z[0] + z[1]) + (z[2] + z[3]);
+    }
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
+      if (WMU) zs += __shfl_xor(zs, off, 64);
+    }
+    if (valid && g == 0) {
+      float *m = pb.M + ((size_t)chain * pb.V + s) * P;
+#pragma unroll
+      for (int c = 0; c < P; c++) m[c] = acc[c];
+      if (WMU) {
+        double r = (double)zs;
+#pragma unroll
+        for (int c = 0; c < P; c++) r -= (double)xsum[c] * (double)bs[c];
+        bpart += (double)wmv * r;
+      }
+    }
+  }
+  if (WMU) {
+    double bp[1] = {bpart};
+    block_sum_store<1>(bp, pb.bstat + (size_t)chain * BSTAT_SLOTS + slot);
+  }
+}
+
 template <int P, bool WMU, bool BMF>
 __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, int G, uint32_t sweep)
 {
   sweep += *pb.sweep_base;
   latent_vl_body<P, WMU, BMF>(pb, G, sweep, blockIdx.y, blockIdx.x, gridDim.x, blockIdx.x);
+}
+
+// the inversion path at G = 4; a kernel of its own so its registers are its own
+template <int P, bool WMU>
+__global__ __launch_bounds__(256) void k_latent_vl_mfm(DevProblem pb, uint32_t sweep)
+{
+  sweep += *pb.sweep_base;
+  latent_vl_mfm_body<P, WMU>(pb, sweep, blockIdx.y, blockIdx.x, gridDim.x, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -799,7 +945,6 @@ __global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int 
 // output layout is lane l -> D[4 (l / 16) + i][l % 16].
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo)
 {
@@ -1095,20 +1240,13 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
     nb[2 * d] = v.x;
     nb[2 * d + 1] = v.y;
   }
-  float bk[6][P];
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     if (nb[d] >= 0) {
       nn++;
       const float *src = betc + (size_t)nb[d] * P;
 #pragma unroll
-      for (int c = 0; c < P; c++) {
-        bk[d][c] = src[c];
-        sv[c] += bk[d][c];
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < P; c++) bk[d][c] = 0.0f;
+      for (int c = 0; c < P; c++) sv[c] += src[c];
     }
   }
   // m_j = X'(Z_j - b WM_j) = X'Z_j - b WM_j X'1, with the b of this sweep
@@ -1181,9 +1319,17 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
 #pragma unroll
   for (int e = 0; e < NE; e++) acc[e] = 0.0;
   if (active) {
+    // the neighbours are of the other colour, unchanged by this kernel: read
+    // again here rather than held in registers through the draw
 #pragma unroll
     for (int d = 0; d < 6; d++)
-      if (nb[d] >= 0) add_pair<P>(out, bk[d], (d & 1) != 0, acc);
+      if (nb[d] >= 0) {
+        float bk[P];
+        const float *src = betc + (size_t)nb[d] * P;
+#pragma unroll
+        for (int c = 0; c < P; c++) bk[c] = src[c];
+        add_pair<P>(out, bk, (d & 1) != 0, acc);
+      }
   }
   double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
   block_sum_store<NE>(acc, parts + (size_t)blk * NE);
@@ -1382,6 +1528,15 @@ __global__ __launch_bounds__(256) void k_latent_multi(const DevProblem *pbs, con
   latent_vl_body<P, false, BMF>(pb, pb.lat_g, sweep + *pb.sweep_base, it.chain, it.blk, 1 << 30, 0);
 }
 
+// members drawn by inversion at G = 4
+template <int P>
+__global__ __launch_bounds__(256) void k_latent_multi_mfm(const DevProblem *pbs, const WorkItem *items, uint32_t sweep)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  latent_vl_mfm_body<P, false>(pb, sweep + *pb.sweep_base, it.chain, it.blk, 1 << 30, 0);
+}
+
 template <int P, bool PREC>
 __global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const WorkItem *items, int colour,
                                                     uint32_t sweep)
@@ -1438,19 +1593,24 @@ static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
     default: return hipErrorInvalidValue; \
   }
 
-// Subject groups per voxel for k_latent_vl: enough lanes per voxel to give
-// the chip many waves, few enough that every lane keeps several quads.
+// Subject groups per voxel for the Z-free latent kernels: enough lanes per
+// voxel to give the chip many waves, few enough that every lane keeps several
+// quads. Past 16 quads the inversion path keeps G = 4, where its means run on
+// the matrix cores.
 int latent_vl_groups(const DevProblem &pb)
 {
   if (pb.NQ <= 4) return 1;
   if (pb.NQ <= 16) return 2;
-  if (pb.NQ <= 128) return 4;
+  if (pb.NQ <= 128 || pb.latent_method == 0) return 4;
   return 8;
 }
 
+// X staging of either body (the inversion path at G = 4 pads X to 4R quads,
+// R = ceil(NQ / 4)), plus the fallback areas of method 1
 static size_t latent_vl_lds(const DevProblem &pb)
 {
-  return (size_t)pb.NQ * pb.P * sizeof(float4) + (pb.latent_method == 1 ? vl_fallback_lds(pb.P) : 0);
+  const size_t x = (size_t)(4 * ((pb.NQ + 3) / 4)) * pb.P * sizeof(float4);
+  return x + (pb.latent_method == 1 ? vl_fallback_lds(pb.P) : 0);
 }
 
 static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st, int *parts)
@@ -1467,6 +1627,9 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
     if (bmf) {                                                                                             \
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, true>), grid, dim3(256), lds, st, pb, G, sweep);   \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, true>), grid, dim3(256), lds, st, pb, G, sweep);      \
+    } else if (G == 4) {                                                                                   \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep);         \
+      else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep);            \
     } else {                                                                                               \
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, false>), grid, dim3(256), lds, st, pb, G, sweep);  \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, false>), grid, dim3(256), lds, st, pb, G, sweep);     \
@@ -1666,14 +1829,16 @@ hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st)
 }
 
 hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
-                               uint32_t sweep, hipStream_t st)
+                               bool g4, uint32_t sweep, hipStream_t st)
 {
   if (n <= 0) return hipSuccess;
-  if (lds > 65536) return hipErrorInvalidValue;
-#define L_(PP)                                                                                                 \
-  do {                                                                                                         \
-    if (bmf) hipLaunchKernelGGL((k_latent_multi<PP, true>), dim3((unsigned)n), dim3(256), lds, st, pbs, items, sweep); \
-    else hipLaunchKernelGGL((k_latent_multi<PP, false>), dim3((unsigned)n), dim3(256), lds, st, pbs, items, sweep);    \
+  if (lds > 65536 || (g4 && bmf)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)n);
+#define L_(PP)                                                                                             \
+  do {                                                                                                     \
+    if (g4) hipLaunchKernelGGL((k_latent_multi_mfm<PP>), grid, dim3(256), lds, st, pbs, items, sweep);       \
+    else if (bmf) hipLaunchKernelGGL((k_latent_multi<PP, true>), grid, dim3(256), lds, st, pbs, items, sweep); \
+    else hipLaunchKernelGGL((k_latent_multi<PP, false>), grid, dim3(256), lds, st, pbs, items, sweep);       \
   } while (0)
   BSG_DISPATCH_P(P, L_)
 #undef L_

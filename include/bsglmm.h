@@ -178,6 +178,11 @@ int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count);     /* FP64 Cholesky 
  * chain state is not changed), and keeps the fastest for later sweeps.
  * best_g receives it; ms_per_g[4] (optional) the mean times, -1 if skipped. */
 int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g);
+/* Sets the lane split G of the Z-free latent kernel (1, 2, 4 or 8; 0 = chosen
+ * from N), e.g. a split bsg_autotune found in an earlier process. Sweeps are
+ * exact at every split; the split changes FP32 summation order only. */
+int bsg_set_lane_split(bsg_ctx *ctx, int g);
+int bsg_get_lane_split(bsg_ctx *ctx, int *g); /* the split the next sweep uses */
 typedef struct bsg_batch bsg_batch;
 /* Groups up to 32 contexts (independent analyses: own masks, N, P and chain
  * counts) so that each phase of a sweep is one kernel launch over all of

@@ -156,13 +156,28 @@ def test_z_free_fused_sweep(nsubs, ncovar, nchains, method):
     """Without Z (z_store 2) X'Z is formed from draws held in registers: it
     must match X' times the oracle's own draws (same Philox counters), and the
     coefficient updates that consume it must match the oracle's."""
-    print(f"case c1 z_store=2 method={method} N={nsubs} P={ncovar} chains={nchains}", flush=True)
+    check_z_free(nsubs, ncovar, nchains, method)
+
+
+# Lane splits of the inversion path: G = 4 forms the means on the matrix
+# cores with one MFMA for P = 3, 4, two for P = 5..8 and none for P <= 2.
+@pytest.mark.parametrize("nsubs,ncovar,g", [(300, 6, 1), (300, 6, 2), (300, 6, 4), (300, 6, 8), (200, 8, 4),
+                                            (90, 2, 4), (57, 5, 4), (13, 3, 4)])
+def test_z_free_every_lane_split(nsubs, ncovar, g):
+    check_z_free(nsubs, ncovar, 2 if g == 4 else 1, 0, g)
+
+
+def check_z_free(nsubs, ncovar, nchains, method, g=0):
+    print(f"case c1 z_store=2 method={method} N={nsubs} P={ncovar} chains={nchains} G={g}", flush=True)
     d = synth.generate("c1", nsubs=nsubs, ncovar=ncovar)
     nx, ny, nz = d["grid"]
     lay = oracle.layout(d["mask"], nx, ny, nz)
     L = Layout(d["mask"], nx, ny, nz)
     S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2,
                 latent_method=method)
+    if g:
+        S.lane_split = g
+        assert S.lane_split == g
     O = oracle.Oracle(lay, d["covar"], d["data"], seed=SEED)
     P = S.P
     rng = np.random.default_rng(11)
@@ -226,13 +241,15 @@ def test_chains_are_independent_streams():
     assert rel_err(S2.get_z(chain=2), O.update_z(sc0, 0)) < 1e-5
 
 
-@pytest.mark.parametrize("z_store,xtz_mode,method", [(0, 2, 0), (0, 0, 0), (2, 2, 0), (2, 2, 1)])
-def test_wm_coefficient_sweep(z_store, xtz_mode, method):
+@pytest.mark.parametrize("z_store,xtz_mode,method,nsubs", [(0, 2, 0, None), (0, 0, 0, None), (2, 2, 0, None),
+                                                           (2, 2, 1, None), (2, 2, 0, 150)])
+def test_wm_coefficient_sweep(z_store, xtz_mode, method, nsubs):
     """With a WM image the sweep draws b between the latent draw and the
     coefficients (mcmc.cpp:344-356): Z uses the old b, the coefficients use
-    X'(Z - b_new WM). Each step is checked against the oracle."""
-    print(f"case c1 WM z_store={z_store} xtz_mode={xtz_mode}", flush=True)
-    d = synth.generate("c1")
+    X'(Z - b_new WM). Each step is checked against the oracle. N = 150 puts
+    the Z-free form at four lanes per voxel (means on the matrix cores)."""
+    print(f"case c1 WM z_store={z_store} xtz_mode={xtz_mode} method={method} N={nsubs}", flush=True)
+    d = synth.generate("c1", nsubs=nsubs)
     nx, ny, nz = d["grid"]
     lay = oracle.layout(d["mask"], nx, ny, nz)
     L = Layout(d["mask"], nx, ny, nz)

@@ -8,7 +8,8 @@ checkerboard coefficient updates (two colours), FP64 pair sums and the device
 Wishart draw. All inputs are resident in HBM before timing starts.
 
 Headline form: X'Z is formed inside the latent kernel from draws held in
-registers, and Z is not written (k_latent_vl; Z is a transient of the sweep).
+registers (FP32 FMA), and Z is not written (k_latent_vl_mfm, whose per-draw
+means x_i'b_j come from FP32 MFMAs; Z is a transient of the sweep).
 Beside it, in the same run: the fused kernel that also stores Z, and the
 sweep with a separate X'Z kernel re-reading Z (FP32 FMA, and BF16 hi+lo on
 MFMA). Prints one JSON line with the dominant kernel's roofline (its own
@@ -44,7 +45,7 @@ def parse():
     ap.add_argument("--batched-burnin", type=int, default=1000)
     ap.add_argument("--uneven-steps", type=int, default=50,
                     help="batch sweeps of the 12-analysis uneven batch (0 skips it)")
-    ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "k_latent_vl_c2.json"),
+    ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "k_latent_vl_mfm_c2.json"),
                     help="measured traffic and VALU counts of the dominant kernel (rocprofv3 summary)")
     return ap.parse_args()
 
@@ -310,7 +311,7 @@ def main():
 
     value = a.steps / head["wall"]
     sweep_bytes = N * V * (8 + 1 / 8) + V * (24 * P + 5)  # algorithmic bytes of one sweep
-    # k_latent_vl moves y bits, b_j, WM_j in and m_j out: N V / 8 + 8 P V + 4 V
+    # the Z-free latent kernel moves y bits, b_j, WM_j in and m_j out: N V / 8 + 8 P V + 4 V
     vl_bytes = N * V / 8 + 8 * P * V + 4 * V
     lat_s = head["lat_ms"] / 1e3
     achieved = vl_bytes / lat_s / 1e9
@@ -338,7 +339,7 @@ def main():
         "warmup": a.warmup, "ms_per_step": 1e3 * head["wall"] / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{a.config}: grid {nx}x{ny}x{nz} ellipsoidal-shell mask, plain single chain, "
-                               f"FP32 state, FP32-FMA X'Z fused into the latent kernel, Z not kept",
+                               f"FP32 state, X'Z on FP32 FMA fused into the latent kernel (means x'b on FP32 MFMA), Z not kept",
                    "V": V, "N": N, "P": P, "chains": 1},
         "latent_draws_per_s": value * N * V,
         "latent_kernel_ms": head["lat_ms"],
@@ -348,7 +349,7 @@ def main():
                            "achieved": sweep_bytes * value / 1e9, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": sweep_bytes * value / 1e9 / HBM_PEAK_GBS},
         "copy_kernel_gbs": copy_gbs,
-        "roofline": {"kernel": "k_latent_vl (latent draw + X'Z, Z not kept)", "bound": "hbm",
+        "roofline": {"kernel": "k_latent_vl_mfm (latent draw + X'Z, Z not kept)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": vl_bytes, "valu_issue": valu},

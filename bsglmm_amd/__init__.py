@@ -386,7 +386,9 @@ def devseed(seed):
 
 class Batch:
     """Independent analyses (Samplers in the Z-free form) swept together:
-    each phase of a sweep is one kernel launch over every analysis and chain."""
+    each phase of a sweep is one kernel launch per covariate count over every
+    analysis and chain with that count (the latent phase one more for the
+    members at four lanes per voxel)."""
 
     def __init__(self, samplers):
         self.samplers = list(samplers)

@@ -516,33 +516,43 @@ static int accum_launch(bsg_ctx *ctx, hipStream_t st)
 
 // Full sweeps in the reference order (mcmc.cpp:311-394): latent draw (+ X'Z),
 // colour-0 then colour-1 coefficients, CAR precision. The CAR-precision draw
-// of sweep t (final pair-sum reduction and Wishart) runs on stream2 and only
-// has to finish before the colour-0 update of sweep t+1, so it overlaps the
-// latent draw of sweep t+1. Each call ends joined on `stream`.
+// of sweep t (final pair-sum reduction and Wishart) only has to finish before
+// the colour-0 update of sweep t+1, and nothing in the latent draw of t+1
+// reads it. Where the Z-free kernel allows (latent_carries_wishart), block 0
+// of the next latent launch draws it (w_inline), so a sweep is four launches
+// on one stream with no event waits; the last sweep of a call draws it in a
+// launch of its own. Otherwise it runs on stream2 beside the next latent draw,
+// joined by events (wish_pending). Each call ends joined on `stream`.
 // lat_ev (2 per sweep, optional) brackets each latent launch for timing.
 // One sweep of problem pb, whose Philox sweep word is *pb.sweep_base + off.
 // lat_ev2 (optional) brackets the latent launch.
 static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, hipEvent_t *lat_ev2,
-                         bool &wish_pending)
+                         bool &wish_pending, bool &w_inline)
 {
   hipStream_t sa = ctx->stream, sb = ctx->stream2;
   const bool fused = ctx->opts.xtz_mode == 2;
   const int V = ctx->L.totvox;
   if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[0], sa));
   int parts = 0;
-  HIPCHK(bsg::launch_latent(pb, fused, ctx->opts.z_store, off, sa, 0, V, &parts));
+  HIPCHK(bsg::launch_latent(pb, fused, ctx->opts.z_store, off, sa, 0, V, &parts, w_inline ? 1 : 0, off - 1));
+  w_inline = false;
   if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[1], sa));
   if (!fused) HIPCHK(bsg::launch_xtz(pb, ctx->opts.xtz_mode, sa, 0, V));
   HIPCHK(bsg::launch_bwm(pb, parts, off, sa));  // WM coefficient (mcmc.cpp:344-345)
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
+  wish_pending = false;
   HIPCHK(bsg::launch_coef(pb, 0, false, off, sa));
   if (ctx->L.n[1] > 0) {
     HIPCHK(bsg::launch_coef(pb, 1, true, off, sa));  // colour 1 + pair partials
-    HIPCHK(hipEventRecord(ctx->e_c0, sa));
-    HIPCHK(hipStreamWaitEvent(sb, ctx->e_c0, 0));
-    HIPCHK(bsg::launch_wishart(pb, true, off, sb));
-    HIPCHK(hipEventRecord(ctx->e_c1, sb));
-    wish_pending = true;
+    if (fused && ctx->opts.z_store == 2 && bsg::latent_carries_wishart(pb)) {
+      w_inline = true;  // drawn by the next latent launch, or at the end of the call
+    } else {
+      HIPCHK(hipEventRecord(ctx->e_c0, sa));
+      HIPCHK(hipStreamWaitEvent(sb, ctx->e_c0, 0));
+      HIPCHK(bsg::launch_wishart(pb, true, off, sb));
+      HIPCHK(hipEventRecord(ctx->e_c1, sb));
+      wish_pending = true;
+    }
   } else {
     HIPCHK(bsg::launch_pairs(pb, sa));
     HIPCHK(bsg::launch_wishart(pb, false, off, sa));
@@ -553,9 +563,9 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
 static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_every = 0)
 {
   hipStream_t sa = ctx->stream;
-  bool wish_pending = false;
+  bool wish_pending = false, w_inline = false;
   for (int i = 0; i < nsweeps; i++) {
-    const int rc = enqueue_sweep(ctx, ctx->pb, ctx->sweep, lat_ev ? lat_ev + 2 * i : nullptr, wish_pending);
+    const int rc = enqueue_sweep(ctx, ctx->pb, ctx->sweep, lat_ev ? lat_ev + 2 * i : nullptr, wish_pending, w_inline);
     if (rc != BSG_OK) return rc;
     ctx->sweep++;
     if (accum_every > 0 && (i + 1) % accum_every == 0) {
@@ -563,6 +573,7 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
       if (rc2 != BSG_OK) return rc2;
     }
   }
+  if (w_inline) HIPCHK(bsg::launch_wishart(ctx->pb, true, ctx->sweep - 1, sa));  // the last sweep's draw
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // join
   return BSG_OK;
 }
@@ -581,9 +592,10 @@ static int build_graph(bsg_ctx *ctx, int k)
   bsg::DevProblem pg = ctx->pb;
   pg.sweep_base = ctx->sweep_words + 1;
   HIPCHK(hipStreamBeginCapture(sa, hipStreamCaptureModeThreadLocal));
-  bool wish_pending = false;
+  bool wish_pending = false, w_inline = false;
   int rc = BSG_OK;
-  for (int i = 0; i < k && rc == BSG_OK; i++) rc = enqueue_sweep(ctx, pg, (uint32_t)i, nullptr, wish_pending);
+  for (int i = 0; i < k && rc == BSG_OK; i++) rc = enqueue_sweep(ctx, pg, (uint32_t)i, nullptr, wish_pending, w_inline);
+  if (rc == BSG_OK && w_inline && bsg::launch_wishart(pg, true, (uint32_t)(k - 1), sa) != hipSuccess) rc = BSG_ERR_HIP;
   if (rc == BSG_OK && wish_pending && hipStreamWaitEvent(sa, ctx->e_c1, 0) != hipSuccess) rc = BSG_ERR_HIP;
   if (rc == BSG_OK && bsg::launch_tick(ctx->sweep_words + 1, (uint32_t)k, sa) != hipSuccess) rc = BSG_ERR_HIP;
   hipGraph_t g = nullptr;

@@ -97,8 +97,11 @@ struct BatchIndex {
 // kernel launchers (bsg_kernels.hip)
 // Returns, in *bstat_parts, how many WM-statistic partials the launch wrote
 // (0 when the kernel used does not form them; see launch_bwm).
+// wish = 1: the launch also draws the CAR precision of sweep wsweep (block 0;
+// only where latent_carries_wishart)
 hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, int zstore, uint32_t sweep, hipStream_t st,
-                         int v0, int v1, int *bstat_parts = nullptr);
+                         int v0, int v1, int *bstat_parts = nullptr, int wish = 0, uint32_t wsweep = 0);
+bool latent_carries_wishart(const DevProblem &pb);
 // WM coefficient draw. parts > 0: the latent kernel already wrote that many
 // partials; parts == 0: they are formed here from the stored Z first.
 hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st);

@@ -821,12 +821,27 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, int G, uint32_
   latent_vl_body<P, WMU, BMF>(pb, G, sweep, blockIdx.y, blockIdx.x, gridDim.x, blockIdx.x);
 }
 
-// the inversion path at G = 4; a kernel of its own so its registers are its own
+template <int P>
+__device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part, uint32_t sweep);
+
+// the inversion path at G = 4; a kernel of its own so its registers are its
+// own. With wish = 1, block 0 draws the CAR precision of sweep wsweep (the
+// sweep before: its colour-1 partials are complete, and nothing here reads
+// Lambda) beside the latent blocks, instead of a launch of its own.
 template <int P, bool WMU>
-__global__ __launch_bounds__(256) void k_latent_vl_mfm(DevProblem pb, uint32_t sweep)
+__global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_vl_mfm(DevProblem pb, uint32_t sweep, int wish,
+                                                                       uint32_t wsweep)
 {
-  sweep += *pb.sweep_base;
-  latent_vl_mfm_body<P, WMU>(pb, sweep, blockIdx.y, blockIdx.x, gridDim.x, blockIdx.x);
+  const uint32_t base = *pb.sweep_base;
+  // P <= 6: the draw's registers spill only in its own code under the bound
+  if (P <= 6 && wish && blockIdx.x == 0) {
+    constexpr int NE = P * (P + 1) / 2;
+    wishart_block<P>(pb, blockIdx.y, (pb.V - pb.n0 + 255) / 256,
+                     pb.partials + (size_t)blockIdx.y * pb.max_parts * NE, wsweep + base);
+    return;
+  }
+  const int b = blockIdx.x - wish;
+  latent_vl_mfm_body<P, WMU>(pb, sweep + base, blockIdx.y, b, gridDim.x - wish, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -1613,23 +1628,26 @@ static size_t latent_vl_lds(const DevProblem &pb)
   return x + (pb.latent_method == 1 ? vl_fallback_lds(pb.P) : 0);
 }
 
-static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st, int *parts)
+static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st, int *parts,
+                                   int wish, uint32_t wsweep)
 {
   const size_t lds = latent_vl_lds(pb);
   const int vpb = 4 * (64 / G);  // voxels per block tile
   const long long tiles = ((long long)(pb.ve - pb.vb) + vpb - 1) / vpb;
   const long long per = (tiles + BSTAT_SLOTS - 1) / BSTAT_SLOTS;  // tiles per block, equal shares
-  dim3 grid((unsigned)((tiles + per - 1) / per), (unsigned)pb.nchains);
+  const int nlb = (int)((tiles + per - 1) / per);
   const bool wmu = pb.wm_update != 0, bmf = pb.latent_method == 1;
   if (G != 1 && G != 2 && G != 4 && G != 8) return hipErrorInvalidValue;
+  if (wish && (G != 4 || bmf || pb.P > 6)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)(nlb + wish), (unsigned)pb.nchains);
 #define V_(PP)                                                                                             \
   do {                                                                                                     \
     if (bmf) {                                                                                             \
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, true>), grid, dim3(256), lds, st, pb, G, sweep);   \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, true>), grid, dim3(256), lds, st, pb, G, sweep);      \
     } else if (G == 4) {                                                                                   \
-      if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep);         \
-      else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep);            \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep); \
+      else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep);    \
     } else {                                                                                               \
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, false>), grid, dim3(256), lds, st, pb, G, sweep);  \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, false>), grid, dim3(256), lds, st, pb, G, sweep);     \
@@ -1637,8 +1655,16 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
   } while (0)
   BSG_DISPATCH_P(pb.P, V_)
 #undef V_
-  if (parts) *parts = wmu ? (int)grid.x : 0;
+  if (parts) *parts = wmu ? nlb : 0;
   return hipGetLastError();
+}
+
+// The Z-free latent launch can carry the previous sweep's Wishart draw (block
+// 0 of k_latent_vl_mfm): inversion draws at four lanes per voxel.
+bool latent_carries_wishart(const DevProblem &pb)
+{
+  const int G = pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb);
+  return G == 4 && pb.P <= 6 && pb.latent_method == 0 && latent_vl_lds(pb) <= 65536;
 }
 
 // Voxel ranges: latent and X'Z launches cover the sorted voxels [v0, v1), so a
@@ -1652,18 +1678,19 @@ static bool set_range(DevProblem &pb, int v0, int v1)
 }
 
 hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t sweep, hipStream_t st, int v0, int v1,
-                         int *bstat_parts)
+                         int *bstat_parts, int wish, uint32_t wsweep)
 {
   DevProblem pb = pb0;
   if (bstat_parts) *bstat_parts = 0;
   if (!set_range(pb, v0, v1)) return hipErrorInvalidValue;
-  if (v0 == v1) return hipSuccess;
+  if (v0 == v1 && !wish) return hipSuccess;
   if (fused && zstore == 2) {
     // the Z-free form needs X in LDS; the WM statistic needs the whole voxel range
     if (latent_vl_lds(pb) > 65536) return hipErrorInvalidValue;
     if (pb.wm_update && (v0 != 0 || v1 != pb.V)) return hipErrorInvalidValue;
-    return launch_latent_vl(pb, pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb), sweep, st, bstat_parts);
+    return launch_latent_vl(pb, pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb), sweep, st, bstat_parts, wish, wsweep);
   }
+  if (wish) return hipErrorInvalidValue;
   int lshift, reps;
   lane_grouping(pb, lshift, reps);
   const int vpw = 64 >> lshift;

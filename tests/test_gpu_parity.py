@@ -288,13 +288,49 @@ def test_wm_coefficient_sweep(z_store, xtz_mode, method, nsubs):
     assert abs(b_gpu - s_orc / w_orc) < 6.0 / np.sqrt(w_orc)
 
 
-@pytest.mark.parametrize("z_store,xtz_mode,wm", [(2, 2, False), (0, 1, True)])
-def test_graph_replays_equal_direct_sweeps(z_store, xtz_mode, wm):
+@pytest.mark.parametrize("nsubs,ncovar,nchains", [(150, 4, 2), (90, 6, 1), (300, 3, 1)])
+def test_wishart_carried_by_next_latent_launch(nsubs, ncovar, nchains):
+    """At four lanes per voxel block 0 of the Z-free latent launch of sweep t+1
+    draws the CAR precision of sweep t, and the last sweep of a call draws it in
+    a launch of its own. After every call Lambda must be the oracle's Wishart of
+    the GPU's own pair sums, and sweeps split over several calls must equal the
+    same sweeps in one call bit for bit."""
+    print(f"case c1 carried Wishart N={nsubs} P={ncovar} chains={nchains}", flush=True)
+    d = synth.generate("c1", nsubs=nsubs, ncovar=ncovar)
+    nx, ny, nz = d["grid"]
+    lay = oracle.layout(d["mask"], nx, ny, nz)
+    L = Layout(d["mask"], nx, ny, nz)
+    mk = lambda: Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2)
+    A, B = mk(), mk()
+    assert A.lane_split == 4
+    O = oracle.Oracle(lay, d["covar"], d["data"], seed=SEED)
+    for n in (1, 3, 4):
+        A.sweep(n)
+        A.synchronize()
+        for c in range(nchains):
+            O.set_chain(c)
+            t = A.get_pair_sums(c)
+            assert np.max(np.abs(t - O.pair_sums(A.get_spat_coef(c))) / np.abs(t).max()) < 1e-12
+            lam_orc = O.wishart(t, lay["totvox"] + 19, A.sweep_index() - 1)
+            assert rel_err(A.get_prec(c), lam_orc, floor=np.abs(lam_orc).max()) < 1e-6
+    B.sweep(8)
+    B.synchronize()
+    assert A.sweep_index() == B.sweep_index()
+    for c in range(nchains):
+        assert np.array_equal(A.get_spat_coef(c), B.get_spat_coef(c))
+        assert np.array_equal(A.get_prec(c), B.get_prec(c))
+
+
+@pytest.mark.parametrize("z_store,xtz_mode,wm,nsubs", [(2, 2, False, None), (0, 1, True, None), (2, 2, False, 150),
+                                                       (2, 2, True, 150)])
+def test_graph_replays_equal_direct_sweeps(z_store, xtz_mode, wm, nsubs):
     """Sweeps replayed from a captured HIP graph (device sweep base advanced by
     the graph's last node) give bit-identical state to direct launches, over
-    several replays with direct sweeps and host reads between them."""
-    print(f"case c1 graph z_store={z_store} xtz_mode={xtz_mode} wm={wm}", flush=True)
-    d = synth.generate("c1")
+    several replays with direct sweeps and host reads between them. N = 150
+    runs at four lanes per voxel, where each latent launch carries the previous
+    sweep's Wishart draw."""
+    print(f"case c1 graph z_store={z_store} xtz_mode={xtz_mode} wm={wm} N={nsubs}", flush=True)
+    d = synth.generate("c1", nsubs=nsubs)
     nx, ny, nz = d["grid"]
     L = Layout(d["mask"], nx, ny, nz)
     w = np.random.default_rng(1).uniform(0, 1, L.totvox).astype(np.float32) if wm else None

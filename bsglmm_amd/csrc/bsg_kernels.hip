@@ -1200,9 +1200,18 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
 #pragma unroll
     for (int j = k + 1; j < P; j++) d -= U[k][j] * U[k][j];
     if (!(d >= pivot_min)) return false;
-    T ukk = sqrt(d);
+    T ukk, inv;
+    if constexpr (sizeof(T) == 4) {
+      // pivots are >= pivot_min (no denormals): hardware square root and
+      // reciprocal, 1 ulp, in place of the library's correctly rounded forms
+      ukk = __builtin_amdgcn_sqrtf(d);
+      inv = __builtin_amdgcn_rcpf(ukk);
+    } else {
+      ukk = sqrt(d);
+      inv = T(1) / ukk;
+    }
     U[k][k] = ukk;
-    dinv[k] = T(1) / ukk;
+    dinv[k] = inv;
 #pragma unroll
     for (int i = 0; i < k; i++) {
       T v = Q[i][k];
@@ -1289,11 +1298,12 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
     philox10((uint32_t)s, sweep, TAG_COEF | ((uint32_t)g << 4), (uint32_t)chain, pb.key0, pb.key1, rr);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      float rad = sqrtf(-2.0f * logf(u01f(rr[2 * h])));
-      float sn, cs;
-      sincospif(2.0f * u01f(rr[2 * h + 1]), &sn, &cs);
-      if (4 * g + 2 * h < P) z[4 * g + 2 * h] = rad * cs;
-      if (4 * g + 2 * h + 1 < P) z[4 * g + 2 * h + 1] = rad * sn;
+      // Box-Muller with the hardware v_log (log2), v_sqrt and v_sin/v_cos
+      // (argument in revolutions): R = sqrt(-2 ln2 log2 u0), angle 2 pi u1
+      const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01f(rr[2 * h])));
+      const float ang = u01f(rr[2 * h + 1]);
+      if (4 * g + 2 * h < P) z[4 * g + 2 * h] = rad * __builtin_amdgcn_cosf(ang);
+      if (4 * g + 2 * h + 1 < P) z[4 * g + 2 * h + 1] = rad * __builtin_amdgcn_sinf(ang);
     }
   }
   float out[P];

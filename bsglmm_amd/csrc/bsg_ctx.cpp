@@ -31,6 +31,9 @@ struct bsg_ctx {
   uint32_t sweep = 0;  // index of the next sweep (the Philox counter word), same for all chains
   hipStream_t stream = nullptr;   // all work; the caller-visible stream
   hipStream_t stream2 = nullptr;  // CAR-precision draw of sweep t beside the latent draw of t+1
+  // the last Z-free latent launch was split: the X'Z of its leftover tiles is
+  // in partial sums until a coefficient kernel (or bsg_get_xtz) adds them
+  bool tail_pending = false;
   bsg::DevProblem pb{};
   std::vector<void *> allocs;
   hipEvent_t ev[7] = {};
@@ -295,6 +298,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.n0 = L.n[0];
   pb.vb = 0;
   pb.ve = V;
+  pb.zfree = o.z_store == 2 ? 1 : 0;
   // b is drawn each sweep only when a WM image is given (mcmc.cpp:344-345, UPDATE_WM)
   pb.wm_update = wm ? 1 : 0;
   pb.latent_method = o.latent_method;
@@ -341,6 +345,24 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
 
   ALLOC(pb.bwm, (size_t)C);
   ALLOC(pb.bstat, (size_t)C * bsg::BSTAT_SLOTS);
+
+  // tail split of the Z-free latent launch at four lanes per voxel: the
+  // leftover 64-voxel tiles of a full-range launch (T mod CUs) run as K
+  // pieces each when that spreads them over the chip (k_latent_vl_mfm)
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) {
+      set_error("bsg_create: cannot read the compute-unit count");
+      return BSG_ERR_HIP;
+    }
+    pb.ncu = ncu;
+    const int tiles = (V + 63) / 64, tail = tiles % ncu, R = (N + 15) / 16;
+    const int K = tail > 0 ? std::min(std::min(ncu / tail, R), 16) : 0;
+    pb.tail_tiles = K > 1 ? tail : 0;
+    pb.tail_k = K > 1 ? K : 1;
+    if (K > 1) ALLOC(pb.mpart, (size_t)C * tail * K * 64 * P);
+  }
 
   ALLOC(pb.flags, (size_t)4);
   ALLOC(ctx->sweep_words, (size_t)2);
@@ -396,11 +418,19 @@ int bsg_destroy(bsg_ctx *ctx)
     return BSG_ERR_ARG;                     \
   }
 
+// 1 when a full-range latent launch of pb in this context's form leaves the
+// X'Z of its leftover tiles in partial sums (tail split of the Z-free kernel)
+static int split_tail(const bsg_ctx *ctx, const bsg::DevProblem &pb)
+{
+  return ctx->opts.z_store == 2 && bsg::latent_split_active(pb, nullptr, nullptr) ? 1 : 0;
+}
+
 int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
   HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream, 0,
                             ctx->L.totvox, &ctx->bstat_parts));
+  ctx->tail_pending = split_tail(ctx, ctx->pb) != 0;
   return BSG_OK;
 }
 
@@ -489,7 +519,7 @@ int bsg_update_coef(bsg_ctx *ctx, int colour)
     set_error("bsg_update_coef: colour must be 0 or 1");
     return BSG_ERR_ARG;
   }
-  HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->sweep, ctx->stream));
+  HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->sweep, ctx->stream, ctx->tail_pending ? 1 : 0));
   return BSG_OK;
 }
 
@@ -535,15 +565,17 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
   if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[0], sa));
   int parts = 0;
   HIPCHK(bsg::launch_latent(pb, fused, ctx->opts.z_store, off, sa, 0, V, &parts, w_inline ? 1 : 0, off - 1));
+  ctx->tail_pending = false;  // the colour kernels of this sweep add the pieces
   w_inline = false;
   if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[1], sa));
   if (!fused) HIPCHK(bsg::launch_xtz(pb, ctx->opts.xtz_mode, sa, 0, V));
   HIPCHK(bsg::launch_bwm(pb, parts, off, sa));  // WM coefficient (mcmc.cpp:344-345)
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
   wish_pending = false;
-  HIPCHK(bsg::launch_coef(pb, 0, false, off, sa));
+  const int tail = split_tail(ctx, pb);
+  HIPCHK(bsg::launch_coef(pb, 0, false, off, sa, tail));
   if (ctx->L.n[1] > 0) {
-    HIPCHK(bsg::launch_coef(pb, 1, true, off, sa));  // colour 1 + pair partials
+    HIPCHK(bsg::launch_coef(pb, 1, true, off, sa, tail));  // colour 1 + pair partials
     if (fused && ctx->opts.z_store == 2 && bsg::latent_carries_wishart(pb)) {
       w_inline = true;  // drawn by the next latent launch, or at the end of the call
     } else {
@@ -824,10 +856,12 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st, 0, ctx->L.totvox));
     HIPCHK(bsg::launch_bwm(ctx->pb, parts, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
-    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st));
+    const int tail = split_tail(ctx, ctx->pb);
+    ctx->tail_pending = false;
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st, tail));
     HIPCHK(hipEventRecord(ctx->ev[3], st));
     const bool one_pass = ctx->L.n[1] > 0;
-    if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, st));
+    if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, st, tail));
     HIPCHK(hipEventRecord(ctx->ev[4], st));
     if (!one_pass) HIPCHK(bsg::launch_pairs(ctx->pb, st));
     HIPCHK(hipEventRecord(ctx->ev[5], st));
@@ -906,6 +940,23 @@ int bsg_get_xtz(bsg_ctx *ctx, int chain, float *m)
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipMemcpy(mm.data(), ctx->pb.M + (size_t)chain * V * P, sizeof(float) * mm.size(),
                    hipMemcpyDeviceToHost));
+  int nmain = 0, K = 1;
+  if (ctx->tail_pending && bsg::latent_split_active(ctx->pb, &nmain, &K)) {
+    // leftover tiles of a split launch: add the pieces in piece order, as the
+    // coefficient kernels do (the same FP32 sums)
+    const size_t per_tile = (size_t)K * 64 * P;
+    std::vector<float> parts(per_tile * ctx->pb.tail_tiles);
+    HIPCHK(hipMemcpy(parts.data(), ctx->pb.mpart + (size_t)chain * parts.size(), sizeof(float) * parts.size(),
+                     hipMemcpyDeviceToHost));
+    for (int s = nmain * 64; s < V; s++) {
+      const float *tp = parts.data() + (size_t)((s >> 6) - nmain) * per_tile + (size_t)(s & 63) * P;
+      for (int c = 0; c < P; c++) {
+        float sum = 0.0f;
+        for (int kk = 0; kk < K; kk++) sum += tp[(size_t)kk * 64 * P + c];
+        mm[(size_t)s * P + c] = sum;
+      }
+    }
+  }
   for (int s = 0; s < V; s++)
     for (int c = 0; c < P; c++) m[(size_t)c * V + ctx->L.rank_of_sorted[s]] = mm[(size_t)s * P + c];
   return BSG_OK;
@@ -921,6 +972,18 @@ int bsg_set_xtz(bsg_ctx *ctx, int chain, const float *m)
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipMemcpy(ctx->pb.M + (size_t)chain * V * P, mm.data(), sizeof(float) * mm.size(),
                    hipMemcpyHostToDevice));
+  int nmain = 0, K = 1;
+  if (ctx->tail_pending && bsg::latent_split_active(ctx->pb, &nmain, &K)) {
+    // this chain's leftover tiles still hold pieces: piece 0 = the new value,
+    // the others 0, so the piece sum the coefficient kernels form is m exactly
+    const size_t per_tile = (size_t)K * 64 * P;
+    std::vector<float> parts(per_tile * ctx->pb.tail_tiles, 0.0f);
+    for (int s = nmain * 64; s < V; s++)
+      for (int c = 0; c < P; c++)
+        parts[(size_t)((s >> 6) - nmain) * per_tile + (size_t)(s & 63) * P + c] = mm[(size_t)s * P + c];
+    HIPCHK(hipMemcpy(ctx->pb.mpart + (size_t)chain * parts.size(), parts.data(), sizeof(float) * parts.size(),
+                     hipMemcpyHostToDevice));
+  }
   return BSG_OK;
 }
 
@@ -1140,6 +1203,7 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
     return rc;
   }
   ctx->pb.lat_g = bg;
+  ctx->tail_pending = split_tail(ctx, ctx->pb) != 0;  // the last timed launch ran at this split
   if (best_g) *best_g = bg;
   return BSG_OK;
 }
@@ -1207,8 +1271,8 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     if (pb.lat_g <= 0) pb.lat_g = bsg::latent_vl_groups(pb);
     pbs.push_back(pb);
     b->lat_lds[pb.P] = std::max(b->lat_lds[pb.P], bsg::latent_vl_lds_bytes(pb));
-    const int tiles = bsg::latent_vl_tiles(pb, pb.lat_g);
     const int lat_list = (pb.lat_g == 4 && c->opts.latent_method == 0) ? 5 : 0;
+    const int tiles = lat_list == 5 ? bsg::latent_vl_blocks_g4(pb) : bsg::latent_vl_tiles(pb, pb.lat_g);
     for (int ch = 0; ch < c->C; ch++) {
       for (int t = 0; t < tiles; t++) lists[lat_list].push_back({a, ch, t});
       for (int k = 0; k < (c->L.n[0] + 255) / 256; k++) lists[1].push_back({a, ch, k});

@@ -73,6 +73,12 @@ struct DevProblem {
   int wm_update;
   double wm2;              // N sum_j WM_j^2
   double *bstat;           // [chain][BSTAT_SLOTS] block partials of sum_j WM_j (sum_i Z_ij - X'1 b_j)
+  // tail split of the Z-free latent launch at G = 4 (k_latent_vl_mfm): the
+  // launch's T mod ncu leftover 64-voxel tiles run as tail_k pieces each
+  int ncu;                 // compute units of the device
+  int tail_tiles, tail_k;  // leftover tiles of a full-range launch; pieces per tile (<= 1: no split)
+  float *mpart;            // [chain][tail_tiles][tail_k][64][P] partial X'Z of the pieces
+  int zfree;               // Z not kept, X'Z fused into the latent kernel (z_store 2)
 };
 
 // Posterior accumulators (device, per chain): see k_accum.
@@ -106,7 +112,9 @@ bool latent_carries_wishart(const DevProblem &pb);
 // partials; parts == 0: they are formed here from the stored Z first.
 hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st, int v0, int v1);
-hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st);
+// tail = 1: the last latent launch of this problem was split (latent_split_active)
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail = 0);
+bool latent_split_active(const DevProblem &pb, int *nmain, int *K);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
@@ -116,6 +124,7 @@ hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st);
 // Z-free latent form: lanes per voxel chosen from N, its tile count and LDS bytes
 int latent_vl_groups(const DevProblem &pb);
 int latent_vl_tiles(const DevProblem &pb, int G);
+int latent_vl_blocks_g4(const DevProblem &pb);  // blocks of a full-range G = 4 inversion launch (tail split)
 size_t latent_vl_lds_bytes(const DevProblem &pb);
 // batched analyses: one launch per phase and covariate count P over the work
 // items; latent items of members at G = 4 (inversion) take their own launch (g4)

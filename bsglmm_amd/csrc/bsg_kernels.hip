@@ -690,7 +690,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int P, bool WMU>
 __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_t sweep, int chain, int t0,
-                                                   int tstep, int slot)
+                                                   int tstep, int slot, int ra = 0, int rb = -1,
+                                                   float *part = nullptr)
 {
   static_assert(P <= 8, "means take at most two MFMAs");
   constexpr bool MFM = P >= 3;
@@ -742,8 +743,9 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
     float acc[P], zs = 0.0f;
 #pragma unroll
     for (int c = 0; c < P; c++) acc[c] = 0.0f;
-    int am = 0;  // step of the means' A offsets
-    for (int r = 0; r < R; r++, am += 4 * P) {
+    const int rend = rb < 0 ? R : rb;
+    int am = ra * 4 * P;  // step of the means' A offsets
+    for (int r = ra; r < rend; r++, am += 4 * P) {
       const int q = q0 + r;  // passes NQ only in the last group, where X reads zero
       float4 x[P];  // read before the draw only for the FMA means: not live through it otherwise
       if (!MFM) {
@@ -796,7 +798,12 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
       for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
       if (WMU) zs += __shfl_xor(zs, off, 64);
     }
-    if (valid && g == 0) {
+    if (part) {  // a piece of a split tile: partial sums, tile-local voxel order
+      if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < P; c++) part[((threadIdx.x >> 6) * 16 + vl) * P + c] = acc[c];
+      }
+    } else if (valid && g == 0) {
       float *m = pb.M + ((size_t)chain * pb.V + s) * P;
 #pragma unroll
       for (int c = 0; c < P; c++) m[c] = acc[c];
@@ -824,24 +831,109 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, int G, uint32_
 template <int P>
 __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part, uint32_t sweep);
 
+// Tail split of the inversion path at G = 4. A launch of T 64-voxel tiles over
+// the chip's C compute units runs ceil(T / C) tiles on some units where T / C
+// would be even. When the T mod C leftover tiles are few, each is split into
+// K pieces by subject rounds (piece k takes rounds [kR/K, (k+1)R/K)), which
+// spread over the whole chip. Block b < nmain runs tile b; block nmain + tK + k
+// runs piece k of leftover tile nmain + t and writes the tile's 64 partial
+// X'Z sums to mpart. The coefficient kernel that next reads such a voxel adds
+// its K partials in piece order (vl_tail_sum) and writes m_j back: the kernel
+// boundary orders the writes before the reads, so no fence or counter is
+// needed (a device-scope fence here writes back and invalidates the L2 of the
+// XCD, which slowed the whole launch). The pieces draw exactly the variates of
+// the whole tile (same counters); only the order of the X'Z sum differs. The
+// split depends on the problem alone, so a batch runs the same blocks as the
+// analysis alone.
+__host__ __device__ inline void vl_split(const DevProblem &pb, int &nmain, int &K)
+{
+  const int tiles = (pb.V + 63) / 64;
+  if (!pb.wm_update && pb.tail_k > 1 && tiles <= BSTAT_SLOTS && tiles % pb.ncu == pb.tail_tiles) {
+    nmain = tiles - pb.tail_tiles;
+    K = pb.tail_k;
+  } else {
+    nmain = tiles;
+    K = 1;
+  }
+}
+
+// Subject groups per voxel for the Z-free latent kernels: enough lanes per
+// voxel to give the chip many waves, few enough that every lane keeps several
+// quads. Past 16 quads the inversion path keeps G = 4, where its means run on
+// the matrix cores.
+__host__ __device__ inline int vl_groups(const DevProblem &pb)
+{
+  if (pb.lat_g > 0) return pb.lat_g;
+  if (pb.NQ <= 4) return 1;
+  if (pb.NQ <= 16) return 2;
+  if (pb.NQ <= 128 || pb.latent_method == 0) return 4;
+  return 8;
+}
+
+// True when a full-range Z-free latent launch of this problem runs split
+// (inversion draws at G = 4 and a leftover to spread): its leftover tiles'
+// X'Z is then K partial sums in mpart until a coefficient kernel adds them.
+__host__ __device__ inline bool vl_split_active(const DevProblem &pb, int &nmain, int &K)
+{
+  vl_split(pb, nmain, K);
+  if (!pb.zfree || pb.latent_method != 0 || vl_groups(pb) != 4) K = 1;
+  return K > 1;
+}
+
+// Block b of a full-range G = 4 inversion launch of sweep `sweep` (its Philox
+// word), split as vl_split says; tstep strides an unsplit launch whose blocks
+// hold several tiles.
+template <int P, bool WMU>
+__device__ __forceinline__ void vl_mfm_block(const DevProblem &pb, uint32_t sweep, int chain, int b, int tstep,
+                                             int nmain, int K)
+{
+  if (WMU || K <= 1) {
+    latent_vl_mfm_body<P, WMU>(pb, sweep, chain, b, tstep, b);
+    return;
+  }
+  if (b < nmain) {
+    latent_vl_mfm_body<P, false>(pb, sweep, chain, b, 1 << 30, 0);
+    return;
+  }
+  const int pidx = b - nmain, tl = pidx / K, k = pidx - tl * K;
+  const int R = (pb.NQ + 3) >> 2;
+  latent_vl_mfm_body<P, false>(pb, sweep, chain, nmain + tl, 1 << 30, 0, k * R / K, (k + 1) * R / K,
+                               pb.mpart + (((size_t)chain * pb.tail_tiles + tl) * K + k) * 64 * P);
+}
+
+// m_j of sorted voxel s of a leftover tile of a split launch: its K partial
+// sums added in piece order (the same order wherever it is formed)
+template <int P>
+__device__ __forceinline__ void vl_tail_sum(const DevProblem &pb, int chain, int s, int nmain, int K, float (&m)[P])
+{
+  const int tl = (s >> 6) - nmain, v = s & 63;
+  const float *tp = pb.mpart + ((size_t)chain * pb.tail_tiles + tl) * K * 64 * P + (size_t)v * P;
+#pragma unroll
+  for (int c = 0; c < P; c++) m[c] = 0.0f;
+  for (int kk = 0; kk < K; kk++) {
+#pragma unroll
+    for (int c = 0; c < P; c++) m[c] += tp[(size_t)kk * 64 * P + c];
+  }
+}
+
 // the inversion path at G = 4; a kernel of its own so its registers are its
 // own. With wish = 1, block 0 draws the CAR precision of sweep wsweep (the
 // sweep before: its colour-1 partials are complete, and nothing here reads
 // Lambda) beside the latent blocks, instead of a launch of its own.
 template <int P, bool WMU>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_vl_mfm(DevProblem pb, uint32_t sweep, int wish,
-                                                                       uint32_t wsweep)
+                                                                       uint32_t wsweep, int nmain, int K)
 {
   const uint32_t base = *pb.sweep_base;
+  const int chain = blockIdx.y;
   // P <= 6: the draw's registers spill only in its own code under the bound
   if (P <= 6 && wish && blockIdx.x == 0) {
     constexpr int NE = P * (P + 1) / 2;
-    wishart_block<P>(pb, blockIdx.y, (pb.V - pb.n0 + 255) / 256,
-                     pb.partials + (size_t)blockIdx.y * pb.max_parts * NE, wsweep + base);
+    wishart_block<P>(pb, chain, (pb.V - pb.n0 + 255) / 256, pb.partials + (size_t)chain * pb.max_parts * NE,
+                     wsweep + base);
     return;
   }
-  const int b = blockIdx.x - wish;
-  latent_vl_mfm_body<P, WMU>(pb, sweep + base, blockIdx.y, b, gridDim.x - wish, b);
+  vl_mfm_block<P, WMU>(pb, sweep + base, chain, blockIdx.x - wish, gridDim.x - wish, nmain, K);
 }
 
 // ---------------------------------------------------------------------------
@@ -1242,7 +1334,8 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
 
 // Block `blk` of colour `colour` (256 voxels of its list) for chain `chain`.
 template <int P, bool PREC>
-__device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint32_t sweep, int chain, int blk)
+__device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint32_t sweep, int chain, int blk,
+                                          int tail)
 {
   constexpr int NE = P * (P + 1) / 2;
   const int q = blk * 256 + threadIdx.x;
@@ -1273,12 +1366,24 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
       for (int c = 0; c < P; c++) sv[c] += src[c];
     }
   }
-  // m_j = X'(Z_j - b WM_j) = X'Z_j - b WM_j X'1, with the b of this sweep
-  const float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
+  // m_j = X'(Z_j - b WM_j) = X'Z_j - b WM_j X'1, with the b of this sweep;
+  // a voxel of a split launch's leftover tiles (tail) has its X'Z in pieces
+  float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
   const float bw = pb.bwm[chain] * pb.wm[s];
   float m[P];
+  int nmain, K;
+  if (tail && vl_split_active(pb, nmain, K) && s >= nmain * 64) {
+    vl_tail_sum<P>(pb, chain, s, nmain, K, m);
+    if (active) {
 #pragma unroll
-  for (int i = 0; i < P; i++) m[i] = fmaf(-bw, pb.xsum[i], mz[i]);
+      for (int i = 0; i < P; i++) mz[i] = m[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < P; i++) m[i] = mz[i];
+  }
+#pragma unroll
+  for (int i = 0; i < P; i++) m[i] = fmaf(-bw, pb.xsum[i], m[i]);
   float Q[P][P], r[P];
 #pragma unroll
   for (int i = 0; i < P; i++) {
@@ -1361,10 +1466,10 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
 }
 
 template <int P, bool PREC>
-__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep)
+__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep, int tail)
 {
   sweep += *pb.sweep_base;
-  coef_body<P, PREC>(pb, colour, sweep, blockIdx.y, blockIdx.x);
+  coef_body<P, PREC>(pb, colour, sweep, blockIdx.y, blockIdx.x, tail);
 }
 
 // ---------------------------------------------------------------------------
@@ -1553,13 +1658,17 @@ __global__ __launch_bounds__(256) void k_latent_multi(const DevProblem *pbs, con
   latent_vl_body<P, false, BMF>(pb, pb.lat_g, sweep + *pb.sweep_base, it.chain, it.blk, 1 << 30, 0);
 }
 
-// members drawn by inversion at G = 4
+// members drawn by inversion at G = 4: item blk is block blk of the member's
+// own launch (tail split included)
 template <int P>
-__global__ __launch_bounds__(256) void k_latent_multi_mfm(const DevProblem *pbs, const WorkItem *items, uint32_t sweep)
+__global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_multi_mfm(const DevProblem *pbs,
+                                                                          const WorkItem *items, uint32_t sweep)
 {
   const WorkItem it = items[blockIdx.x];
   const DevProblem pb = pbs[it.a];
-  latent_vl_mfm_body<P, false>(pb, sweep + *pb.sweep_base, it.chain, it.blk, 1 << 30, 0);
+  int nmain, K;
+  vl_split(pb, nmain, K);
+  vl_mfm_block<P, false>(pb, sweep + *pb.sweep_base, it.chain, it.blk, 1 << 30, nmain, K);
 }
 
 template <int P, bool PREC>
@@ -1568,7 +1677,7 @@ __global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const
 {
   const WorkItem it = items[blockIdx.x];
   const DevProblem pb = pbs[it.a];
-  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk);
+  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk, 1);  // batch latent always precedes
 }
 
 template <int P>
@@ -1618,16 +1727,21 @@ static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
     default: return hipErrorInvalidValue; \
   }
 
-// Subject groups per voxel for the Z-free latent kernels: enough lanes per
-// voxel to give the chip many waves, few enough that every lane keeps several
-// quads. Past 16 quads the inversion path keeps G = 4, where its means run on
-// the matrix cores.
+// the lane split chosen from N (vl_groups without a set split)
 int latent_vl_groups(const DevProblem &pb)
 {
-  if (pb.NQ <= 4) return 1;
-  if (pb.NQ <= 16) return 2;
-  if (pb.NQ <= 128 || pb.latent_method == 0) return 4;
-  return 8;
+  DevProblem q = pb;
+  q.lat_g = 0;
+  return vl_groups(q);
+}
+
+bool latent_split_active(const DevProblem &pb, int *nmain, int *K)
+{
+  int nm, k;
+  const bool on = vl_split_active(pb, nm, k);
+  if (nmain) *nmain = nm;
+  if (K) *K = k;
+  return on;
 }
 
 // X staging of either body (the inversion path at G = 4 pads X to 4R quads,
@@ -1645,10 +1759,16 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
   const int vpb = 4 * (64 / G);  // voxels per block tile
   const long long tiles = ((long long)(pb.ve - pb.vb) + vpb - 1) / vpb;
   const long long per = (tiles + BSTAT_SLOTS - 1) / BSTAT_SLOTS;  // tiles per block, equal shares
-  const int nlb = (int)((tiles + per - 1) / per);
+  int nlb = (int)((tiles + per - 1) / per);
   const bool wmu = pb.wm_update != 0, bmf = pb.latent_method == 1;
   if (G != 1 && G != 2 && G != 4 && G != 8) return hipErrorInvalidValue;
   if (wish && (G != 4 || bmf || pb.P > 6)) return hipErrorInvalidValue;
+  // tail split of the inversion path at G = 4 over the whole range
+  int nmain = nlb, K = 1;
+  if (G == vl_groups(pb) && pb.vb == 0 && pb.ve == pb.V && vl_split_active(pb, nmain, K))
+    nlb = nmain + pb.tail_tiles * K;
+  else
+    nmain = nlb, K = 1;
   dim3 grid((unsigned)(nlb + wish), (unsigned)pb.nchains);
 #define V_(PP)                                                                                             \
   do {                                                                                                     \
@@ -1656,8 +1776,8 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, true>), grid, dim3(256), lds, st, pb, G, sweep);   \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, true>), grid, dim3(256), lds, st, pb, G, sweep);      \
     } else if (G == 4) {                                                                                   \
-      if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep); \
-      else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep);    \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, nmain, K); \
+      else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, nmain, K);    \
     } else {                                                                                               \
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, false>), grid, dim3(256), lds, st, pb, G, sweep);  \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, false>), grid, dim3(256), lds, st, pb, G, sweep);     \
@@ -1787,7 +1907,7 @@ hipError_t launch_xtz(const DevProblem &pb0, int mode, hipStream_t st, int v0, i
   return hipGetLastError();
 }
 
-hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st)
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail)
 {
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   if (nc == 0) return with_prec ? hipErrorInvalidValue : hipSuccess;
@@ -1795,9 +1915,9 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 #define L_(PP)                                                                                \
   do {                                                                                        \
     if (with_prec)                                                                            \
-      hipLaunchKernelGGL((k_coef<PP, true>), grid, dim3(256), 0, st, pb, colour, sweep);            \
+      hipLaunchKernelGGL((k_coef<PP, true>), grid, dim3(256), 0, st, pb, colour, sweep, tail);      \
     else                                                                                      \
-      hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour, sweep);           \
+      hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour, sweep, tail);     \
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
@@ -1917,6 +2037,13 @@ hipError_t launch_accum_multi(int P, const DevProblem *pbs, const Accum *accs, c
 }
 
 int latent_vl_tiles(const DevProblem &pb, int G) { return (pb.V + 4 * (64 / G) - 1) / (4 * (64 / G)); }
+
+int latent_vl_blocks_g4(const DevProblem &pb)
+{
+  int nmain, K;
+  vl_split(pb, nmain, K);
+  return K > 1 ? nmain + pb.tail_tiles * K : nmain;
+}
 
 size_t latent_vl_lds_bytes(const DevProblem &pb) { return latent_vl_lds(pb); }
 

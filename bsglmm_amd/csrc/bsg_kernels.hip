@@ -53,9 +53,11 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, 
 
 // odd multiples of 2^-24: exact in float and double, never 0 or 1.
 // (1 + k 2^-23) - (1 - 2^-24) = (2k + 1) 2^-24 exactly (Sterbenz), k = x >> 9.
+// v_alignbit_b32(0x7F, x, 9) = (0x7F << 23) | (x >> 9) = 0x3F800000 | (x >> 9)
+// in one instruction.
 __device__ __forceinline__ float u01f(uint32_t x)
 {
-  return __uint_as_float(0x3F800000u | (x >> 9)) - 0.99999994039535522f;
+  return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, x, 9)) - 0.99999994039535522f;
 }
 __device__ __forceinline__ double u01d(uint32_t x)
 {

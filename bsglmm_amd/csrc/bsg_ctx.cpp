@@ -520,6 +520,7 @@ int bsg_update_coef(bsg_ctx *ctx, int colour)
     return BSG_ERR_ARG;
   }
   HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->sweep, ctx->stream, ctx->tail_pending ? 1 : 0));
+  if (colour == 0) ctx->tail_pending = false;  // colour 0 wrote every leftover voxel's m_j
   return BSG_OK;
 }
 
@@ -575,7 +576,7 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
   const int tail = split_tail(ctx, pb);
   HIPCHK(bsg::launch_coef(pb, 0, false, off, sa, tail));
   if (ctx->L.n[1] > 0) {
-    HIPCHK(bsg::launch_coef(pb, 1, true, off, sa, tail));  // colour 1 + pair partials
+    HIPCHK(bsg::launch_coef(pb, 1, true, off, sa, 0));  // colour 1 + pair partials
     if (fused && ctx->opts.z_store == 2 && bsg::latent_carries_wishart(pb)) {
       w_inline = true;  // drawn by the next latent launch, or at the end of the call
     } else {
@@ -861,7 +862,7 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
     HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st, tail));
     HIPCHK(hipEventRecord(ctx->ev[3], st));
     const bool one_pass = ctx->L.n[1] > 0;
-    if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, st, tail));
+    if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, st, 0));
     HIPCHK(hipEventRecord(ctx->ev[4], st));
     if (!one_pass) HIPCHK(bsg::launch_pairs(ctx->pb, st));
     HIPCHK(hipEventRecord(ctx->ev[5], st));
@@ -1272,10 +1273,12 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     pbs.push_back(pb);
     b->lat_lds[pb.P] = std::max(b->lat_lds[pb.P], bsg::latent_vl_lds_bytes(pb));
     const int lat_list = (pb.lat_g == 4 && c->opts.latent_method == 0) ? 5 : 0;
+    // colour-0 items past its blocks write colour 1's m_j of a split latent launch
+    const int combine = bsg::latent_split_active(pb, nullptr, nullptr) ? pb.tail_tiles : 0;
     const int tiles = lat_list == 5 ? bsg::latent_vl_blocks_g4(pb) : bsg::latent_vl_tiles(pb, pb.lat_g);
     for (int ch = 0; ch < c->C; ch++) {
       for (int t = 0; t < tiles; t++) lists[lat_list].push_back({a, ch, t});
-      for (int k = 0; k < (c->L.n[0] + 255) / 256; k++) lists[1].push_back({a, ch, k});
+      for (int k = 0; k < (c->L.n[0] + 255) / 256 + combine; k++) lists[1].push_back({a, ch, k});
       for (int k = 0; k < (c->L.n[1] + 255) / 256; k++) lists[2].push_back({a, ch, k});
       lists[3].push_back({a, ch, 0});
       for (int k = 0; k < (c->L.totvox + 255) / 256; k++) lists[4].push_back({a, ch, k});

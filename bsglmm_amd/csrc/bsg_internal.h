@@ -112,7 +112,9 @@ bool latent_carries_wishart(const DevProblem &pb);
 // partials; parts == 0: they are formed here from the stored Z first.
 hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st, int v0, int v1);
-// tail = 1: the last latent launch of this problem was split (latent_split_active)
+// tail = 1: the last latent launch of this problem was split (latent_split_active):
+// the update adds its own voxels' pieces, and colour 0 also writes colour 1's
+// m_j from their pieces in extra blocks, so colour 1 then runs with tail = 0
 hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail = 0);
 bool latent_split_active(const DevProblem &pb, int *nmain, int *K);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);

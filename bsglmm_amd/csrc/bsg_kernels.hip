@@ -1467,10 +1467,34 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   block_sum_store<NE>(acc, parts + (size_t)blk * NE);
 }
 
+// Colour 1's voxels in the leftover tiles of a split latent launch: block t of
+// the colour-0 launch's extra blocks adds their pieces (vl_tail_sum) and
+// writes m_j, beside the colour-0 update, so the colour-1 update reads m_j
+// like any other voxel's.
+template <int P>
+__device__ __forceinline__ void tail_combine_block(const DevProblem &pb, int chain, int t)
+{
+  int nmain, K;
+  if (!vl_split_active(pb, nmain, K) || threadIdx.x >= 64) return;
+  const int s = (nmain + t) * 64 + threadIdx.x;
+  if (s < pb.n0 || s >= pb.V) return;
+  float m[P];
+  vl_tail_sum<P>(pb, chain, s, nmain, K, m);
+  float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
+#pragma unroll
+  for (int c = 0; c < P; c++) mz[c] = m[c];
+}
+
+// blocks past the colour's own (colour 0 with tail) run tail_combine_block
 template <int P, bool PREC>
 __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep, int tail)
 {
   sweep += *pb.sweep_base;
+  const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
+  if ((int)blockIdx.x >= ncb) {
+    tail_combine_block<P>(pb, blockIdx.y, blockIdx.x - ncb);
+    return;
+  }
   coef_body<P, PREC>(pb, colour, sweep, blockIdx.y, blockIdx.x, tail);
 }
 
@@ -1679,7 +1703,14 @@ __global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const
 {
   const WorkItem it = items[blockIdx.x];
   const DevProblem pb = pbs[it.a];
-  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk, 1);  // batch latent always precedes
+  // the batch's latent launch always precedes: colour 0 adds its own leftover
+  // pieces and, in the items past its blocks, colour 1's (tail_combine_block)
+  const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
+  if (it.blk >= ncb) {
+    tail_combine_block<P>(pb, it.chain, it.blk - ncb);
+    return;
+  }
+  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk, colour == 0 ? 1 : 0);
 }
 
 template <int P>
@@ -1913,7 +1944,9 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 {
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   if (nc == 0) return with_prec ? hipErrorInvalidValue : hipSuccess;
-  dim3 grid((unsigned)((nc + 255) / 256), (unsigned)pb.nchains);
+  int nmain, K;
+  const int extra = (colour == 0 && tail && vl_split_active(pb, nmain, K)) ? pb.tail_tiles : 0;
+  dim3 grid((unsigned)((nc + 255) / 256 + extra), (unsigned)pb.nchains);
 #define L_(PP)                                                                                \
   do {                                                                                        \
     if (with_prec)                                                                            \

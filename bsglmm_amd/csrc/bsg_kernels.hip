@@ -1359,13 +1359,21 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
     nb[2 * d] = v.x;
     nb[2 * d + 1] = v.y;
   }
+  float bk[PREC ? 6 : 1][P];  // colour 1 keeps the neighbours for its pair terms
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     if (nb[d] >= 0) {
       nn++;
       const float *src = betc + (size_t)nb[d] * P;
 #pragma unroll
-      for (int c = 0; c < P; c++) sv[c] += src[c];
+      for (int c = 0; c < P; c++) {
+        const float v = src[c];
+        sv[c] += v;
+        if (PREC) bk[PREC ? d : 0][c] = v;
+      }
+    } else if (PREC) {
+#pragma unroll
+      for (int c = 0; c < P; c++) bk[PREC ? d : 0][c] = 0.0f;
     }
   }
   // m_j = X'(Z_j - b WM_j) = X'Z_j - b WM_j X'1, with the b of this sweep;
@@ -1451,17 +1459,9 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
 #pragma unroll
   for (int e = 0; e < NE; e++) acc[e] = 0.0;
   if (active) {
-    // the neighbours are of the other colour, unchanged by this kernel: read
-    // again here rather than held in registers through the draw
 #pragma unroll
     for (int d = 0; d < 6; d++)
-      if (nb[d] >= 0) {
-        float bk[P];
-        const float *src = betc + (size_t)nb[d] * P;
-#pragma unroll
-        for (int c = 0; c < P; c++) bk[c] = src[c];
-        add_pair<P>(out, bk, (d & 1) != 0, acc);
-      }
+      if (nb[d] >= 0) add_pair<P>(out, bk[PREC ? d : 0], (d & 1) != 0, acc);
   }
   double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
   block_sum_store<NE>(acc, parts + (size_t)blk * NE);

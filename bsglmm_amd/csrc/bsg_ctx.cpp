@@ -348,7 +348,9 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
 
   // tail split of the Z-free latent launch at four lanes per voxel: the
   // leftover 64-voxel tiles of a full-range launch (T mod CUs) run as K
-  // pieces each when that spreads them over the chip (k_latent_vl_mfm)
+  // pieces each when that spreads them over the chip (k_latent_vl_mfm). One
+  // chain only: with several, chains are the grid's second dimension, so one
+  // chain's pieces would be dispatched mid-launch, where they balance nothing
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -358,7 +360,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     }
     pb.ncu = ncu;
     const int tiles = (V + 63) / 64, tail = tiles % ncu, R = (N + 15) / 16;
-    const int K = tail > 0 ? std::min(std::min(ncu / tail, R), 16) : 0;
+    const int K = (tail > 0 && C == 1) ? std::min(std::min(ncu / tail, R), 16) : 0;
     pb.tail_tiles = K > 1 ? tail : 0;
     pb.tail_k = K > 1 ? K : 1;
     if (K > 1) ALLOC(pb.mpart, (size_t)C * tail * K * 64 * P);

@@ -168,50 +168,43 @@ def test_z_free_every_lane_split(nsubs, ncovar, g):
 
 
 def test_split_tail_single_steps_and_full_size():
-    """Four lanes per voxel split a launch's leftover 64-voxel tiles into
-    pieces whose partial X'Z sums the next coefficient kernel (or get_xtz)
-    adds. Small mask: fewer tiles than compute units, so every tile is split;
-    the X'Z read after the latent step alone, X'Z set by the host, and both
-    colours' updates against the oracle. Full C2 size: whole tiles plus split
+    """Four lanes per voxel split a one-chain launch's leftover 64-voxel tiles
+    into pieces whose partial X'Z sums the next coefficient kernel (or
+    get_xtz) adds. Small mask: fewer tiles than compute units, so every tile
+    is split; the X'Z read after the latent step alone against the oracle,
+    then X'Z set by the host (which rewrites the pieces) and both colours'
+    updates against the oracle on it. Full C2 size: whole tiles plus split
     leftovers, against the unsplit eight-lanes-per-voxel form."""
     print("case split tail: c1 N=150 P=4 steps; c2 full size G=4 vs G=8", flush=True)
     d = synth.generate("c1", nsubs=150, ncovar=4)
     nx, ny, nz = d["grid"]
     lay = oracle.layout(d["mask"], nx, ny, nz)
     L = Layout(d["mask"], nx, ny, nz)
-    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=2, xtz_mode=2, z_store=2)
+    S = Sampler(L, d["covar"], d["data"], seed=SEED, xtz_mode=2, z_store=2)
     assert S.lane_split == 4
     O = oracle.Oracle(lay, d["covar"], d["data"], seed=SEED)
     rng = np.random.default_rng(12)
     prec0 = (np.eye(4) * 1.5 + 0.2).astype(np.float32)
-    scs = [spat_coef_from_ranks(d["beta_true"] + 0.1 * rng.standard_normal(d["beta_true"].shape)
-                                .astype(np.float32), lay) for _ in range(2)]
+    sc0 = spat_coef_from_ranks(d["beta_true"] + 0.1 * rng.standard_normal(d["beta_true"].shape)
+                               .astype(np.float32), lay)
     sweep = 6
     S.set_sweep_index(sweep)
-    for c in range(2):
-        S.set_spat_coef(scs[c], chain=c)
-        S.set_prec(prec0, chain=c)
+    S.set_spat_coef(sc0)
+    S.set_prec(prec0)
     S.update_latent()
     X = O.covar.astype(np.float64)
-    m_set = []
-    for c in range(2):
-        O.set_chain(c)
-        z_orc = O.update_z(scs[c], sweep)
-        m_gpu = S.get_xtz(c)
-        scale = np.abs(X) @ np.abs(z_orc)
-        assert np.max(np.abs(m_gpu - O.xtz(z_orc.astype(np.float32))) / scale) < 2e-5
-        m_set.append((O.xtz(z_orc) * (1.0 + 1e-3)).astype(np.float32))  # host values differ from the pieces
-    S.set_xtz(m_set[1], chain=1)  # chain 1 from the host, chain 0 still from the pieces
+    z_orc = O.update_z(sc0, sweep)
+    scale = np.abs(X) @ np.abs(z_orc)
+    assert np.max(np.abs(S.get_xtz() - O.xtz(z_orc.astype(np.float32))) / scale) < 2e-5
+    m_set = (O.xtz(z_orc) * (1.0 + 1e-3)).astype(np.float32)  # host values differ from the pieces
+    S.set_xtz(m_set)
+    assert np.array_equal(S.get_xtz(), m_set)
     S.update_coef(0)
     S.update_coef(1)
-    for c in range(2):
-        O.set_chain(c)
-        m_used = S.get_xtz(c).astype(np.float64)
-        if c == 1:
-            assert np.array_equal(S.get_xtz(1), m_set[1])
-        ref = O.update_coef(0, m_used, prec0, scs[c].astype(np.float64), sweep)
-        ref = O.update_coef(1, m_used, prec0, ref, sweep)
-        assert rel_err(S.get_spat_coef(c), ref) < 1e-5
+    assert np.array_equal(S.get_xtz(), m_set)
+    ref = O.update_coef(0, m_set.astype(np.float64), prec0, sc0.astype(np.float64), sweep)
+    ref = O.update_coef(1, m_set.astype(np.float64), prec0, ref, sweep)
+    assert rel_err(S.get_spat_coef(), ref) < 1e-5
 
     d = synth.generate("c2")
     L = Layout(d["mask"], *d["grid"])

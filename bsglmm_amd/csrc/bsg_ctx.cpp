@@ -560,7 +560,7 @@ static int accum_launch(bsg_ctx *ctx, hipStream_t st)
 // One sweep of problem pb, whose Philox sweep word is *pb.sweep_base + off.
 // lat_ev2 (optional) brackets the latent launch.
 static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, hipEvent_t *lat_ev2,
-                         bool &wish_pending, bool &w_inline)
+                         bool &wish_pending, bool &w_inline, const bsg::Accum *ac = nullptr, int batch = 0)
 {
   hipStream_t sa = ctx->stream, sb = ctx->stream2;
   const bool fused = ctx->opts.xtz_mode == 2;
@@ -576,9 +576,10 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
   wish_pending = false;
   const int tail = split_tail(ctx, pb);
-  HIPCHK(bsg::launch_coef(pb, 0, false, off, sa, tail));
+  // a save sweep (ac): each colour update accumulates its voxels' new b
+  HIPCHK(bsg::launch_coef(pb, 0, false, off, sa, tail, ac, batch));
   if (ctx->L.n[1] > 0) {
-    HIPCHK(bsg::launch_coef(pb, 1, true, off, sa, 0));  // colour 1 + pair partials
+    HIPCHK(bsg::launch_coef(pb, 1, true, off, sa, 0, ac, batch));  // colour 1 + pair partials
     if (fused && ctx->opts.z_store == 2 && bsg::latent_carries_wishart(pb)) {
       w_inline = true;  // drawn by the next latent launch, or at the end of the call
     } else {
@@ -599,14 +600,20 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
 {
   hipStream_t sa = ctx->stream;
   bool wish_pending = false, w_inline = false;
+  if (accum_every > 0 && !ctx->acc.sum) {
+    set_error("posterior accumulation: call bsg_accum_reset first");
+    return BSG_ERR_ARG;
+  }
   for (int i = 0; i < nsweeps; i++) {
-    const int rc = enqueue_sweep(ctx, ctx->pb, ctx->sweep, lat_ev ? lat_ev + 2 * i : nullptr, wish_pending, w_inline);
+    // save sweeps accumulate inside the colour updates (mcmc.cpp:584-651 saves
+    // after the sweep; each voxel's final b of the sweep is what it adds)
+    const bool save = accum_every > 0 && (i + 1) % accum_every == 0;
+    const int batch = save ? std::min(ctx->acc_count / ctx->acc_batch_len, ctx->acc.nbatches - 1) : 0;
+    const int rc = enqueue_sweep(ctx, ctx->pb, ctx->sweep, lat_ev ? lat_ev + 2 * i : nullptr, wish_pending, w_inline,
+                                 save ? &ctx->acc : nullptr, batch);
     if (rc != BSG_OK) return rc;
     ctx->sweep++;
-    if (accum_every > 0 && (i + 1) % accum_every == 0) {
-      const int rc2 = accum_launch(ctx, sa);  // reads only the coefficients
-      if (rc2 != BSG_OK) return rc2;
-    }
+    if (save) ctx->acc_count++;
   }
   if (w_inline) HIPCHK(bsg::launch_wishart(ctx->pb, true, ctx->sweep - 1, sa));  // the last sweep's draw
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // join
@@ -1219,9 +1226,10 @@ struct bsg_batch {
   bsg::DevProblem *d_pbs = nullptr;
   bsg::WorkItem *d_items = nullptr;
   bsg::Accum *d_accs = nullptr;
-  // item ranges per phase (latent, coef 0, coef 1, wishart, accum, latent of
-  // the members drawn by inversion at G = 4) and P
-  int off[6][bsg::MAX_P + 1] = {}, cnt[6][bsg::MAX_P + 1] = {};
+  // item ranges per phase (latent, coef 0, coef 1, wishart, latent of the
+  // members drawn by inversion at G = 4) and P; save sweeps accumulate in
+  // the coefficient phases
+  int off[5][bsg::MAX_P + 1] = {}, cnt[5][bsg::MAX_P + 1] = {};
   size_t lat_lds[bsg::MAX_P + 1] = {};  // per P: the largest X staging of its members
   bool bmf = false;
   std::vector<bsg::Accum> h_accs;
@@ -1253,7 +1261,7 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     return BSG_ERR_NOMEM;
   }
   std::vector<bsg::DevProblem> pbs;
-  std::vector<bsg::WorkItem> lists[6];
+  std::vector<bsg::WorkItem> lists[5];
   for (int a = 0; a < n; a++) {
     bsg_ctx *c = members[a];
     if (!c) {
@@ -1274,22 +1282,21 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     if (pb.lat_g <= 0) pb.lat_g = bsg::latent_vl_groups(pb);
     pbs.push_back(pb);
     b->lat_lds[pb.P] = std::max(b->lat_lds[pb.P], bsg::latent_vl_lds_bytes(pb));
-    const int lat_list = (pb.lat_g == 4 && c->opts.latent_method == 0) ? 5 : 0;
+    const int lat_list = (pb.lat_g == 4 && c->opts.latent_method == 0) ? 4 : 0;
     // colour-0 items past its blocks write colour 1's m_j of a split latent launch
     const int combine = bsg::latent_split_active(pb, nullptr, nullptr) ? pb.tail_tiles : 0;
-    const int tiles = lat_list == 5 ? bsg::latent_vl_blocks_g4(pb) : bsg::latent_vl_tiles(pb, pb.lat_g);
+    const int tiles = lat_list == 4 ? bsg::latent_vl_blocks_g4(pb) : bsg::latent_vl_tiles(pb, pb.lat_g);
     for (int ch = 0; ch < c->C; ch++) {
       for (int t = 0; t < tiles; t++) lists[lat_list].push_back({a, ch, t});
       for (int k = 0; k < (c->L.n[0] + 255) / 256 + combine; k++) lists[1].push_back({a, ch, k});
       for (int k = 0; k < (c->L.n[1] + 255) / 256; k++) lists[2].push_back({a, ch, k});
       lists[3].push_back({a, ch, 0});
-      for (int k = 0; k < (c->L.totvox + 255) / 256; k++) lists[4].push_back({a, ch, k});
     }
     b->m.push_back(c);
   }
   b->bmf = members[0]->opts.latent_method == 1;
   std::vector<bsg::WorkItem> all;
-  for (int k = 0; k < 6; k++)
+  for (int k = 0; k < 5; k++)
     for (int P = 1; P <= bsg::MAX_P; P++) {
       b->off[k][P] = (int)all.size();
       for (const bsg::WorkItem &w : lists[k])
@@ -1331,13 +1338,23 @@ static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
     for (int P = 1; P <= bsg::MAX_P; P++) {
       HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds[P], b->bmf, false, sw,
                                       sa));
-      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[5][P], b->cnt[5][P], b->lat_lds[P], false, true, sw, sa));
+      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[4][P], b->cnt[4][P], b->lat_lds[P], false, true, sw, sa));
     }
     if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, b->e_c1, 0));
+    // a save sweep: the colour updates accumulate each member's voxels
+    const bool save = accum_every > 0 && (i + 1) % accum_every == 0;
+    if (save) {
+      for (size_t a = 0; a < b->m.size(); a++) {
+        bsg_ctx *c = b->m[a];
+        bidx.b[a] = std::min(c->acc_count / c->acc_batch_len, c->acc.nbatches - 1);
+        c->acc_count++;
+      }
+    }
+    const bsg::Accum *accs = save ? b->d_accs : nullptr;
     for (int P = 1; P <= bsg::MAX_P; P++)
-      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, sa));
+      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, sa, accs, bidx));
     for (int P = 1; P <= bsg::MAX_P; P++)
-      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, sa));
+      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, sa, accs, bidx));
     HIPCHK(hipEventRecord(b->e_c0, sa));
     HIPCHK(hipStreamWaitEvent(sb, b->e_c0, 0));
     for (int P = 1; P <= bsg::MAX_P; P++)
@@ -1345,15 +1362,6 @@ static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
     HIPCHK(hipEventRecord(b->e_c1, sb));
     wish_pending = true;
     for (bsg_ctx *c : b->m) c->sweep++;
-    if (accum_every > 0 && (i + 1) % accum_every == 0) {
-      for (size_t a = 0; a < b->m.size(); a++) {
-        bsg_ctx *c = b->m[a];
-        bidx.b[a] = std::min(c->acc_count / c->acc_batch_len, c->acc.nbatches - 1);
-        c->acc_count++;
-      }
-      for (int P = 1; P <= bsg::MAX_P; P++)
-        HIPCHK(bsg::launch_accum_multi(P, b->d_pbs, b->d_accs, bidx, it + b->off[4][P], b->cnt[4][P], sa));
-    }
   }
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, b->e_c1, 0));
   HIPCHK(hipStreamSynchronize(sa));

@@ -115,7 +115,9 @@ hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st, int v0, in
 // tail = 1: the last latent launch of this problem was split (latent_split_active):
 // the update adds its own voxels' pieces, and colour 0 also writes colour 1's
 // m_j from their pieces in extra blocks, so colour 1 then runs with tail = 0
-hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail = 0);
+// ac != null: a save sweep, each updated voxel is accumulated into ac (batch)
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail = 0,
+                       const Accum *ac = nullptr, int batch = 0);
 bool latent_split_active(const DevProblem &pb, int *nmain, int *K);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);
@@ -132,12 +134,11 @@ size_t latent_vl_lds_bytes(const DevProblem &pb);
 // items; latent items of members at G = 4 (inversion) take their own launch (g4)
 hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
                                bool g4, uint32_t sweep, hipStream_t st);
+// accs != null: a save sweep, each member's updated voxels are accumulated (batch bidx)
 hipError_t launch_coef_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, int colour, bool prec,
-                             uint32_t sweep, hipStream_t st);
+                             uint32_t sweep, hipStream_t st, const Accum *accs, const BatchIndex &bidx);
 hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,
                                 hipStream_t st);
-hipError_t launch_accum_multi(int P, const DevProblem *pbs, const Accum *accs, const BatchIndex &bidx,
-                              const WorkItem *items, int n, hipStream_t st);
 // plane means: part[chain][blocks][P] FP64 block partials
 hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st);
 // per-subject log-likelihood: part [chain][tiles][N][T], out [chain][N][T]

@@ -1272,6 +1272,57 @@ __device__ __forceinline__ void add_pair(const float (&bj)[P], const float (&bk)
 }
 
 // ---------------------------------------------------------------------------
+// Posterior accumulation of one saved draw per voxel (k_accum describes the
+// sums). Phi is Abramowitz & Stegun 26.2.17 (|error| < 7.5e-8), as the
+// reference's ProbSDNorm (covar.cpp:327-348).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double phi_as(double x)
+{
+  const double t = 1.0 / (1.0 + 0.2316419 * fabs(x));
+  const double poly = t * (0.319381530 + t * (-0.356563782 + t * (1.781477937 + t * (-1.821255978 + t * 1.330274429))));
+  const double tail = 0.39894228 * exp(-0.5 * x * x) * poly;
+  return x >= 0.0 ? 1.0 - tail : tail;
+}
+
+// One voxel's save (coefficients b of sorted voxel s after this sweep's
+// update): the colour updates call it for their own voxels on save sweeps,
+// k_accum for every voxel.
+template <int P>
+__device__ __forceinline__ void accum_voxel(const DevProblem &pb, const Accum &ac, int batch, int chain, int s,
+                                            const float (&b)[P])
+{
+  const double bw = (double)pb.bwm[chain] * (double)pb.wm[s];
+  const size_t base = (size_t)chain * pb.V + s;
+  const int K = P + ac.ntypes;
+  float *bat = ac.batch + (((size_t)chain * ac.nbatches + batch) * pb.V + s) * K;
+#pragma unroll
+  for (int c = 0; c < P; c++) {
+    const double v = (double)b[c];
+    ac.sum[base * P + c] += v;
+    ac.sq[base * P + c] += v * v;
+    bat[c] += b[c];
+  }
+  for (int t = 0; t < ac.ntypes && t < P; t++) {
+    const double pr = phi_as((double)b[t] + bw);
+    ac.prb[base * ac.ntypes + t] += pr;
+    bat[P + t] += (float)pr;
+  }
+  ac.bwm[base] += bw;
+}
+
+template <int P>
+__device__ __forceinline__ void accum_body(const DevProblem &pb, const Accum &ac, int batch, int chain, int blk)
+{
+  const int s = blk * 256 + threadIdx.x;
+  if (s >= pb.V) return;
+  const float *bp = pb.beta + ((size_t)chain * pb.V + s) * P;
+  float b[P];
+#pragma unroll
+  for (int c = 0; c < P; c++) b[c] = bp[c];
+  accum_voxel<P>(pb, ac, batch, chain, s, b);
+}
+
+// ---------------------------------------------------------------------------
 // Coefficient update of one colour. With Q = X'X + n_j Lambda and
 // r = m + Lambda sumV, the reference draws b = Q^-1 r + chol_lower(Q^-1) z
 // (covarGPU.cu:354-392). Factor Q = U U' with U upper (the Cholesky factor of
@@ -1337,7 +1388,7 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
 // Block `blk` of colour `colour` (256 voxels of its list) for chain `chain`.
 template <int P, bool PREC>
 __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint32_t sweep, int chain, int blk,
-                                          int tail)
+                                          int tail, const Accum &ac, int batch)
 {
   constexpr int NE = P * (P + 1) / 2;
   const int q = blk * 256 + threadIdx.x;
@@ -1452,6 +1503,7 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   if (active) {
 #pragma unroll
     for (int c = 0; c < P; c++) dst[c] = out[c];
+    if (ac.sum) accum_voxel<P>(pb, ac, batch, chain, s, out);  // a save sweep: this voxel's final b
   }
   if (!PREC) return;
 
@@ -1485,9 +1537,11 @@ __device__ __forceinline__ void tail_combine_block(const DevProblem &pb, int cha
   for (int c = 0; c < P; c++) mz[c] = m[c];
 }
 
-// blocks past the colour's own (colour 0 with tail) run tail_combine_block
+// blocks past the colour's own (colour 0 with tail) run tail_combine_block;
+// ac.sum != null: a save sweep, each updated voxel is accumulated (batch)
 template <int P, bool PREC>
-__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep, int tail)
+__global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep, int tail, Accum ac,
+                                              int batch)
 {
   sweep += *pb.sweep_base;
   const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
@@ -1495,7 +1549,7 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
     tail_combine_block<P>(pb, blockIdx.y, blockIdx.x - ncb);
     return;
   }
-  coef_body<P, PREC>(pb, colour, sweep, blockIdx.y, blockIdx.x, tail);
+  coef_body<P, PREC>(pb, colour, sweep, blockIdx.y, blockIdx.x, tail, ac, batch);
 }
 
 // ---------------------------------------------------------------------------
@@ -1541,40 +1595,6 @@ __global__ __launch_bounds__(256) void k_pairs(DevProblem pb)
 // per batch of draws for batch-means Monte-Carlo errors.
 // Phi is Abramowitz & Stegun 26.2.17 (|error| < 7.5e-8), as the reference's
 // ProbSDNorm (covar.cpp:327-348).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double phi_as(double x)
-{
-  const double t = 1.0 / (1.0 + 0.2316419 * fabs(x));
-  const double poly = t * (0.319381530 + t * (-0.356563782 + t * (1.781477937 + t * (-1.821255978 + t * 1.330274429))));
-  const double tail = 0.39894228 * exp(-0.5 * x * x) * poly;
-  return x >= 0.0 ? 1.0 - tail : tail;
-}
-
-template <int P>
-__device__ __forceinline__ void accum_body(const DevProblem &pb, const Accum &ac, int batch, int chain, int blk)
-{
-  const int s = blk * 256 + threadIdx.x;
-  if (s >= pb.V) return;
-  const float *b = pb.beta + ((size_t)chain * pb.V + s) * P;
-  const double bw = (double)pb.bwm[chain] * (double)pb.wm[s];
-  const size_t base = (size_t)chain * pb.V + s;
-  const int K = P + ac.ntypes;
-  float *bat = ac.batch + (((size_t)chain * ac.nbatches + batch) * pb.V + s) * K;
-#pragma unroll
-  for (int c = 0; c < P; c++) {
-    const double v = (double)b[c];
-    ac.sum[base * P + c] += v;
-    ac.sq[base * P + c] += v * v;
-    bat[c] += b[c];
-  }
-  for (int t = 0; t < ac.ntypes && t < P; t++) {
-    const double pr = phi_as((double)b[t] + bw);
-    ac.prb[base * ac.ntypes + t] += pr;
-    bat[P + t] += (float)pr;
-  }
-  ac.bwm[base] += bw;
-}
-
 template <int P>
 __global__ __launch_bounds__(256) void k_accum(DevProblem pb, Accum ac, int batch)
 {
@@ -1699,7 +1719,7 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_multi_mfm(const 
 
 template <int P, bool PREC>
 __global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const WorkItem *items, int colour,
-                                                    uint32_t sweep)
+                                                    uint32_t sweep, const Accum *accs, BatchIndex bidx)
 {
   const WorkItem it = items[blockIdx.x];
   const DevProblem pb = pbs[it.a];
@@ -1710,7 +1730,9 @@ __global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const
     tail_combine_block<P>(pb, it.chain, it.blk - ncb);
     return;
   }
-  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk, colour == 0 ? 1 : 0);
+  const Accum ac = accs ? accs[it.a] : Accum{};
+  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk, colour == 0 ? 1 : 0, ac,
+                     accs ? bidx.b[it.a] : 0);
 }
 
 template <int P>
@@ -1723,14 +1745,6 @@ __global__ __launch_bounds__(256) void k_wishart_multi(const DevProblem *pbs, co
                    sweep + *pb.sweep_base);
 }
 
-template <int P>
-__global__ __launch_bounds__(256) void k_accum_multi(const DevProblem *pbs, const Accum *accs, BatchIndex bidx,
-                                                     const WorkItem *items)
-{
-  const WorkItem it = items[blockIdx.x];
-  const DevProblem pb = pbs[it.a];
-  accum_body<P>(pb, accs[it.a], bidx.b[it.a], it.chain, it.blk);
-}
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1940,8 +1954,10 @@ hipError_t launch_xtz(const DevProblem &pb0, int mode, hipStream_t st, int v0, i
   return hipGetLastError();
 }
 
-hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail)
+hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail,
+                       const Accum *ac, int batch)
 {
+  const Accum acc = ac ? *ac : Accum{};
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   if (nc == 0) return with_prec ? hipErrorInvalidValue : hipSuccess;
   int nmain, K;
@@ -1950,9 +1966,9 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 #define L_(PP)                                                                                \
   do {                                                                                        \
     if (with_prec)                                                                            \
-      hipLaunchKernelGGL((k_coef<PP, true>), grid, dim3(256), 0, st, pb, colour, sweep, tail);      \
+      hipLaunchKernelGGL((k_coef<PP, true>), grid, dim3(256), 0, st, pb, colour, sweep, tail, acc, batch);  \
     else                                                                                      \
-      hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour, sweep, tail);     \
+      hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour, sweep, tail, acc, batch); \
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
@@ -2038,13 +2054,17 @@ hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *ite
 }
 
 hipError_t launch_coef_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, int colour, bool prec,
-                             uint32_t sweep, hipStream_t st)
+                             uint32_t sweep, hipStream_t st, const Accum *accs, const BatchIndex &bidx)
 {
   if (n <= 0) return hipSuccess;
-#define L_(PP)                                                                                                    \
-  do {                                                                                                            \
-    if (prec) hipLaunchKernelGGL((k_coef_multi<PP, true>), dim3((unsigned)n), dim3(256), 0, st, pbs, items, colour, sweep); \
-    else hipLaunchKernelGGL((k_coef_multi<PP, false>), dim3((unsigned)n), dim3(256), 0, st, pbs, items, colour, sweep);    \
+#define L_(PP)                                                                                            \
+  do {                                                                                                    \
+    if (prec)                                                                                             \
+      hipLaunchKernelGGL((k_coef_multi<PP, true>), dim3((unsigned)n), dim3(256), 0, st, pbs, items, colour, sweep, \
+                         accs, bidx);                                                                     \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_coef_multi<PP, false>), dim3((unsigned)n), dim3(256), 0, st, pbs, items, colour, sweep, \
+                         accs, bidx);                                                                     \
   } while (0)
   BSG_DISPATCH_P(P, L_)
 #undef L_
@@ -2061,15 +2081,6 @@ hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *it
   return hipGetLastError();
 }
 
-hipError_t launch_accum_multi(int P, const DevProblem *pbs, const Accum *accs, const BatchIndex &bidx,
-                              const WorkItem *items, int n, hipStream_t st)
-{
-  if (n <= 0) return hipSuccess;
-#define L_(PP) hipLaunchKernelGGL(k_accum_multi<PP>, dim3((unsigned)n), dim3(256), 0, st, pbs, accs, bidx, items)
-  BSG_DISPATCH_P(P, L_)
-#undef L_
-  return hipGetLastError();
-}
 
 int latent_vl_tiles(const DevProblem &pb, int G) { return (pb.V + 4 * (64 / G) - 1) / (4 * (64 / G)); }
 

@@ -160,11 +160,14 @@ def test_z_free_fused_sweep(nsubs, ncovar, nchains, method):
 
 
 # Lane splits of the inversion path: G = 4 forms the means on the matrix
-# cores with one MFMA for P = 3, 4, two for P = 5..8 and none for P <= 2.
-@pytest.mark.parametrize("nsubs,ncovar,g", [(300, 6, 1), (300, 6, 2), (300, 6, 4), (300, 6, 8), (200, 8, 4),
-                                            (90, 2, 4), (57, 5, 4), (13, 3, 4)])
-def test_z_free_every_lane_split(nsubs, ncovar, g):
-    check_z_free(nsubs, ncovar, 2 if g == 4 else 1, 0, g)
+# cores with one MFMA for P = 3, 4, two for P = 5..8 and none for P <= 2; one
+# chain at G = 4 also splits the leftover tiles (carrying the Wishart draw in
+# the latent launch up to P = 6, on the second stream past it).
+@pytest.mark.parametrize("nsubs,ncovar,g,nchains", [(300, 6, 1, 1), (300, 6, 2, 1), (300, 6, 4, 2), (300, 6, 8, 1),
+                                                    (200, 8, 4, 1), (200, 8, 4, 2), (90, 2, 4, 2), (57, 5, 4, 1),
+                                                    (13, 3, 4, 2), (150, 7, 4, 1)])
+def test_z_free_every_lane_split(nsubs, ncovar, g, nchains):
+    check_z_free(nsubs, ncovar, nchains, 0, g)
 
 
 def test_split_tail_single_steps_and_full_size():

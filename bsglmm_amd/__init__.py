@@ -97,6 +97,8 @@ def lib():
         "bsg_autotune": (C.c_int, [vp, C.c_int, pi, pd]),
         "bsg_set_lane_split": (C.c_int, [vp, C.c_int]),
         "bsg_get_lane_split": (C.c_int, [vp, pi]),
+        "bsg_set_overlap": (C.c_int, [vp, C.c_int]),
+        "bsg_get_overlap": (C.c_int, [vp, pi]),
         "bsg_batch_create": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
         "bsg_batch_destroy": (C.c_int, [vp]),
         "bsg_batch_sweep": (C.c_int, [vp, C.c_int]),
@@ -203,7 +205,8 @@ class Sampler:
         return dict(zip(self.KERNELS, ms))
 
     def sweep_timed(self, n):
-        """n sweeps; returns (summed latent-kernel ms, total ms) from HIP events."""
+        """n sweeps; returns (summed latent-kernel ms, total ms) from HIP events
+        (overlapped sweeps: the summed spans of each sweep's two launches)."""
         lat, tot = C.c_double(), C.c_double()
         _check(lib().bsg_sweep_timed(self._h, int(n), C.byref(lat), C.byref(tot)))
         return lat.value, tot.value
@@ -293,6 +296,20 @@ class Sampler:
     @lane_split.setter
     def lane_split(self, g):
         _check(lib().bsg_set_lane_split(self._h, int(g)))
+
+    @property
+    def overlap(self):
+        """True when the next sweep runs overlapped: two launches per sweep,
+        each colour's coefficient update beside the other colour's latent
+        draws (one chain, four lanes per voxel, no WM image, P <= 6). The
+        chain is the same either way. Assign False to run the serial order."""
+        on = C.c_int()
+        _check(lib().bsg_get_overlap(self._h, C.byref(on)))
+        return bool(on.value)
+
+    @overlap.setter
+    def overlap(self, on):
+        _check(lib().bsg_set_overlap(self._h, 1 if on else 0))
 
     def coef_means(self, chain=0):
         """Plane means of the coefficients over the mask (alphaMean)."""

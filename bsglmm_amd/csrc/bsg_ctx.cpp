@@ -31,9 +31,13 @@ struct bsg_ctx {
   uint32_t sweep = 0;  // index of the next sweep (the Philox counter word), same for all chains
   hipStream_t stream = nullptr;   // all work; the caller-visible stream
   hipStream_t stream2 = nullptr;  // CAR-precision draw of sweep t beside the latent draw of t+1
-  // the last Z-free latent launch was split: the X'Z of its leftover tiles is
-  // in partial sums until a coefficient kernel (or bsg_get_xtz) adds them
-  bool tail_pending = false;
+  // colour set (bit c) whose leftover tiles' X'Z the last colour-tiled latent
+  // launch left in partial sums, until a coefficient kernel (or bsg_get_xtz)
+  // adds them
+  int tail_pending = 0;
+  // full sweeps run overlapped (k_overlap) where supported; set_overlap
+  bool overlap = true;
+  bool overlap_ran = false;  // flags[2] (a Lambda wait past its bound) to check
   bsg::DevProblem pb{};
   std::vector<void *> allocs;
   hipEvent_t ev[7] = {};
@@ -51,6 +55,7 @@ struct bsg_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_k = 0;
+  bool graph_overlap = false;  // the form the graph was captured in
 
   void free_accum()
   {
@@ -346,11 +351,11 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(pb.bwm, (size_t)C);
   ALLOC(pb.bstat, (size_t)C * bsg::BSTAT_SLOTS);
 
-  // tail split of the Z-free latent launch at four lanes per voxel: the
-  // leftover 64-voxel tiles of a full-range launch (T mod CUs) run as K
-  // pieces each when that spreads them over the chip (k_latent_vl_mfm). One
-  // chain only: with several, chains are the grid's second dimension, so one
-  // chain's pieces would be dispatched mid-launch, where they balance nothing
+  // colour tiles of the Z-free latent launch at four lanes per voxel: each
+  // colour's leftover 64-voxel tiles (T mod CUs) run as K pieces each when
+  // that spreads them over the chip (k_latent_vl_mfm). One chain only: with
+  // several, chains are the grid's second dimension, so one chain's pieces
+  // would be dispatched mid-launch, where they balance nothing
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -359,12 +364,20 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
       return BSG_ERR_HIP;
     }
     pb.ncu = ncu;
-    const int tiles = (V + 63) / 64, tail = tiles % ncu, R = (N + 15) / 16;
-    const int K = (tail > 0 && C == 1) ? std::min(std::min(ncu / tail, R), 16) : 0;
-    pb.tail_tiles = K > 1 ? tail : 0;
-    pb.tail_k = K > 1 ? K : 1;
-    if (K > 1) ALLOC(pb.mpart, (size_t)C * tail * K * 64 * P);
+    pb.ctile = C == 1 ? 1 : 0;
+    const int R = (N + 15) / 16;
+    int mp = 0;
+    for (int c = 0; c < 2; c++) {
+      const int tiles = (L.n[c] + 63) / 64, tail = tiles % ncu;
+      const int K = (tail > 0 && C == 1) ? std::min(std::min(ncu / tail, R), 16) : 0;
+      pb.tl_left[c] = K > 1 ? tail : 0;
+      pb.tl_k[c] = K > 1 ? K : 1;
+      if (K > 1) mp += tail * K * 64 * P;
+    }
+    pb.mp_stride = mp;
+    if (mp > 0) ALLOC(pb.mpart, (size_t)C * mp);
   }
+  ALLOC(pb.lamtag, (size_t)C * P * P);
 
   ALLOC(pb.flags, (size_t)4);
   ALLOC(ctx->sweep_words, (size_t)2);
@@ -391,6 +404,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemsetAsync(pb.bwm, 0, sizeof(float) * C, st));
 
   HIPCHK(hipMemsetAsync(pb.flags, 0, sizeof(uint32_t) * 4, st));
+  HIPCHK(hipMemsetAsync(pb.lamtag, 0, sizeof(unsigned long long) * (size_t)C * P * P, st));
   HIPCHK(hipMemsetAsync(ctx->sweep_words, 0, sizeof(uint32_t) * 2, st));
 
   std::vector<float> lam((size_t)C * P * P, 0.0f);
@@ -420,19 +434,11 @@ int bsg_destroy(bsg_ctx *ctx)
     return BSG_ERR_ARG;                     \
   }
 
-// 1 when a full-range latent launch of pb in this context's form leaves the
-// X'Z of its leftover tiles in partial sums (tail split of the Z-free kernel)
-static int split_tail(const bsg_ctx *ctx, const bsg::DevProblem &pb)
-{
-  return ctx->opts.z_store == 2 && bsg::latent_split_active(pb, nullptr, nullptr) ? 1 : 0;
-}
-
 int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
   HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream, 0,
-                            ctx->L.totvox, &ctx->bstat_parts));
-  ctx->tail_pending = split_tail(ctx, ctx->pb) != 0;
+                            ctx->L.totvox, &ctx->bstat_parts, 0, 0, &ctx->tail_pending));
   return BSG_OK;
 }
 
@@ -521,8 +527,9 @@ int bsg_update_coef(bsg_ctx *ctx, int colour)
     set_error("bsg_update_coef: colour must be 0 or 1");
     return BSG_ERR_ARG;
   }
-  HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->sweep, ctx->stream, ctx->tail_pending ? 1 : 0));
-  if (colour == 0) ctx->tail_pending = false;  // colour 0 wrote every leftover voxel's m_j
+  HIPCHK(bsg::launch_coef(ctx->pb, colour, false, ctx->sweep, ctx->stream, ctx->tail_pending));
+  // colour 0 writes every leftover voxel's m_j, colour 1 its own
+  ctx->tail_pending = colour == 0 ? 0 : (ctx->tail_pending & 1);
   return BSG_OK;
 }
 
@@ -566,18 +573,19 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
   const bool fused = ctx->opts.xtz_mode == 2;
   const int V = ctx->L.totvox;
   if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[0], sa));
-  int parts = 0;
-  HIPCHK(bsg::launch_latent(pb, fused, ctx->opts.z_store, off, sa, 0, V, &parts, w_inline ? 1 : 0, off - 1));
-  ctx->tail_pending = false;  // the colour kernels of this sweep add the pieces
+  int parts = 0, pieces = 0;
+  HIPCHK(bsg::launch_latent(pb, fused, ctx->opts.z_store, off, sa, 0, V, &parts, w_inline ? 1 : 0, off - 1,
+                            &pieces));
+  ctx->tail_pending = 0;  // the colour kernels of this sweep add the pieces
   w_inline = false;
   if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[1], sa));
   if (!fused) HIPCHK(bsg::launch_xtz(pb, ctx->opts.xtz_mode, sa, 0, V));
   HIPCHK(bsg::launch_bwm(pb, parts, off, sa));  // WM coefficient (mcmc.cpp:344-345)
   if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, ctx->e_c1, 0));  // Lambda of sweep t-1
   wish_pending = false;
-  const int tail = split_tail(ctx, pb);
-  // a save sweep (ac): each colour update accumulates its voxels' new b
-  HIPCHK(bsg::launch_coef(pb, 0, false, off, sa, tail, ac, batch));
+  // a save sweep (ac): each colour update accumulates its voxels' new b;
+  // colour 0 adds its own pieces and writes colour 1's m_j from theirs
+  HIPCHK(bsg::launch_coef(pb, 0, false, off, sa, pieces, ac, batch));
   if (ctx->L.n[1] > 0) {
     HIPCHK(bsg::launch_coef(pb, 1, true, off, sa, 0, ac, batch));  // colour 1 + pair partials
     if (fused && ctx->opts.z_store == 2 && bsg::latent_carries_wishart(pb)) {
@@ -596,6 +604,45 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
   return BSG_OK;
 }
 
+// Full sweeps run overlapped (k_overlap) when the context allows it: one
+// chain, Z-free inversion draws at four lanes per voxel with colour tiles, b
+// fixed, both colours present and the Wishart draw small enough to carry.
+static bool use_overlap(const bsg_ctx *ctx)
+{
+  return ctx->overlap && ctx->opts.z_store == 2 && ctx->opts.xtz_mode == 2 && bsg::overlap_supported(ctx->pb);
+}
+
+// k full sweeps of problem pb, Philox words *pb.sweep_base + off0 + i, in the
+// overlapped form: colour 0's latent draws of the first sweep, then per sweep
+// phase 0 (colour 1 drawn, colour 0 updated, Lambda of the sweep before drawn
+// inside) and phase 1 (colour 1 updated, colour 0 of the next sweep drawn),
+// then the last sweep's Lambda. Same variates and sums as enqueue_sweep.
+// ev (optional, 2 per sweep) brackets each sweep's two launches. Save sweeps
+// (accum_every > 0) accumulate inside the colour updates.
+static int enqueue_overlap(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off0, int k, hipEvent_t *ev,
+                           int accum_every)
+{
+  hipStream_t sa = ctx->stream;
+  if (k <= 0) return BSG_OK;
+  const int tail0 = pb.tl_k[0] > 1 ? 1 : 0, tail1 = pb.tl_k[1] > 1 ? 2 : 0;
+  HIPCHK(bsg::launch_latent(pb, true, 2, off0, sa, 0, ctx->L.n[0]));
+  for (int i = 0; i < k; i++) {
+    const uint32_t off = off0 + (uint32_t)i;
+    const bool save = accum_every > 0 && (i + 1) % accum_every == 0;
+    const int batch = save ? std::min(ctx->acc_count / ctx->acc_batch_len, ctx->acc.nbatches - 1) : 0;
+    const bsg::Accum *ac = save ? &ctx->acc : nullptr;
+    if (ev) HIPCHK(hipEventRecord(ev[2 * i], sa));
+    HIPCHK(bsg::launch_overlap(pb, 0, off, 1, i > 0 ? 1 : 0, off - 1, tail0, sa, ac, batch));
+    HIPCHK(bsg::launch_overlap(pb, 1, off, i + 1 < k ? 1 : 0, 0, 0, tail1, sa, ac, batch));
+    if (ev) HIPCHK(hipEventRecord(ev[2 * i + 1], sa));
+    if (save) ctx->acc_count++;
+  }
+  HIPCHK(bsg::launch_wishart(pb, true, off0 + (uint32_t)k - 1, sa));
+  ctx->tail_pending = 0;
+  ctx->overlap_ran = true;
+  return BSG_OK;
+}
+
 static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_every = 0)
 {
   hipStream_t sa = ctx->stream;
@@ -603,6 +650,11 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
   if (accum_every > 0 && !ctx->acc.sum) {
     set_error("posterior accumulation: call bsg_accum_reset first");
     return BSG_ERR_ARG;
+  }
+  if (use_overlap(ctx)) {
+    const int rc = enqueue_overlap(ctx, ctx->pb, ctx->sweep, nsweeps, lat_ev, accum_every);
+    if (rc == BSG_OK) ctx->sweep += (uint32_t)nsweeps;
+    return rc;
   }
   for (int i = 0; i < nsweeps; i++) {
     // save sweeps accumulate inside the colour updates (mcmc.cpp:584-651 saves
@@ -625,7 +677,7 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
 // advances by k, so one executable graph serves every group of k sweeps.
 static int build_graph(bsg_ctx *ctx, int k)
 {
-  if (ctx->graph_exec && ctx->graph_k == k) return BSG_OK;
+  if (ctx->graph_exec && ctx->graph_k == k && ctx->graph_overlap == use_overlap(ctx)) return BSG_OK;
   if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->graph) hipGraphDestroy(ctx->graph);
   ctx->graph_exec = nullptr;
@@ -636,7 +688,10 @@ static int build_graph(bsg_ctx *ctx, int k)
   HIPCHK(hipStreamBeginCapture(sa, hipStreamCaptureModeThreadLocal));
   bool wish_pending = false, w_inline = false;
   int rc = BSG_OK;
-  for (int i = 0; i < k && rc == BSG_OK; i++) rc = enqueue_sweep(ctx, pg, (uint32_t)i, nullptr, wish_pending, w_inline);
+  ctx->graph_overlap = use_overlap(ctx);
+  if (ctx->graph_overlap) rc = enqueue_overlap(ctx, pg, 0, k, nullptr, 0);
+  for (int i = 0; i < k && rc == BSG_OK && !ctx->graph_overlap; i++)
+    rc = enqueue_sweep(ctx, pg, (uint32_t)i, nullptr, wish_pending, w_inline);
   if (rc == BSG_OK && w_inline && bsg::launch_wishart(pg, true, (uint32_t)(k - 1), sa) != hipSuccess) rc = BSG_ERR_HIP;
   if (rc == BSG_OK && wish_pending && hipStreamWaitEvent(sa, ctx->e_c1, 0) != hipSuccess) rc = BSG_ERR_HIP;
   if (rc == BSG_OK && bsg::launch_tick(ctx->sweep_words + 1, (uint32_t)k, sa) != hipSuccess) rc = BSG_ERR_HIP;
@@ -844,6 +899,11 @@ int bsg_synchronize(bsg_ctx *ctx)
               " voxel(s) had a non positive definite posterior precision");
     return BSG_ERR_NUM;
   }
+  if (flags[2]) {
+    set_error("overlapped sweep: a colour-0 block waited past its bound for the CAR precision drawn in the "
+              "same launch");
+    return BSG_ERR_HIP;
+  }
   return BSG_OK;
 }
 
@@ -860,15 +920,14 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
     const bool fused = ctx->opts.xtz_mode == 2;
     const uint32_t sw = ctx->sweep;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    int parts = 0;
-    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, st, 0, ctx->L.totvox, &parts));
+    int parts = 0, pieces = 0;
+    HIPCHK(bsg::launch_latent(ctx->pb, fused, ctx->opts.z_store, sw, st, 0, ctx->L.totvox, &parts, 0, 0, &pieces));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
     if (!fused) HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, st, 0, ctx->L.totvox));
     HIPCHK(bsg::launch_bwm(ctx->pb, parts, sw, st));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
-    const int tail = split_tail(ctx, ctx->pb);
-    ctx->tail_pending = false;
-    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st, tail));
+    ctx->tail_pending = 0;
+    HIPCHK(bsg::launch_coef(ctx->pb, 0, false, sw, st, pieces));
     HIPCHK(hipEventRecord(ctx->ev[3], st));
     const bool one_pass = ctx->L.n[1] > 0;
     if (one_pass) HIPCHK(bsg::launch_coef(ctx->pb, 1, true, sw, st, 0));
@@ -950,19 +1009,21 @@ int bsg_get_xtz(bsg_ctx *ctx, int chain, float *m)
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipMemcpy(mm.data(), ctx->pb.M + (size_t)chain * V * P, sizeof(float) * mm.size(),
                    hipMemcpyDeviceToHost));
-  int nmain = 0, K = 1;
-  if (ctx->tail_pending && bsg::latent_split_active(ctx->pb, &nmain, &K)) {
-    // leftover tiles of a split launch: add the pieces in piece order, as the
+  for (int col = 0; col < 2; col++) {
+    if (!((ctx->tail_pending >> col) & 1)) continue;
+    // leftover tiles of this colour: add the pieces in piece order, as the
     // coefficient kernels do (the same FP32 sums)
-    const size_t per_tile = (size_t)K * 64 * P;
-    std::vector<float> parts(per_tile * ctx->pb.tail_tiles);
-    HIPCHK(hipMemcpy(parts.data(), ctx->pb.mpart + (size_t)chain * parts.size(), sizeof(float) * parts.size(),
-                     hipMemcpyDeviceToHost));
-    for (int s = nmain * 64; s < V; s++) {
-      const float *tp = parts.data() + (size_t)((s >> 6) - nmain) * per_tile + (size_t)(s & 63) * P;
+    const bsg::VlRange r = bsg::vl_range(ctx->pb, col);
+    const size_t per_tile = (size_t)r.K * 64 * P;
+    std::vector<float> parts(per_tile * r.left);
+    HIPCHK(hipMemcpy(parts.data(), ctx->pb.mpart + (size_t)chain * ctx->pb.mp_stride + r.moff,
+                     sizeof(float) * parts.size(), hipMemcpyDeviceToHost));
+    for (int s = r.base + r.nmain * 64; s < r.end; s++) {
+      const int loc = s - r.base;
+      const float *tp = parts.data() + (size_t)((loc >> 6) - r.nmain) * per_tile + (size_t)(loc & 63) * P;
       for (int c = 0; c < P; c++) {
         float sum = 0.0f;
-        for (int kk = 0; kk < K; kk++) sum += tp[(size_t)kk * 64 * P + c];
+        for (int kk = 0; kk < r.K; kk++) sum += tp[(size_t)kk * 64 * P + c];
         mm[(size_t)s * P + c] = sum;
       }
     }
@@ -982,17 +1043,20 @@ int bsg_set_xtz(bsg_ctx *ctx, int chain, const float *m)
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipMemcpy(ctx->pb.M + (size_t)chain * V * P, mm.data(), sizeof(float) * mm.size(),
                    hipMemcpyHostToDevice));
-  int nmain = 0, K = 1;
-  if (ctx->tail_pending && bsg::latent_split_active(ctx->pb, &nmain, &K)) {
-    // this chain's leftover tiles still hold pieces: piece 0 = the new value,
-    // the others 0, so the piece sum the coefficient kernels form is m exactly
-    const size_t per_tile = (size_t)K * 64 * P;
-    std::vector<float> parts(per_tile * ctx->pb.tail_tiles, 0.0f);
-    for (int s = nmain * 64; s < V; s++)
+  for (int col = 0; col < 2; col++) {
+    if (!((ctx->tail_pending >> col) & 1)) continue;
+    // this colour's leftover tiles still hold pieces: piece 0 = the new
+    // value, the others 0, so the piece sum the coefficient kernels form is m
+    const bsg::VlRange r = bsg::vl_range(ctx->pb, col);
+    const size_t per_tile = (size_t)r.K * 64 * P;
+    std::vector<float> parts(per_tile * r.left, 0.0f);
+    for (int s = r.base + r.nmain * 64; s < r.end; s++) {
+      const int loc = s - r.base;
       for (int c = 0; c < P; c++)
-        parts[(size_t)((s >> 6) - nmain) * per_tile + (size_t)(s & 63) * P + c] = mm[(size_t)s * P + c];
-    HIPCHK(hipMemcpy(ctx->pb.mpart + (size_t)chain * parts.size(), parts.data(), sizeof(float) * parts.size(),
-                     hipMemcpyHostToDevice));
+        parts[(size_t)((loc >> 6) - r.nmain) * per_tile + (size_t)(loc & 63) * P + c] = mm[(size_t)s * P + c];
+    }
+    HIPCHK(hipMemcpy(ctx->pb.mpart + (size_t)chain * ctx->pb.mp_stride + r.moff, parts.data(),
+                     sizeof(float) * parts.size(), hipMemcpyHostToDevice));
   }
   return BSG_OK;
 }
@@ -1164,6 +1228,24 @@ int bsg_get_lane_split(bsg_ctx *ctx, int *g)
   return BSG_OK;
 }
 
+int bsg_set_overlap(bsg_ctx *ctx, int on)
+{
+  CTXCHK(ctx);
+  ctx->overlap = on != 0;
+  return BSG_OK;
+}
+
+int bsg_get_overlap(bsg_ctx *ctx, int *on)
+{
+  CTXCHK(ctx);
+  if (!on) {
+    set_error("bsg_get_overlap: NULL output");
+    return BSG_ERR_ARG;
+  }
+  *on = use_overlap(ctx) ? 1 : 0;
+  return BSG_OK;
+}
+
 // ---- autotuning of the Z-free latent kernel ----------------------------
 int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
 {
@@ -1179,7 +1261,7 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int keep = ctx->pb.lat_g;
-  int rc = BSG_OK;
+  int rc = BSG_OK, pieces = ctx->tail_pending;
   for (int k = 0; k < 4 && rc == BSG_OK; k++) {
     const int G = cand[k];
     double ms = -1.0;
@@ -1192,7 +1274,9 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
         break;
       }
       for (int r = 0; r < reps; r++)
-        if (bsg::launch_latent(ctx->pb, true, 2, ctx->sweep, ctx->stream, 0, ctx->L.totvox) != hipSuccess) rc = BSG_ERR_HIP;
+        if (bsg::launch_latent(ctx->pb, true, 2, ctx->sweep, ctx->stream, 0, ctx->L.totvox, nullptr, 0, 0,
+                               &pieces) != hipSuccess)
+          rc = BSG_ERR_HIP;
       float f = 0.0f;
       if (rc == BSG_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                            hipEventElapsedTime(&f, e0, e1) != hipSuccess))
@@ -1213,7 +1297,7 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
     return rc;
   }
   ctx->pb.lat_g = bg;
-  ctx->tail_pending = split_tail(ctx, ctx->pb) != 0;  // the last timed launch ran at this split
+  ctx->tail_pending = pieces;  // the pieces the last timed launch left
   if (best_g) *best_g = bg;
   return BSG_OK;
 }
@@ -1283,8 +1367,8 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     pbs.push_back(pb);
     b->lat_lds[pb.P] = std::max(b->lat_lds[pb.P], bsg::latent_vl_lds_bytes(pb));
     const int lat_list = (pb.lat_g == 4 && c->opts.latent_method == 0) ? 4 : 0;
-    // colour-0 items past its blocks write colour 1's m_j of a split latent launch
-    const int combine = bsg::latent_split_active(pb, nullptr, nullptr) ? pb.tail_tiles : 0;
+    // colour-0 items past its blocks write colour 1's m_j from its pieces
+    const int combine = bsg::vl_ctiles(pb) ? bsg::vl_range(pb, 1).left : 0;
     const int tiles = lat_list == 4 ? bsg::latent_vl_blocks_g4(pb) : bsg::latent_vl_tiles(pb, pb.lat_g);
     for (int ch = 0; ch < c->C; ch++) {
       for (int t = 0; t < tiles; t++) lists[lat_list].push_back({a, ch, t});

@@ -73,13 +73,68 @@ struct DevProblem {
   int wm_update;
   double wm2;              // N sum_j WM_j^2
   double *bstat;           // [chain][BSTAT_SLOTS] block partials of sum_j WM_j (sum_i Z_ij - X'1 b_j)
-  // tail split of the Z-free latent launch at G = 4 (k_latent_vl_mfm): the
-  // launch's T mod ncu leftover 64-voxel tiles run as tail_k pieces each
+  // colour tiling of the Z-free latent launch at G = 4 (k_latent_vl_mfm, one
+  // chain): each colour's voxels in 64-voxel tiles from its first voxel; a
+  // colour's T mod ncu leftover tiles run as tl_k pieces each when tl_k > 1
   int ncu;                 // compute units of the device
-  int tail_tiles, tail_k;  // leftover tiles of a full-range launch; pieces per tile (<= 1: no split)
-  float *mpart;            // [chain][tail_tiles][tail_k][64][P] partial X'Z of the pieces
+  int ctile;               // colour tiling allowed (one chain)
+  int tl_left[2], tl_k[2]; // per colour: leftover tiles, pieces per leftover tile (<= 1: no split)
+  float *mpart;            // [chain][colour 0 pieces | colour 1 pieces][64][P] partial X'Z of the pieces
+  int mp_stride;           // floats of mpart per chain
   int zfree;               // Z not kept, X'Z fused into the latent kernel (z_store 2)
+  // Lambda of the overlapped sweep: (sweep tag << 32 | float bits) per entry,
+  // written by the launch that draws it, read by the colour-0 blocks of the
+  // same launch once the tag matches (no fence needed: each entry carries its tag)
+  unsigned long long *lamtag;  // [chain][P][P]
 };
+
+// Lanes per voxel of the Z-free latent kernels: enough lanes per voxel to give
+// the chip many waves, few enough that every lane keeps several quads. Past 16
+// quads the inversion path keeps G = 4, where its means run on the matrix cores.
+__host__ __device__ inline int vl_groups(const DevProblem &pb)
+{
+  if (pb.lat_g > 0) return pb.lat_g;
+  if (pb.NQ <= 4) return 1;
+  if (pb.NQ <= 16) return 2;
+  if (pb.NQ <= 128 || pb.latent_method == 0) return 4;
+  return 8;
+}
+
+// Colour tiling applies: one chain, Z-free inversion draws at G = 4, b fixed
+__host__ __device__ inline bool vl_ctiles(const DevProblem &pb)
+{
+  return pb.ctile && pb.zfree && pb.latent_method == 0 && !pb.wm_update && vl_groups(pb) == 4;
+}
+
+// Tiles of colour c (sorted voxels [base, end)): nmain whole tiles, then
+// `left` leftover tiles of K pieces each, whose partial X'Z sums sit at
+// mpart + chain * mp_stride + moff (piece k of leftover tile t at (t K + k) 64 P)
+struct VlRange {
+  int base, end, nmain, left, K, moff;
+};
+__host__ __device__ inline VlRange vl_range(const DevProblem &pb, int c)
+{
+  VlRange r;
+  r.base = c ? pb.n0 : 0;
+  r.end = c ? pb.V : pb.n0;
+  const int tiles = (r.end - r.base + 63) / 64;
+  r.left = pb.tl_k[c] > 1 ? pb.tl_left[c] : 0;
+  r.K = r.left ? pb.tl_k[c] : 1;
+  r.nmain = tiles - r.left;
+  r.moff = (c && pb.tl_k[0] > 1) ? pb.tl_left[0] * pb.tl_k[0] * 64 * pb.P : 0;
+  return r;
+}
+// blocks of a colour-tiled launch over colour set cs (bit c: colour c)
+__host__ __device__ inline int vl_blocks(const DevProblem &pb, int cs)
+{
+  int n = 0;
+  for (int c = 0; c < 2; c++)
+    if ((cs >> c) & 1) {
+      const VlRange r = vl_range(pb, c);
+      n += r.nmain + r.left * r.K;
+    }
+  return n;
+}
 
 // Posterior accumulators (device, per chain): see k_accum.
 struct Accum {
@@ -105,20 +160,33 @@ struct BatchIndex {
 // (0 when the kernel used does not form them; see launch_bwm).
 // wish = 1: the launch also draws the CAR precision of sweep wsweep (block 0;
 // only where latent_carries_wishart)
+// A colour-tiled launch (vl_ctiles) over the whole range, or over one colour's
+// voxels [0, n0) / [n0, V), writes its leftover tiles' X'Z as pieces:
+// *pieces (optional) gets the colour set whose m_j then sit in mpart.
 hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, int zstore, uint32_t sweep, hipStream_t st,
-                         int v0, int v1, int *bstat_parts = nullptr, int wish = 0, uint32_t wsweep = 0);
+                         int v0, int v1, int *bstat_parts = nullptr, int wish = 0, uint32_t wsweep = 0,
+                         int *pieces = nullptr);
 bool latent_carries_wishart(const DevProblem &pb);
+// Overlapped sweep (one chain, colour tiled, Lambda drawn inside the launch):
+// phase 0: [Wishart of sweep wsweep when wish] colour 1's latent draws of
+//          `sweep`, then colour 0's update of `sweep` (reading the new Lambda
+//          by its tags when wish);
+// phase 1: colour 0's latent draws of sweep + 1 (when lat), then colour 1's
+//          update of `sweep` (pair partials).
+// tail: colour set whose leftover pieces the update adds (its own colour).
+bool overlap_supported(const DevProblem &pb);
+hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int lat, int wish, uint32_t wsweep,
+                          int tail, hipStream_t st, const Accum *ac = nullptr, int batch = 0);
 // WM coefficient draw. parts > 0: the latent kernel already wrote that many
 // partials; parts == 0: they are formed here from the stored Z first.
 hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st);
 hipError_t launch_xtz(const DevProblem &pb, int mode, hipStream_t st, int v0, int v1);
-// tail = 1: the last latent launch of this problem was split (latent_split_active):
-// the update adds its own voxels' pieces, and colour 0 also writes colour 1's
-// m_j from their pieces in extra blocks, so colour 1 then runs with tail = 0
+// tail: colour set (bit c) whose leftover-tile m_j are still pieces in mpart.
+// The update adds its own colour's pieces; colour 0 also writes colour 1's
+// m_j from theirs in extra blocks, so colour 1 then finds them whole.
 // ac != null: a save sweep, each updated voxel is accumulated into ac (batch)
 hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st, int tail = 0,
                        const Accum *ac = nullptr, int batch = 0);
-bool latent_split_active(const DevProblem &pb, int *nmain, int *K);
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);

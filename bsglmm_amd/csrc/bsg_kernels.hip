@@ -691,8 +691,8 @@ __device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int P, bool WMU>
-__device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_t sweep, int chain, int t0,
-                                                   int tstep, int slot, int ra = 0, int rb = -1,
+__device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_t sweep, int chain, int vb, int ve,
+                                                   int t0, int tstep, int slot, int ra = 0, int rb = -1,
                                                    float *part = nullptr)
 {
   static_assert(P <= 8, "means take at most two MFMAs");
@@ -721,12 +721,12 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
   const int am0 = ((ga * R) * P + min(g, P - 1)) * 4 + ea;
   const int am1 = ((ga * R) * P + min(g + 4, P - 1)) * 4 + ea;
   const int q0 = g * R;
-  const int tiles = (pb.ve - pb.vb + 63) / 64;
+  const int tiles = (ve - vb + 63) / 64;
   double bpart = 0.0;
   for (int t = t0; t < tiles; t += tstep) {
-    const int s = pb.vb + (t * 4 + (threadIdx.x >> 6)) * 16 + vl;
-    const bool valid = s < pb.ve;
-    const int sv = valid ? s : pb.ve - 1;
+    const int s = vb + (t * 4 + (threadIdx.x >> 6)) * 16 + vl;
+    const bool valid = s < ve;
+    const int sv = valid ? s : ve - 1;
     const float wmv = pb.wm[sv];
     const float bw = b * wmv;
     const float *bp = pb.beta + ((size_t)chain * pb.V + sv) * P;
@@ -831,100 +831,74 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, int G, uint32_
 }
 
 template <int P>
-__device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part, uint32_t sweep);
+__device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part, uint32_t sweep,
+                              bool tagged = false);
 
-// Tail split of the inversion path at G = 4. A launch of T 64-voxel tiles over
-// the chip's C compute units runs ceil(T / C) tiles on some units where T / C
-// would be even. When the T mod C leftover tiles are few, each is split into
-// K pieces by subject rounds (piece k takes rounds [kR/K, (k+1)R/K)), which
-// spread over the whole chip. Block b < nmain runs tile b; block nmain + tK + k
-// runs piece k of leftover tile nmain + t and writes the tile's 64 partial
-// X'Z sums to mpart. The coefficient kernel that next reads such a voxel adds
-// its K partials in piece order (vl_tail_sum) and writes m_j back: the kernel
-// boundary orders the writes before the reads, so no fence or counter is
-// needed (a device-scope fence here writes back and invalidates the L2 of the
-// XCD, which slowed the whole launch). The pieces draw exactly the variates of
-// the whole tile (same counters); only the order of the X'Z sum differs. The
-// split depends on the problem alone, so a batch runs the same blocks as the
-// analysis alone.
-__host__ __device__ inline void vl_split(const DevProblem &pb, int &nmain, int &K)
-{
-  const int tiles = (pb.V + 63) / 64;
-  if (!pb.wm_update && pb.tail_k > 1 && tiles <= BSTAT_SLOTS && tiles % pb.ncu == pb.tail_tiles) {
-    nmain = tiles - pb.tail_tiles;
-    K = pb.tail_k;
-  } else {
-    nmain = tiles;
-    K = 1;
-  }
-}
-
-// Subject groups per voxel for the Z-free latent kernels: enough lanes per
-// voxel to give the chip many waves, few enough that every lane keeps several
-// quads. Past 16 quads the inversion path keeps G = 4, where its means run on
-// the matrix cores.
-__host__ __device__ inline int vl_groups(const DevProblem &pb)
-{
-  if (pb.lat_g > 0) return pb.lat_g;
-  if (pb.NQ <= 4) return 1;
-  if (pb.NQ <= 16) return 2;
-  if (pb.NQ <= 128 || pb.latent_method == 0) return 4;
-  return 8;
-}
-
-// True when a full-range Z-free latent launch of this problem runs split
-// (inversion draws at G = 4 and a leftover to spread): its leftover tiles'
-// X'Z is then K partial sums in mpart until a coefficient kernel adds them.
-__host__ __device__ inline bool vl_split_active(const DevProblem &pb, int &nmain, int &K)
-{
-  vl_split(pb, nmain, K);
-  if (!pb.zfree || pb.latent_method != 0 || vl_groups(pb) != 4) K = 1;
-  return K > 1;
-}
-
-// Block b of a full-range G = 4 inversion launch of sweep `sweep` (its Philox
-// word), split as vl_split says; tstep strides an unsplit launch whose blocks
-// hold several tiles.
-template <int P, bool WMU>
-__device__ __forceinline__ void vl_mfm_block(const DevProblem &pb, uint32_t sweep, int chain, int b, int tstep,
-                                             int nmain, int K)
-{
-  if (WMU || K <= 1) {
-    latent_vl_mfm_body<P, WMU>(pb, sweep, chain, b, tstep, b);
-    return;
-  }
-  if (b < nmain) {
-    latent_vl_mfm_body<P, false>(pb, sweep, chain, b, 1 << 30, 0);
-    return;
-  }
-  const int pidx = b - nmain, tl = pidx / K, k = pidx - tl * K;
-  const int R = (pb.NQ + 3) >> 2;
-  latent_vl_mfm_body<P, false>(pb, sweep, chain, nmain + tl, 1 << 30, 0, k * R / K, (k + 1) * R / K,
-                               pb.mpart + (((size_t)chain * pb.tail_tiles + tl) * K + k) * 64 * P);
-}
-
-// m_j of sorted voxel s of a leftover tile of a split launch: its K partial
-// sums added in piece order (the same order wherever it is formed)
+// Colour tiling of the inversion path at G = 4 (one chain). Each colour's
+// voxels form 64-voxel tiles from its first sorted voxel (vl_range). A launch
+// of T tiles over the chip's C compute units runs ceil(T / C) tiles on some
+// units where T / C would be even, so a colour's T mod C leftover tiles are
+// split, when few, into K pieces by subject rounds (piece k takes rounds
+// [kR/K, (k+1)R/K)), which spread over the whole chip. A launch over colour
+// set cs runs the whole tiles of each colour in it, then the pieces; each
+// piece writes its tile's 64 partial X'Z sums to mpart. The coefficient
+// kernel that next reads such a voxel adds its K partials in piece order
+// (vl_tail_sum) and writes m_j back: the kernel boundary orders the writes
+// before the reads, so no fence or counter is needed (a device-scope fence
+// here writes back and invalidates the L2 of the XCD, which slowed the whole
+// launch). The pieces draw exactly the variates of the whole tile (same
+// counters); only the order of the X'Z sum differs. Tiles and pieces depend
+// on the problem alone, so the whole-range launch, the one-colour launches of
+// the overlapped sweep and a batch all form the same sums.
 template <int P>
-__device__ __forceinline__ void vl_tail_sum(const DevProblem &pb, int chain, int s, int nmain, int K, float (&m)[P])
+__device__ __forceinline__ void vl_ctile_block(const DevProblem &pb, uint32_t sweep, int chain, int b, int cs)
 {
-  const int tl = (s >> 6) - nmain, v = s & 63;
-  const float *tp = pb.mpart + ((size_t)chain * pb.tail_tiles + tl) * K * 64 * P + (size_t)v * P;
+  const VlRange r0 = vl_range(pb, 0), r1 = vl_range(pb, 1);
+  const int m0 = (cs & 1) ? r0.nmain : 0, m1 = (cs & 2) ? r1.nmain : 0;
+  if (b < m0 + m1) {
+    const bool c0 = b < m0;
+    latent_vl_mfm_body<P, false>(pb, sweep, chain, c0 ? r0.base : r1.base, c0 ? r0.end : r1.end, c0 ? b : b - m0,
+                                 1 << 30, 0);
+    return;
+  }
+  b -= m0 + m1;
+  const int p0 = (cs & 1) ? r0.left * r0.K : 0;
+  const VlRange r = b < p0 ? r0 : r1;
+  if (b >= p0) b -= p0;
+  const int tl = b / r.K, k = b - tl * r.K;
+  const int R = (pb.NQ + 3) >> 2;
+  latent_vl_mfm_body<P, false>(pb, sweep, chain, r.base, r.end, r.nmain + tl, 1 << 30, 0, k * R / r.K,
+                               (k + 1) * R / r.K,
+                               pb.mpart + (size_t)chain * pb.mp_stride + r.moff + ((size_t)tl * r.K + k) * 64 * P);
+}
+
+// m_j of sorted voxel s when it lies in a leftover tile of its colour: its K
+// partial sums added in piece order (the same order wherever it is formed)
+template <int P>
+__device__ __forceinline__ bool vl_tail_sum(const DevProblem &pb, int chain, int s, float (&m)[P])
+{
+  const VlRange r = vl_range(pb, s >= pb.n0 ? 1 : 0);
+  const int loc = s - r.base, tl = (loc >> 6) - r.nmain;
+  if (tl < 0) return false;
+  const float *tp = pb.mpart + (size_t)chain * pb.mp_stride + r.moff + (size_t)tl * r.K * 64 * P + (size_t)(loc & 63) * P;
 #pragma unroll
   for (int c = 0; c < P; c++) m[c] = 0.0f;
-  for (int kk = 0; kk < K; kk++) {
+  for (int kk = 0; kk < r.K; kk++) {
 #pragma unroll
     for (int c = 0; c < P; c++) m[c] += tp[(size_t)kk * 64 * P + c];
   }
+  return true;
 }
 
 // the inversion path at G = 4; a kernel of its own so its registers are its
-// own. With wish = 1, block 0 draws the CAR precision of sweep wsweep (the
-// sweep before: its colour-1 partials are complete, and nothing here reads
-// Lambda) beside the latent blocks, instead of a launch of its own.
+// own. cs > 0: a colour-tiled launch over colour set cs; cs = 0: tiles of
+// [pb.vb, pb.ve) strided over the grid. With wish = 1, block 0 draws the CAR
+// precision of sweep wsweep (the sweep before: its colour-1 partials are
+// complete, and nothing here reads Lambda) beside the latent blocks, instead
+// of a launch of its own.
 template <int P, bool WMU>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_vl_mfm(DevProblem pb, uint32_t sweep, int wish,
-                                                                       uint32_t wsweep, int nmain, int K)
+                                                                       uint32_t wsweep, int cs)
 {
   const uint32_t base = *pb.sweep_base;
   const int chain = blockIdx.y;
@@ -935,7 +909,9 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_vl_mfm(DevProble
                      wsweep + base);
     return;
   }
-  vl_mfm_block<P, WMU>(pb, sweep + base, chain, blockIdx.x - wish, gridDim.x - wish, nmain, K);
+  const int b = blockIdx.x - wish;
+  if (!WMU && cs) vl_ctile_block<P>(pb, sweep + base, chain, b, cs);
+  else latent_vl_mfm_body<P, WMU>(pb, sweep + base, chain, pb.vb, pb.ve, b, gridDim.x - wish, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,9 +1141,11 @@ __device__ void chol_lower(double (&a)[P][P])
 }
 
 // Block of 256 threads: partial sums, Bartlett variates, FP64 algebra.
+// tagged: Lambda is also written to lamtag, each entry with tag 1 beside its
+// bits, for blocks of the same launch that wait for it (k_overlap)
 template <int P>
 __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part,
-                              uint32_t sweep)
+                              uint32_t sweep, bool tagged)
 {
   constexpr int NE = P * (P + 1) / 2;
   __shared__ double T[NE];
@@ -1234,6 +1212,10 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
       for (int k = 0; k <= K; k++) v += B[i][k] * B[j][k];
       lam64[i * P + j] = v;
       lam[i * P + j] = (float)v;
+      if (tagged)
+        __hip_atomic_store(pb.lamtag + (size_t)chain * P * P + i * P + j,
+                           (1ull << 32) | (unsigned long long)__float_as_uint((float)v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1388,7 +1370,7 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
 // Block `blk` of colour `colour` (256 voxels of its list) for chain `chain`.
 template <int P, bool PREC>
 __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint32_t sweep, int chain, int blk,
-                                          int tail, const Accum &ac, int batch)
+                                          int tail, const Accum &ac, int batch, const float *lamp = nullptr)
 {
   constexpr int NE = P * (P + 1) / 2;
   const int q = blk * 256 + threadIdx.x;
@@ -1396,7 +1378,7 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   const bool active = q < nc;
   const int s = (active ? q : nc - 1) + (colour ? pb.n0 : 0);
   const float *betc = pb.beta + (size_t)chain * pb.V * P;
-  const float *lam = pb.lam + (size_t)chain * P * P;
+  const float *lam = lamp ? lamp : pb.lam + (size_t)chain * P * P;
 
   float sv[P];
 #pragma unroll
@@ -1428,13 +1410,11 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
     }
   }
   // m_j = X'(Z_j - b WM_j) = X'Z_j - b WM_j X'1, with the b of this sweep;
-  // a voxel of a split launch's leftover tiles (tail) has its X'Z in pieces
+  // a voxel of a leftover tile of its colour (tail) has its X'Z in pieces
   float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
   const float bw = pb.bwm[chain] * pb.wm[s];
   float m[P];
-  int nmain, K;
-  if (tail && vl_split_active(pb, nmain, K) && s >= nmain * 64) {
-    vl_tail_sum<P>(pb, chain, s, nmain, K, m);
+  if (((tail >> colour) & 1) && vl_tail_sum<P>(pb, chain, s, m)) {
     if (active) {
 #pragma unroll
       for (int i = 0; i < P; i++) mz[i] = m[i];
@@ -1519,25 +1499,25 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   block_sum_store<NE>(acc, parts + (size_t)blk * NE);
 }
 
-// Colour 1's voxels in the leftover tiles of a split latent launch: block t of
-// the colour-0 launch's extra blocks adds their pieces (vl_tail_sum) and
-// writes m_j, beside the colour-0 update, so the colour-1 update reads m_j
-// like any other voxel's.
+// Colour 1's voxels in its leftover tiles: block t of the colour-0 launch's
+// extra blocks adds their pieces (vl_tail_sum) and writes m_j, beside the
+// colour-0 update, so the colour-1 update reads m_j like any other voxel's.
 template <int P>
 __device__ __forceinline__ void tail_combine_block(const DevProblem &pb, int chain, int t)
 {
-  int nmain, K;
-  if (!vl_split_active(pb, nmain, K) || threadIdx.x >= 64) return;
-  const int s = (nmain + t) * 64 + threadIdx.x;
-  if (s < pb.n0 || s >= pb.V) return;
+  const VlRange r = vl_range(pb, 1);
+  if (threadIdx.x >= 64) return;
+  const int s = r.base + (r.nmain + t) * 64 + threadIdx.x;
+  if (s >= r.end) return;
   float m[P];
-  vl_tail_sum<P>(pb, chain, s, nmain, K, m);
+  vl_tail_sum<P>(pb, chain, s, m);
   float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
 #pragma unroll
   for (int c = 0; c < P; c++) mz[c] = m[c];
 }
 
-// blocks past the colour's own (colour 0 with tail) run tail_combine_block;
+// blocks past the colour's own (colour 0 with colour 1 in tail) run
+// tail_combine_block;
 // ac.sum != null: a save sweep, each updated voxel is accumulated (batch)
 template <int P, bool PREC>
 __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_t sweep, int tail, Accum ac,
@@ -1550,6 +1530,71 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
     return;
   }
   coef_body<P, PREC>(pb, colour, sweep, blockIdx.y, blockIdx.x, tail, ac, batch);
+}
+
+// ---------------------------------------------------------------------------
+// k_overlap: the sweep of one chain as two launches, each colour's update
+// beside the other colour's latent draws. Z_j and X'Z_j depend on b_j alone,
+// and each colour's update reads only the other colour's coefficients, so
+//   phase 0: [block 0: Lambda of sweep wsweep when wish] colour 1's latent
+//            draws of `sweep` (nlat blocks), colour 0's update of `sweep`;
+//   phase 1: colour 0's latent draws of sweep + 1 (nlat blocks, 0 after the
+//            last sweep), colour 1's update of `sweep` with its pair partials;
+// the kernel boundaries order everything else. Latent blocks come first so
+// that they all start at once; the update blocks take the slots that the
+// short leftover pieces free. Colour 0's update needs the Lambda that block
+// 0 of the same launch draws: it reads lamtag, whose entries carry tag 1
+// beside their bits, so a block proceeds once every entry it reads shows the
+// tag, with no fence (phase 1 clears the tags for the next phase 0). The
+// wait is bounded; past the bound flags[2] is set and the host reports it.
+// Every variate and sum is the one the serial sweep forms with colour tiles.
+// ---------------------------------------------------------------------------
+constexpr int LAM_SPIN_MAX = 1 << 20;
+
+template <int P>
+__global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, int phase, uint32_t sweep, int nlat,
+                                                               int wish, uint32_t wsweep, int tail, Accum ac,
+                                                               int batch)
+{
+  constexpr int NE = P * (P + 1) / 2;
+  const uint32_t base = *pb.sweep_base;
+  int b = blockIdx.x;
+  if (phase == 1 && b == 0 && threadIdx.x < P * P) pb.lamtag[threadIdx.x] = 0ull;
+  if (wish) {
+    if (b == 0) {
+      wishart_block<P>(pb, 0, (pb.V - pb.n0 + 255) / 256, pb.partials, wsweep + base, true);
+      return;
+    }
+    b--;
+  }
+  if (b < nlat) {
+    vl_ctile_block<P>(pb, sweep + base + (uint32_t)phase, 0, b, phase ? 1 : 2);
+    return;
+  }
+  b -= nlat;
+  if (phase == 1) {
+    coef_body<P, true>(pb, 1, sweep + base, 0, b, tail, ac, batch);
+    return;
+  }
+  __shared__ float lam_s[P * P];
+  if (wish) {
+    if (threadIdx.x < P * P) {
+      unsigned long long v = 0ull;
+      for (int it = 0;; it++) {
+        v = __hip_atomic_load(pb.lamtag + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((v >> 32) == 1ull) break;
+        if (it >= LAM_SPIN_MAX) {
+          atomicOr(&pb.flags[2], 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+      lam_s[threadIdx.x] = __uint_as_float((uint32_t)v);
+    }
+    __syncthreads();
+  }
+  coef_body<P, false>(pb, 0, sweep + base, 0, b, tail, ac, batch, wish ? lam_s : nullptr);
+  (void)NE;
 }
 
 // ---------------------------------------------------------------------------
@@ -1705,16 +1750,15 @@ __global__ __launch_bounds__(256) void k_latent_multi(const DevProblem *pbs, con
 }
 
 // members drawn by inversion at G = 4: item blk is block blk of the member's
-// own launch (tail split included)
+// own whole-range launch (colour tiles and pieces included)
 template <int P>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_multi_mfm(const DevProblem *pbs,
                                                                           const WorkItem *items, uint32_t sweep)
 {
   const WorkItem it = items[blockIdx.x];
   const DevProblem pb = pbs[it.a];
-  int nmain, K;
-  vl_split(pb, nmain, K);
-  vl_mfm_block<P, false>(pb, sweep + *pb.sweep_base, it.chain, it.blk, 1 << 30, nmain, K);
+  if (vl_ctiles(pb)) vl_ctile_block<P>(pb, sweep + *pb.sweep_base, it.chain, it.blk, 3);
+  else latent_vl_mfm_body<P, false>(pb, sweep + *pb.sweep_base, it.chain, 0, pb.V, it.blk, 1 << 30, 0);
 }
 
 template <int P, bool PREC>
@@ -1723,16 +1767,17 @@ __global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const
 {
   const WorkItem it = items[blockIdx.x];
   const DevProblem pb = pbs[it.a];
-  // the batch's latent launch always precedes: colour 0 adds its own leftover
-  // pieces and, in the items past its blocks, colour 1's (tail_combine_block)
+  // the batch's latent launch always precedes: with colour tiles, colour 0
+  // adds its own leftover pieces and, in the items past its blocks, colour 1's
+  // (tail_combine_block)
   const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
   if (it.blk >= ncb) {
     tail_combine_block<P>(pb, it.chain, it.blk - ncb);
     return;
   }
   const Accum ac = accs ? accs[it.a] : Accum{};
-  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk, colour == 0 ? 1 : 0, ac,
-                     accs ? bidx.b[it.a] : 0);
+  coef_body<P, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk, (colour == 0 && vl_ctiles(pb)) ? 1 : 0,
+                     ac, accs ? bidx.b[it.a] : 0);
 }
 
 template <int P>
@@ -1782,15 +1827,6 @@ int latent_vl_groups(const DevProblem &pb)
   return vl_groups(q);
 }
 
-bool latent_split_active(const DevProblem &pb, int *nmain, int *K)
-{
-  int nm, k;
-  const bool on = vl_split_active(pb, nm, k);
-  if (nmain) *nmain = nm;
-  if (K) *K = k;
-  return on;
-}
-
 // X staging of either body (the inversion path at G = 4 pads X to 4R quads,
 // R = ceil(NQ / 4)), plus the fallback areas of method 1
 static size_t latent_vl_lds(const DevProblem &pb)
@@ -1799,8 +1835,24 @@ static size_t latent_vl_lds(const DevProblem &pb)
   return x + (pb.latent_method == 1 ? vl_fallback_lds(pb.P) : 0);
 }
 
+// colour set of a colour-tiled launch over [pb.vb, pb.ve) at lane split G (0: none)
+static int ctile_set(const DevProblem &pb, int G)
+{
+  if (G != 4 || !vl_ctiles(pb)) return 0;
+  if (pb.vb == 0 && pb.ve == pb.V) return 3;
+  if (pb.vb == 0 && pb.ve == pb.n0) return 1;
+  if (pb.vb == pb.n0 && pb.ve == pb.V) return 2;
+  return 0;
+}
+
+// colours of set cs whose leftover tiles run as pieces
+static int piece_set(const DevProblem &pb, int cs)
+{
+  return cs & ((pb.tl_k[0] > 1 ? 1 : 0) | (pb.tl_k[1] > 1 ? 2 : 0));
+}
+
 static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, hipStream_t st, int *parts,
-                                   int wish, uint32_t wsweep)
+                                   int wish, uint32_t wsweep, int *pieces)
 {
   const size_t lds = latent_vl_lds(pb);
   const int vpb = 4 * (64 / G);  // voxels per block tile
@@ -1810,12 +1862,10 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
   const bool wmu = pb.wm_update != 0, bmf = pb.latent_method == 1;
   if (G != 1 && G != 2 && G != 4 && G != 8) return hipErrorInvalidValue;
   if (wish && (G != 4 || bmf || pb.P > 6)) return hipErrorInvalidValue;
-  // tail split of the inversion path at G = 4 over the whole range
-  int nmain = nlb, K = 1;
-  if (G == vl_groups(pb) && pb.vb == 0 && pb.ve == pb.V && vl_split_active(pb, nmain, K))
-    nlb = nmain + pb.tail_tiles * K;
-  else
-    nmain = nlb, K = 1;
+  // colour tiles (and pieces) of the inversion path at G = 4
+  const int cs = ctile_set(pb, G);
+  if (cs) nlb = vl_blocks(pb, cs);
+  if (pieces) *pieces = piece_set(pb, cs);
   dim3 grid((unsigned)(nlb + wish), (unsigned)pb.nchains);
 #define V_(PP)                                                                                             \
   do {                                                                                                     \
@@ -1823,8 +1873,8 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, true>), grid, dim3(256), lds, st, pb, G, sweep);   \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, true>), grid, dim3(256), lds, st, pb, G, sweep);      \
     } else if (G == 4) {                                                                                   \
-      if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, nmain, K); \
-      else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, nmain, K);    \
+      if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, cs); \
+      else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, cs);    \
     } else {                                                                                               \
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, false>), grid, dim3(256), lds, st, pb, G, sweep);  \
       else hipLaunchKernelGGL((k_latent_vl<PP, false, false>), grid, dim3(256), lds, st, pb, G, sweep);     \
@@ -1855,17 +1905,19 @@ static bool set_range(DevProblem &pb, int v0, int v1)
 }
 
 hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t sweep, hipStream_t st, int v0, int v1,
-                         int *bstat_parts, int wish, uint32_t wsweep)
+                         int *bstat_parts, int wish, uint32_t wsweep, int *pieces)
 {
   DevProblem pb = pb0;
   if (bstat_parts) *bstat_parts = 0;
+  if (pieces) *pieces = 0;
   if (!set_range(pb, v0, v1)) return hipErrorInvalidValue;
   if (v0 == v1 && !wish) return hipSuccess;
   if (fused && zstore == 2) {
     // the Z-free form needs X in LDS; the WM statistic needs the whole voxel range
     if (latent_vl_lds(pb) > 65536) return hipErrorInvalidValue;
     if (pb.wm_update && (v0 != 0 || v1 != pb.V)) return hipErrorInvalidValue;
-    return launch_latent_vl(pb, pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb), sweep, st, bstat_parts, wish, wsweep);
+    return launch_latent_vl(pb, pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb), sweep, st, bstat_parts, wish, wsweep,
+                            pieces);
   }
   if (wish) return hipErrorInvalidValue;
   int lshift, reps;
@@ -1960,8 +2012,7 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
   const Accum acc = ac ? *ac : Accum{};
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   if (nc == 0) return with_prec ? hipErrorInvalidValue : hipSuccess;
-  int nmain, K;
-  const int extra = (colour == 0 && tail && vl_split_active(pb, nmain, K)) ? pb.tail_tiles : 0;
+  const int extra = (colour == 0 && (tail & 2)) ? vl_range(pb, 1).left : 0;
   dim3 grid((unsigned)((nc + 255) / 256 + extra), (unsigned)pb.nchains);
 #define L_(PP)                                                                                \
   do {                                                                                        \
@@ -1971,6 +2022,40 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
       hipLaunchKernelGGL((k_coef<PP, false>), grid, dim3(256), 0, st, pb, colour, sweep, tail, acc, batch); \
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+// The overlapped sweep: one chain, colour tiles, Lambda drawn inside phase 0
+// (P <= 6, as latent_carries_wishart), both colours present.
+bool overlap_supported(const DevProblem &pb)
+{
+  return pb.nchains == 1 && vl_ctiles(pb) && pb.n0 > 0 && pb.V > pb.n0 && latent_carries_wishart(pb) &&
+         pb.lamtag != nullptr;
+}
+
+hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int lat, int wish, uint32_t wsweep,
+                          int tail, hipStream_t st, const Accum *ac, int batch)
+{
+  if (!overlap_supported(pb) || (phase != 0 && phase != 1) || (phase == 1 && wish)) return hipErrorInvalidValue;
+  const Accum acc = ac ? *ac : Accum{};
+  const int colour = phase ? 1 : 0;  // the updated colour; the other one is drawn
+  const int nlat = (phase == 0 || lat) ? vl_blocks(pb, phase ? 1 : 2) : 0;
+  const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
+  const size_t lds = latent_vl_lds(pb);
+  dim3 grid((unsigned)(wish + nlat + ncb));
+#define L_(PP)                                                                                                  \
+  hipLaunchKernelGGL((k_overlap<PP>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep, tail, acc, \
+                     batch)
+  switch (pb.P) {
+    case 1: L_(1); break;
+    case 2: L_(2); break;
+    case 3: L_(3); break;
+    case 4: L_(4); break;
+    case 5: L_(5); break;
+    case 6: L_(6); break;
+    default: return hipErrorInvalidValue;
+  }
 #undef L_
   return hipGetLastError();
 }
@@ -2086,9 +2171,7 @@ int latent_vl_tiles(const DevProblem &pb, int G) { return (pb.V + 4 * (64 / G) -
 
 int latent_vl_blocks_g4(const DevProblem &pb)
 {
-  int nmain, K;
-  vl_split(pb, nmain, K);
-  return K > 1 ? nmain + pb.tail_tiles * K : nmain;
+  return vl_ctiles(pb) ? vl_blocks(pb, 3) : (pb.V + 63) / 64;
 }
 
 size_t latent_vl_lds_bytes(const DevProblem &pb) { return latent_vl_lds(pb); }

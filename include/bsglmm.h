@@ -116,7 +116,9 @@ int bsg_update_car_prec(bsg_ctx *ctx);
 /* waits for the context's stream; reports kernel-side numerical failures */
 int bsg_synchronize(bsg_ctx *ctx);
 /* bsg_sweep with HIP events around every latent kernel: latent_ms receives
- * their summed durations, total_ms (optional) the whole span. Synchronous. */
+ * their summed durations, total_ms (optional) the whole span. Synchronous.
+ * Overlapped sweeps (bsg_get_overlap): the events bracket each sweep's two
+ * launches, which both draw latent variates. */
 int bsg_sweep_timed(bsg_ctx *ctx, int nsweeps, double *latent_ms, double *total_ms);
 /* nsweeps sweeps launched kernel by kernel with HIP events around each one;
  * kernel_ms[6] receives the summed durations of latent, xtz, coef0, coef1,
@@ -183,6 +185,15 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g);
  * exact at every split; the split changes FP32 summation order only. */
 int bsg_set_lane_split(bsg_ctx *ctx, int g);
 int bsg_get_lane_split(bsg_ctx *ctx, int *g); /* the split the next sweep uses */
+/* Overlapped sweeps (on by default; no counterpart in the reference, whose
+ * sweep runs its kernels one after another, mcmc.cpp:311-394): full sweeps
+ * of a one-chain context whose latent kernel runs at four lanes per voxel
+ * (inversion draws, no WM image, P <= 6) run as two launches per sweep, each
+ * colour's coefficient update beside the other colour's latent draws. Same
+ * variates, same sums, same chain as with on = 0. bsg_get_overlap tells
+ * whether the next bsg_sweep* call runs overlapped. */
+int bsg_set_overlap(bsg_ctx *ctx, int on);
+int bsg_get_overlap(bsg_ctx *ctx, int *on);
 typedef struct bsg_batch bsg_batch;
 /* Groups up to 32 contexts (independent analyses: own masks, N, P and chain
  * counts) so that each phase of a sweep is one kernel launch over all of

@@ -378,6 +378,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     if (mp > 0) ALLOC(pb.mpart, (size_t)C * mp);
   }
   ALLOC(pb.lamtag, (size_t)C * P * P);
+  ALLOC(pb.wbart, (size_t)C * P * P);
 
   ALLOC(pb.flags, (size_t)4);
   ALLOC(ctx->sweep_words, (size_t)2);

@@ -82,10 +82,12 @@ struct DevProblem {
   float *mpart;            // [chain][colour 0 pieces | colour 1 pieces][64][P] partial X'Z of the pieces
   int mp_stride;           // floats of mpart per chain
   int zfree;               // Z not kept, X'Z fused into the latent kernel (z_store 2)
-  // Lambda of the overlapped sweep: (sweep tag << 32 | float bits) per entry,
-  // written by the launch that draws it, read by the colour-0 blocks of the
-  // same launch once the tag matches (no fence needed: each entry carries its tag)
+  // Lambda of the overlapped sweep: (1 << 32 | float bits) per entry, written
+  // by the block that draws it, read by the colour-0 blocks of the same
+  // launch once their entries show the tag (no fence needed: each entry
+  // carries its tag); cleared by the launch before
   unsigned long long *lamtag;  // [chain][P][P]
+  double *wbart;               // [chain][P][P] Bartlett factor of the next overlapped Lambda draw
 };
 
 // Lanes per voxel of the Z-free latent kernels: enough lanes per voxel to give
@@ -171,7 +173,8 @@ bool latent_carries_wishart(const DevProblem &pb);
 // phase 0: [Wishart of sweep wsweep when wish] colour 1's latent draws of
 //          `sweep`, then colour 0's update of `sweep` (reading the new Lambda
 //          by its tags when wish);
-// phase 1: colour 0's latent draws of sweep + 1 (when lat), then colour 1's
+// phase 1: the Bartlett factor of `sweep`'s Lambda (for the next phase 0),
+//          colour 0's latent draws of sweep + 1 (when lat), then colour 1's
 //          update of `sweep` (pair partials).
 // tail: colour set whose leftover pieces the update adds (its own colour).
 bool overlap_supported(const DevProblem &pb);

@@ -832,7 +832,7 @@ __global__ __launch_bounds__(256) void k_latent_vl(DevProblem pb, int G, uint32_
 
 template <int P>
 __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part, uint32_t sweep,
-                              bool tagged = false);
+                              bool tagged = false, const double *apre = nullptr);
 
 // Colour tiling of the inversion path at G = 4 (one chain). Each colour's
 // voxels form 64-voxel tiles from its first sorted voxel (vl_range). A launch
@@ -1140,12 +1140,31 @@ __device__ void chol_lower(double (&a)[P][P])
   }
 }
 
+// Entry idx = i P + j of the Bartlett factor of the Wishart draw of sweep
+// `sweep`: sqrt(chi2(df - i)) on the diagonal, N(0, 1) below, 0 above. It
+// depends on the sweep index alone, so it can be drawn ahead (k_overlap).
+template <int P>
+__device__ __forceinline__ double bartlett_entry(const DevProblem &pb, int chain, int idx, uint32_t sweep)
+{
+  const int i = idx / P, j = idx % P;
+  double val = 0.0;
+  if (i == j) {
+    val = sqrt(2.0 * gamma_mt(0.5 * (double)(pb.df - i), (uint32_t)i, sweep, (uint32_t)chain, pb.key0, pb.key1));
+  } else if (j < i) {
+    uint32_t rr[4];
+    philox10((uint32_t)idx, sweep, TAG_WNORM, (uint32_t)chain, pb.key0, pb.key1, rr);
+    val = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
+  }
+  return val;
+}
+
 // Block of 256 threads: partial sums, Bartlett variates, FP64 algebra.
 // tagged: Lambda is also written to lamtag, each entry with tag 1 beside its
-// bits, for blocks of the same launch that wait for it (k_overlap)
+// bits, for blocks of the same launch that wait for it (k_overlap).
+// apre: the Bartlett factor of this sweep, drawn ahead (else drawn here).
 template <int P>
 __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const double *part,
-                              uint32_t sweep, bool tagged)
+                              uint32_t sweep, bool tagged, const double *apre)
 {
   constexpr int NE = P * (P + 1) / 2;
   __shared__ double T[NE];
@@ -1153,19 +1172,8 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
   sum_partials<NE>(nparts, part, T);
   const int lane = threadIdx.x;
   if (lane >= 64) return;
-  for (int idx = lane; idx < P * P; idx += 64) {
-    const int i = idx / P, j = idx % P;
-    double val = 0.0;
-    if (i == j) {
-      val = sqrt(2.0 * gamma_mt(0.5 * (double)(pb.df - i), (uint32_t)i, sweep, (uint32_t)chain,
-                                pb.key0, pb.key1));
-    } else if (j < i) {
-      uint32_t rr[4];
-      philox10((uint32_t)idx, sweep, TAG_WNORM, (uint32_t)chain, pb.key0, pb.key1, rr);
-      val = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
-    }
-    A[i][j] = val;
-  }
+  for (int idx = lane; idx < P * P; idx += 64)
+    A[idx / P][idx % P] = apre ? apre[idx] : bartlett_entry<P>(pb, chain, idx, sweep);
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -1367,10 +1375,80 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
   return true;
 }
 
+// FP64 refactorisation of one voxel's update (its FP32 pivot fell below
+// pivot_min): inputs and result by value, in a function of its own, so that
+// its registers do not count against the FP32 path that nearly every voxel
+// takes. ok = 0: the precision is not positive definite.
+template <int P>
+struct CoefIn {
+  float m[P], sv[P], z[P];
+  float nn;
+};
+template <int P>
+struct CoefOut {
+  float out[P];
+  int ok;
+};
+template <int P>
+__device__ __forceinline__ CoefOut<P> coef_fp64(CoefIn<P> in, const float *lam, const float *xtx)
+{
+  double Qd[P][P], rd[P], zd[P], od[P];
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    double acc = (double)in.m[i];
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+      double lij = (double)lam[i * P + j];
+      acc += lij * (double)in.sv[j];
+      Qd[i][j] = (double)xtx[i * P + j] + (double)in.nn * lij;
+    }
+    rd[i] = acc;
+    zd[i] = (double)in.z[i];
+  }
+  CoefOut<P> o;
+  o.ok = ul_draw<P, double>(Qd, rd, zd, od, 0.0) ? 1 : 0;
+#pragma unroll
+  for (int c = 0; c < P; c++) o.out[c] = (float)od[c];
+  return o;
+}
+
 // Block `blk` of colour `colour` (256 voxels of its list) for chain `chain`.
-template <int P, bool PREC>
+// KEEPNB (colour 1): the six neighbour coefficient vectors stay in registers
+// for the pair terms; otherwise the pair terms read them again (the same
+// values: nothing writes the other colour meanwhile), for kernels whose
+// register budget is shared with the latent draws.
+// The CAR precision drawn by another block of the same launch (k_overlap):
+// its entries carry tag 1 beside their bits in lamtag, so the block copies
+// them to LDS once every entry shows the tag. The wait is bounded; past the
+// bound flags[2] is set (the host reports it) and the block goes on.
+constexpr int LAM_SPIN_MAX = 1 << 20;
+template <int P>
+__device__ __forceinline__ const float *lam_wait(const DevProblem &pb, const unsigned long long *lamw)
+{
+  __shared__ float lam_s[P * P];
+  if (threadIdx.x < P * P) {
+    unsigned long long v = 0ull;
+    for (int it = 0;; it++) {
+      v = __hip_atomic_load(lamw + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((v >> 32) == 1ull) break;
+      if (it >= LAM_SPIN_MAX) {
+        atomicOr(&pb.flags[2], 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+    lam_s[threadIdx.x] = __uint_as_float((uint32_t)v);
+  }
+  __syncthreads();
+  return lam_s;
+}
+
+// lamw != null: Lambda is read through lam_wait, after this block's
+// neighbour and X'Z loads are under way
+template <int P, bool PREC, bool KEEPNB = PREC>
 __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint32_t sweep, int chain, int blk,
-                                          int tail, const Accum &ac, int batch, const float *lamp = nullptr)
+                                          int tail, const Accum &ac, int batch,
+                                          const unsigned long long *lamw = nullptr)
 {
   constexpr int NE = P * (P + 1) / 2;
   const int q = blk * 256 + threadIdx.x;
@@ -1378,7 +1456,7 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   const bool active = q < nc;
   const int s = (active ? q : nc - 1) + (colour ? pb.n0 : 0);
   const float *betc = pb.beta + (size_t)chain * pb.V * P;
-  const float *lam = lamp ? lamp : pb.lam + (size_t)chain * P * P;
+  const float *lam = pb.lam + (size_t)chain * P * P;
 
   float sv[P];
 #pragma unroll
@@ -1392,7 +1470,8 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
     nb[2 * d] = v.x;
     nb[2 * d + 1] = v.y;
   }
-  float bk[PREC ? 6 : 1][P];  // colour 1 keeps the neighbours for its pair terms
+  constexpr bool KB = PREC && KEEPNB;
+  float bk[KB ? 6 : 1][P];  // colour 1 keeps the neighbours for its pair terms
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     if (nb[d] >= 0) {
@@ -1402,11 +1481,11 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
       for (int c = 0; c < P; c++) {
         const float v = src[c];
         sv[c] += v;
-        if (PREC) bk[PREC ? d : 0][c] = v;
+        if (KB) bk[KB ? d : 0][c] = v;
       }
-    } else if (PREC) {
+    } else if (KB) {
 #pragma unroll
-      for (int c = 0; c < P; c++) bk[PREC ? d : 0][c] = 0.0f;
+      for (int c = 0; c < P; c++) bk[KB ? d : 0][c] = 0.0f;
     }
   }
   // m_j = X'(Z_j - b WM_j) = X'Z_j - b WM_j X'1, with the b of this sweep;
@@ -1425,6 +1504,7 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   }
 #pragma unroll
   for (int i = 0; i < P; i++) m[i] = fmaf(-bw, pb.xsum[i], m[i]);
+  if (lamw) lam = lam_wait<P>(pb, lamw);
   float Q[P][P], r[P];
 #pragma unroll
   for (int i = 0; i < P; i++) {
@@ -1456,23 +1536,19 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   float *dst = pb.beta + ((size_t)chain * pb.V + s) * P;
   if (!ul_draw<P, float>(Q, r, z, out, pb.pivot_min)) {
     // FP64 refactorisation of this voxel (FP32 pivot below pivot_min)
-    double Qd[P][P], rd[P], zd[P], od[P];
+    CoefIn<P> in;
 #pragma unroll
     for (int i = 0; i < P; i++) {
-      double acc = (double)m[i];
-#pragma unroll
-      for (int j = 0; j < P; j++) {
-        double lij = (double)lam[i * P + j];
-        acc += lij * (double)sv[j];
-        Qd[i][j] = (double)pb.xtx[i * P + j] + (double)nn * lij;
-      }
-      rd[i] = acc;
-      zd[i] = (double)z[i];
+      in.m[i] = m[i];
+      in.sv[i] = sv[i];
+      in.z[i] = z[i];
     }
+    in.nn = (float)nn;
     if (active) atomicAdd(&pb.flags[0], 1u);
-    if (ul_draw<P, double>(Qd, rd, zd, od, 0.0)) {
+    const CoefOut<P> o = coef_fp64<P>(in, lam, pb.xtx);
+    if (o.ok) {
 #pragma unroll
-      for (int c = 0; c < P; c++) out[c] = (float)od[c];
+      for (int c = 0; c < P; c++) out[c] = o.out[c];
     } else {
       // not positive definite: keep the previous coefficients
       if (active) atomicAdd(&pb.flags[1], 1u);
@@ -1493,7 +1569,17 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   if (active) {
 #pragma unroll
     for (int d = 0; d < 6; d++)
-      if (nb[d] >= 0) add_pair<P>(out, bk[PREC ? d : 0], (d & 1) != 0, acc);
+      if (nb[d] >= 0) {
+        if (KB) {
+          add_pair<P>(out, bk[KB ? d : 0], (d & 1) != 0, acc);
+        } else {
+          float bn[P];
+          const float *src = betc + (size_t)nb[d] * P;
+#pragma unroll
+          for (int c = 0; c < P; c++) bn[c] = src[c];
+          add_pair<P>(out, bn, (d & 1) != 0, acc);
+        }
+      }
   }
   double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
   block_sum_store<NE>(acc, parts + (size_t)blk * NE);
@@ -1539,30 +1625,27 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
 //   phase 0: [block 0: Lambda of sweep wsweep when wish] colour 1's latent
 //            draws of `sweep` (nlat blocks), colour 0's update of `sweep`;
 //   phase 1: colour 0's latent draws of sweep + 1 (nlat blocks, 0 after the
-//            last sweep), colour 1's update of `sweep` with its pair partials;
-// the kernel boundaries order everything else. Latent blocks come first so
-// that they all start at once; the update blocks take the slots that the
-// short leftover pieces free. Colour 0's update needs the Lambda that block
-// 0 of the same launch draws: it reads lamtag, whose entries carry tag 1
-// beside their bits, so a block proceeds once every entry it reads shows the
-// tag, with no fence (phase 1 clears the tags for the next phase 0). The
-// wait is bounded; past the bound flags[2] is set and the host reports it.
-// Every variate and sum is the one the serial sweep forms with colour tiles.
+//            last sweep), colour 1's update of `sweep` with its pair
+//            partials, [last block: the Bartlett factor of this sweep's
+//            Lambda, drawn ahead for the next phase 0; the Lambda tags cleared];
+// the kernel boundaries order everything else. Latent blocks come before
+// the update blocks so that they all start at once (and keep their block
+// index, hence their XCD, from sweep to sweep); the update blocks take the
+// slots that the short leftover pieces free. Colour 0's update needs the
+// Lambda that block 0 of the same launch draws: it reads it by its tags
+// (lam_wait), with no fence. Every variate and sum is the one the serial
+// sweep forms with colour tiles.
 // ---------------------------------------------------------------------------
-constexpr int LAM_SPIN_MAX = 1 << 20;
-
 template <int P>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, int phase, uint32_t sweep, int nlat,
                                                                int wish, uint32_t wsweep, int tail, Accum ac,
                                                                int batch)
 {
-  constexpr int NE = P * (P + 1) / 2;
   const uint32_t base = *pb.sweep_base;
   int b = blockIdx.x;
-  if (phase == 1 && b == 0 && threadIdx.x < P * P) pb.lamtag[threadIdx.x] = 0ull;
-  if (wish) {
+  if (phase == 0 && wish) {
     if (b == 0) {
-      wishart_block<P>(pb, 0, (pb.V - pb.n0 + 255) / 256, pb.partials, wsweep + base, true);
+      wishart_block<P>(pb, 0, (pb.V - pb.n0 + 255) / 256, pb.partials, wsweep + base, true, pb.wbart);
       return;
     }
     b--;
@@ -1572,29 +1655,20 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, 
     return;
   }
   b -= nlat;
-  if (phase == 1) {
-    coef_body<P, true>(pb, 1, sweep + base, 0, b, tail, ac, batch);
+  if (phase == 0) {
+    coef_body<P, false>(pb, 0, sweep + base, 0, b, tail, ac, batch, wish ? pb.lamtag : nullptr);
     return;
   }
-  __shared__ float lam_s[P * P];
-  if (wish) {
-    if (threadIdx.x < P * P) {
-      unsigned long long v = 0ull;
-      for (int it = 0;; it++) {
-        v = __hip_atomic_load(pb.lamtag + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((v >> 32) == 1ull) break;
-        if (it >= LAM_SPIN_MAX) {
-          atomicOr(&pb.flags[2], 1u);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(8);
-      }
-      lam_s[threadIdx.x] = __uint_as_float((uint32_t)v);
-    }
-    __syncthreads();
+  const int ncb = (pb.V - pb.n0 + 255) / 256;
+  if (b < ncb) {
+    coef_body<P, true, false>(pb, 1, sweep + base, 0, b, tail, ac, batch);
+    return;
   }
-  coef_body<P, false>(pb, 0, sweep + base, 0, b, tail, ac, batch, wish ? lam_s : nullptr);
-  (void)NE;
+  // the last block: Bartlett factor of this sweep's Lambda, tags cleared
+  if (threadIdx.x < P * P) {
+    pb.lamtag[threadIdx.x] = 0ull;
+    pb.wbart[threadIdx.x] = bartlett_entry<P>(pb, 0, threadIdx.x, sweep + base);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2027,7 +2101,8 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 }
 
 // The overlapped sweep: one chain, colour tiles, Lambda drawn inside phase 0
-// (P <= 6, as latent_carries_wishart), both colours present.
+// (P <= 6, as latent_carries_wishart), both colours present. Phase 1 draws
+// the Bartlett factor of its sweep's Lambda ahead, for the phase 0 after it.
 bool overlap_supported(const DevProblem &pb)
 {
   return pb.nchains == 1 && vl_ctiles(pb) && pb.n0 > 0 && pb.V > pb.n0 && latent_carries_wishart(pb) &&
@@ -2043,7 +2118,7 @@ hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int l
   const int nlat = (phase == 0 || lat) ? vl_blocks(pb, phase ? 1 : 2) : 0;
   const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
   const size_t lds = latent_vl_lds(pb);
-  dim3 grid((unsigned)(wish + nlat + ncb));
+  dim3 grid((unsigned)(wish + nlat + ncb + (phase ? 1 : 0)));
 #define L_(PP)                                                                                                  \
   hipLaunchKernelGGL((k_overlap<PP>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep, tail, acc, \
                      batch)

@@ -8,15 +8,18 @@ checkerboard coefficient updates (two colours), FP64 pair sums and the device
 Wishart draw. All inputs are resident in HBM before timing starts.
 
 Headline form: X'Z is formed inside the latent kernel from draws held in
-registers (FP32 FMA), and Z is not written (k_latent_vl_mfm, whose per-draw
-means x_i'b_j come from FP32 MFMAs; Z is a transient of the sweep).
-Beside it, in the same run: the fused kernel that also stores Z, and the
-sweep with a separate X'Z kernel re-reading Z (FP32 FMA, and BF16 hi+lo on
-MFMA). Prints one JSON line with the dominant kernel's roofline (its own
-algorithmic bytes against 8 TB/s, plus its VALU issue rate, the bound it
-actually meets), the sweep-level HBM fraction of the algorithmic sweep bytes
-N V (8 + 1/8) + V (24 P + 5), a measured copy-kernel bandwidth, and the
-reference CPU path timed on this host for a bounded sample.
+registers (FP32 FMA), Z is not written, and the sweep is overlapped: two
+launches of k_overlap per sweep, each drawing one colour's latent variates
+(per-draw means x_i'b_j on FP32 MFMAs) beside the other colour's coefficient
+update, with the CAR-precision draw inside the first. Beside it, in the same
+run: the same Z-free sweep in the serial order (k_latent_vl_mfm, then the two
+colour updates; the same chain bit for bit), the fused kernel that also
+stores Z, and the sweep with a separate X'Z kernel re-reading Z (FP32 FMA,
+and BF16 hi+lo on MFMA). Prints one JSON line with the dominant kernel's
+roofline (its own algorithmic bytes against 8 TB/s, plus its VALU issue
+rate, the bound it actually meets), the sweep-level HBM fraction of the
+algorithmic sweep bytes N V (8 + 1/8) + V (24 P + 5), a measured copy-kernel
+bandwidth, and the reference CPU path timed on this host for a bounded sample.
 """
 import argparse
 import json
@@ -45,7 +48,7 @@ def parse():
     ap.add_argument("--batched-burnin", type=int, default=1000)
     ap.add_argument("--uneven-steps", type=int, default=50,
                     help="batch sweeps of the 12-analysis uneven batch (0 skips it)")
-    ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "k_latent_vl_mfm_c2.json"),
+    ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "k_overlap_c2.json"),
                     help="measured traffic and VALU counts of the dominant kernel (rocprofv3 summary)")
     return ap.parse_args()
 
@@ -94,16 +97,19 @@ def cpu_baseline(d, nsweeps):
             "latent_draws_per_s": N * V * nsweeps / dt}
 
 
-def time_mode(L, d, mode, zstore, steps, warmup):
+def time_mode(L, d, mode, zstore, steps, warmup, overlap=True):
     """Timed region: `steps` direct sweeps (bsg_sweep: the CAR-precision draw
     overlaps the next latent draw), bracketed by stream synchronisations. Then,
     in the same run, `steps` more sweeps with HIP events around every latent
     launch on the stream it runs on (kernel duration and sweep span; the
     event packets cost a few microseconds per sweep, so they stay out of the
     timed region), and a short serial pass with events around every kernel
-    for the breakdown."""
+    for the breakdown. Overlapped sweeps (where the form allows them and
+    overlap is True): the events bracket each sweep's two launches."""
     from bsglmm_amd import Sampler
     S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15, xtz_mode=mode, z_store=zstore)
+    S.overlap = overlap
+    ran_overlapped = S.overlap
     S.sweep(warmup)
     S.synchronize()
     t0 = time.perf_counter()
@@ -116,7 +122,7 @@ def time_mode(L, d, mode, zstore, steps, warmup):
     kms = S.sweep_profiled(n_prof)
     S.close()
     return {"wall": wall, "lat_ms": lat_ms / steps, "span_ms": span_ms / steps,
-            "kms": {k: v / n_prof for k, v in kms.items()}}
+            "kms": {k: v / n_prof for k, v in kms.items()}, "overlapped": ran_overlapped}
 
 
 def chain_summaries(L, d, chains, iters, burnin, nbatches=40):
@@ -305,27 +311,33 @@ def main():
     V = L.totvox
     copy_gbs, copy_ms = copy_bandwidth(1 << 30, 20)
 
-    head = time_mode(L, d, 2, 2, a.steps, a.warmup)  # fused X'Z, Z not kept
-    forms = {"fused_xtz_z_stored": (2, 0), "separate_xtz_fp32": (0, 0), "separate_xtz_mfma_bf16x2": (1, 0)}
-    others = {k: time_mode(L, d, m, z, a.steps, a.warmup) for k, (m, z) in forms.items()}
+    head = time_mode(L, d, 2, 2, a.steps, a.warmup)  # fused X'Z, Z not kept, overlapped
+    if not head["overlapped"]:
+        raise RuntimeError("the headline workload must run overlapped")
+    forms = {"z_free_serial": (2, 2, False), "fused_xtz_z_stored": (2, 0, True), "separate_xtz_fp32": (0, 0, True),
+             "separate_xtz_mfma_bf16x2": (1, 0, True)}
+    others = {k: time_mode(L, d, m, z, a.steps, a.warmup, o) for k, (m, z, o) in forms.items()}
 
     value = a.steps / head["wall"]
     sweep_bytes = N * V * (8 + 1 / 8) + V * (24 * P + 5)  # algorithmic bytes of one sweep
-    # the Z-free latent kernel moves y bits, b_j, WM_j in and m_j out: N V / 8 + 8 P V + 4 V
+    # the Z-free latent draws move y bits, b_j, WM_j in and m_j out: N V / 8 + 8 P V + 4 V
     vl_bytes = N * V / 8 + 8 * P * V + 4 * V
-    lat_s = head["lat_ms"] / 1e3
-    achieved = vl_bytes / lat_s / 1e9
+    # the two colour updates read m_j, the neighbours' b, the neighbour table
+    # and WM_j and write b_j: 12 P V + 28 V; one sweep is two k_overlap launches
+    ovl_bytes = (vl_bytes + 12 * P * V + 28 * V) / 2
+    launch_s = head["lat_ms"] / 2 / 1e3  # the events bracket both launches of a sweep
+    achieved = ovl_bytes / launch_s / 1e9
     prof = load_profile(a.profile)
     traffic = prof.get("bytes_per_launch")
     valu = None
     if prof.get("valu_wave_instructions_per_launch"):
         w = prof["valu_wave_instructions_per_launch"]
         # wave64 VALU issue takes 4 cycles on a 16-lane SIMD; 1024 SIMDs at 2.4 GHz
-        valu = {"wave_instructions_per_launch": w, "per_draw": w * 64 / (N * V),
-                "frac": w * 4 / (1024 * 2.4e9 * lat_s), "source": prof.get("source")}
+        valu = {"wave_instructions_per_launch": w, "per_draw": w * 2 * 64 / (N * V),
+                "frac": w * 4 / (1024 * 2.4e9 * launch_s), "source": prof.get("source")}
 
     def form_line(r, z_written, fused):
-        lat_b = N * V * (4 + 1 / 8) + 4 * P * V + (4 * P * V if fused else 0)
+        lat_b = vl_bytes if not z_written else N * V * (4 + 1 / 8) + 4 * P * V + (4 * P * V if fused else 0)
         line = {"value": a.steps / r["wall"], "unit": "sweeps/s", "ms_per_step": 1e3 * r["wall"] / a.steps,
                 "latent_kernel_ms": r["lat_ms"], "latent_gbs": lat_b / (r["lat_ms"] / 1e3) / 1e9,
                 "kernel_ms_serial_breakdown": r["kms"],
@@ -339,21 +351,23 @@ def main():
         "warmup": a.warmup, "ms_per_step": 1e3 * head["wall"] / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{a.config}: grid {nx}x{ny}x{nz} ellipsoidal-shell mask, plain single chain, "
-                               f"FP32 state, X'Z on FP32 FMA fused into the latent kernel (means x'b on FP32 MFMA), Z not kept",
+                               f"FP32 state, X'Z on FP32 FMA fused into the latent draws (means x'b on FP32 MFMA), "
+                               f"Z not kept, overlapped sweep (two launches per sweep)",
                    "V": V, "N": N, "P": P, "chains": 1},
         "latent_draws_per_s": value * N * V,
-        "latent_kernel_ms": head["lat_ms"],
+        "sweep_kernel_ms": head["lat_ms"],
+        "k_overlap_avg_launch_ms": head["lat_ms"] / 2,
         "kernel_ms_serial_breakdown": head["kms"],
         "sweep_gpu_ms_with_latent_events": head["span_ms"],
         "sweep_roofline": {"bound": "hbm", "algorithmic_bytes": sweep_bytes,
                            "achieved": sweep_bytes * value / 1e9, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": sweep_bytes * value / 1e9 / HBM_PEAK_GBS},
         "copy_kernel_gbs": copy_gbs,
-        "roofline": {"kernel": "k_latent_vl_mfm (latent draw + X'Z, Z not kept)", "bound": "hbm",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"kernel": "k_overlap (one colour's latent draws + X'Z beside the other colour's update)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": vl_bytes, "valu_issue": valu},
-        "forms": {k: form_line(r, True, forms[k][0] == 2) for k, r in others.items()},
+                     "algorithmic_bytes_per_launch": ovl_bytes, "valu_issue": valu},
+        "forms": {k: form_line(r, forms[k][1] != 2, forms[k][0] == 2) for k, r in others.items()},
         "cpu_baseline": None,
     }
     if a.batched_chains > 0:

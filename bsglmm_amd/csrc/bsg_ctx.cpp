@@ -353,9 +353,11 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
 
   // colour tiles of the Z-free latent launch at four lanes per voxel: each
   // colour's leftover 64-voxel tiles (T mod CUs) run as K pieces each when
-  // that spreads them over the chip (k_latent_vl_mfm). One chain only: with
-  // several, chains are the grid's second dimension, so one chain's pieces
-  // would be dispatched mid-launch, where they balance nothing
+  // that spreads them over the chip (k_latent_vl_mfm); K <= 8 (more, smaller
+  // pieces measured no faster overlapped and slower serial: each piece pays a
+  // block's set-up). One chain only: with several, chains are the grid's
+  // second dimension, so one chain's pieces would be dispatched mid-launch,
+  // where they balance nothing
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -369,7 +371,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     int mp = 0;
     for (int c = 0; c < 2; c++) {
       const int tiles = (L.n[c] + 63) / 64, tail = tiles % ncu;
-      const int K = (tail > 0 && C == 1) ? std::min(std::min(ncu / tail, R), 16) : 0;
+      const int K = (tail > 0 && C == 1) ? std::min(std::min(ncu / tail, R), 8) : 0;
       pb.tl_left[c] = K > 1 ? tail : 0;
       pb.tl_k[c] = K > 1 ? K : 1;
       if (K > 1) mp += tail * K * 64 * P;

@@ -107,38 +107,38 @@ __device__ void sum_partials(int nparts, const double *part, double *T)
 // Normal tail function and its inverse in FP32, branch free. Coefficients are
 // this project's own fits (bsglmm_amd/csrc/fit_normal_tail.py, kernel form):
 //   Q(t)    = r 2^(G(r) - t^2 log2(e)/2), r = 1/(1 + t/(2 sqrt 2)),
-//             G = log2(e) g with g of degree 9
-//   Qinv(p) = n(s)/d(s), s = sqrt(-log2 p), n, d of degree 4, p in (1e-38, 1/2]
-// FP32 errors: Q 3e-6 relative (t <= 8), Qinv 5e-7 relative (floor 1).
+//             G = log2(e) g with g of degree 7 (fitted for t <= 11.3; past it
+//             Q falls below 1e-29 and underflows to 0 by t = 14)
+//   Qinv(p) = n(s)/d(s), s = sqrt(-log2 p), n of degree 4, d of degree 3,
+//             p in (1e-38, 1/2]
+// FP32 errors: Q 6e-6 relative (t <= 8, from rounding the t^2 term), Qinv
+// 6e-7 relative (floor 1).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float q_tail(float t)
 {
   const float r = __builtin_amdgcn_rcpf(fmaf(0.35355339059327373f, t, 1.0f));
-  float g = 0.24683615565299988f;
-  g = fmaf(g, r, -1.1876296997070312f);
-  g = fmaf(g, r, 2.1505043506622314f);
-  g = fmaf(g, r, -1.6410677433013916f);
-  g = fmaf(g, r, 0.4044766128063202f);
-  g = fmaf(g, r, -0.26960375905036926f);
-  g = fmaf(g, r, 0.13982561230659485f);
-  g = fmaf(g, r, 0.5396761298179626f);
-  g = fmaf(g, r, 1.4427305459976196f);
-  g = fmaf(g, r, -2.8257482051849365f);
+  float g = -0.2613292634487152f;
+  g = fmaf(g, r, 1.0420771837234497f);
+  g = fmaf(g, r, -1.3670768737792969f);
+  g = fmaf(g, r, 0.42922380566596985f);
+  g = fmaf(g, r, -0.014621351845562458f);
+  g = fmaf(g, r, 0.5545806884765625f);
+  g = fmaf(g, r, 1.4430041313171387f);
+  g = fmaf(g, r, -2.825859308242798f);
   return r * __builtin_amdgcn_exp2f(fmaf(-0.72134752044448170f * t, t, g));
 }
 
 __device__ __forceinline__ float q_inv(float p)
 {
   const float s = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(p));
-  float n = 1.0409833192825317f;
-  n = fmaf(n, s, 6.929381370544434f);
-  n = fmaf(n, s, 4.551568984985352f);
-  n = fmaf(n, s, -9.31513786315918f);
-  n = fmaf(n, s, -3.2067947387695312f);
-  float d = 1.4280302821134683e-05f;
-  d = fmaf(d, s, 0.8832118511199951f);
-  d = fmaf(d, s, 5.923495769500732f);
-  d = fmaf(d, s, 6.432500839233398f);
+  float n = 0.686313807964325f;
+  n = fmaf(n, s, 5.580847263336182f);
+  n = fmaf(n, s, 4.647900104522705f);
+  n = fmaf(n, s, -7.771159648895264f);
+  n = fmaf(n, s, -3.143899917602539f);
+  float d = 0.5826886892318726f;
+  d = fmaf(d, s, 4.7574782371521f);
+  d = fmaf(d, s, 5.753643989562988f);
   d = fmaf(d, s, 1.0f);
   return n * __builtin_amdgcn_rcpf(d);
 }

@@ -113,7 +113,9 @@ int bsg_update_coef(bsg_ctx *ctx, int colour);
 /* updateSpatCoefPrecGPU (covarGPU.cu:1063-1112): FP64 pair reduction and a
  * device Wishart draw; advances the sweep index */
 int bsg_update_car_prec(bsg_ctx *ctx);
-/* waits for the context's stream; reports kernel-side numerical failures */
+/* waits for the context's stream; reports kernel-side numerical failures
+ * (BSG_ERR_NUM), and BSG_ERR_HIP if an overlapped sweep's colour-0 blocks
+ * waited past their bound for the CAR precision drawn beside them */
 int bsg_synchronize(bsg_ctx *ctx);
 /* bsg_sweep with HIP events around every latent kernel: latent_ms receives
  * their summed durations, total_ms (optional) the whole span. Synchronous.

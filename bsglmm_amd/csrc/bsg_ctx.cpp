@@ -607,9 +607,9 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
   return BSG_OK;
 }
 
-// Full sweeps run overlapped (k_overlap) when the context allows it: one
-// chain, Z-free inversion draws at four lanes per voxel with colour tiles, b
-// fixed, both colours present and the Wishart draw small enough to carry.
+// Full sweeps run overlapped (k_overlap) when the context allows it: Z-free
+// inversion draws at four lanes per voxel, b fixed, both colours present and
+// the Wishart draw small enough to carry (any number of chains).
 static bool use_overlap(const bsg_ctx *ctx)
 {
   return ctx->overlap && ctx->opts.z_store == 2 && ctx->opts.xtz_mode == 2 && bsg::overlap_supported(ctx->pb);

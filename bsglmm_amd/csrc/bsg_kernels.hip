@@ -1619,8 +1619,8 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// k_overlap: the sweep of one chain as two launches, each colour's update
-// beside the other colour's latent draws. Z_j and X'Z_j depend on b_j alone,
+// k_overlap: the sweep of each chain (the grid's second dimension) as two
+// launches, each colour's update beside the other colour's latent draws. Z_j and X'Z_j depend on b_j alone,
 // and each colour's update reads only the other colour's coefficients, so
 //   phase 0: [block 0: Lambda of sweep wsweep when wish] colour 1's latent
 //            draws of `sweep` (nlat blocks), colour 0's update of `sweep`;
@@ -1636,38 +1636,51 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
 // (lam_wait), with no fence. Every variate and sum is the one the serial
 // sweep forms with colour tiles.
 // ---------------------------------------------------------------------------
-template <int P>
+// MC: several chains (whole tiles); else one chain (colour tiles and pieces),
+// compiled apart so that each keeps its own registers
+template <int P, bool MC>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, int phase, uint32_t sweep, int nlat,
                                                                int wish, uint32_t wsweep, int tail, Accum ac,
                                                                int batch)
 {
+  constexpr int NE = P * (P + 1) / 2;
   const uint32_t base = *pb.sweep_base;
+  const int chain = MC ? (int)blockIdx.y : 0;
   int b = blockIdx.x;
   if (phase == 0 && wish) {
     if (b == 0) {
-      wishart_block<P>(pb, 0, (pb.V - pb.n0 + 255) / 256, pb.partials, wsweep + base, true, pb.wbart);
+      wishart_block<P>(pb, chain, (pb.V - pb.n0 + 255) / 256, pb.partials + (size_t)chain * pb.max_parts * NE,
+                       wsweep + base, true, pb.wbart + (size_t)chain * P * P);
       return;
     }
     b--;
   }
   if (b < nlat) {
-    vl_ctile_block<P>(pb, sweep + base + (uint32_t)phase, 0, b, phase ? 1 : 2);
+    // one chain: colour tiles and pieces; several: one whole tile per block
+    const int c = phase ? 0 : 1;
+    if (!MC) {
+      vl_ctile_block<P>(pb, sweep + base + (uint32_t)phase, chain, b, c ? 2 : 1);
+    } else {
+      const VlRange r = vl_range(pb, c);
+      latent_vl_mfm_body<P, false>(pb, sweep + base + (uint32_t)phase, chain, r.base, r.end, b, 1 << 30, 0);
+    }
     return;
   }
   b -= nlat;
   if (phase == 0) {
-    coef_body<P, false>(pb, 0, sweep + base, 0, b, tail, ac, batch, wish ? pb.lamtag : nullptr);
+    coef_body<P, false>(pb, 0, sweep + base, chain, b, tail, ac, batch,
+                        wish ? pb.lamtag + (size_t)chain * P * P : nullptr);
     return;
   }
   const int ncb = (pb.V - pb.n0 + 255) / 256;
   if (b < ncb) {
-    coef_body<P, true, false>(pb, 1, sweep + base, 0, b, tail, ac, batch);
+    coef_body<P, true, false>(pb, 1, sweep + base, chain, b, tail, ac, batch);
     return;
   }
-  // the last block: Bartlett factor of this sweep's Lambda, tags cleared
+  // the last block of the chain: Bartlett factor of this sweep's Lambda, tags cleared
   if (threadIdx.x < P * P) {
-    pb.lamtag[threadIdx.x] = 0ull;
-    pb.wbart[threadIdx.x] = bartlett_entry<P>(pb, 0, threadIdx.x, sweep + base);
+    pb.lamtag[(size_t)chain * P * P + threadIdx.x] = 0ull;
+    pb.wbart[(size_t)chain * P * P + threadIdx.x] = bartlett_entry<P>(pb, chain, threadIdx.x, sweep + base);
   }
 }
 
@@ -2100,13 +2113,15 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
   return hipGetLastError();
 }
 
-// The overlapped sweep: one chain, colour tiles, Lambda drawn inside phase 0
-// (P <= 6, as latent_carries_wishart), both colours present. Phase 1 draws
-// the Bartlett factor of its sweep's Lambda ahead, for the phase 0 after it.
+// The overlapped sweep: Z-free inversion draws at four lanes per voxel, b
+// fixed, Lambda drawn inside phase 0 (P <= 6, as latent_carries_wishart), both
+// colours present; any number of chains (the grid's second dimension; one
+// chain runs colour tiles and pieces). Phase 1 draws the Bartlett factor of
+// its sweep's Lambda ahead, for the phase 0 after it.
 bool overlap_supported(const DevProblem &pb)
 {
-  return pb.nchains == 1 && vl_ctiles(pb) && pb.n0 > 0 && pb.V > pb.n0 && latent_carries_wishart(pb) &&
-         pb.lamtag != nullptr;
+  return pb.zfree && pb.latent_method == 0 && !pb.wm_update && vl_groups(pb) == 4 && pb.n0 > 0 && pb.V > pb.n0 &&
+         latent_carries_wishart(pb) && pb.lamtag != nullptr;
 }
 
 hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int lat, int wish, uint32_t wsweep,
@@ -2115,13 +2130,20 @@ hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int l
   if (!overlap_supported(pb) || (phase != 0 && phase != 1) || (phase == 1 && wish)) return hipErrorInvalidValue;
   const Accum acc = ac ? *ac : Accum{};
   const int colour = phase ? 1 : 0;  // the updated colour; the other one is drawn
-  const int nlat = (phase == 0 || lat) ? vl_blocks(pb, phase ? 1 : 2) : 0;
+  const VlRange rd = vl_range(pb, phase ? 0 : 1);  // the drawn colour
+  const int nlat = (phase == 0 || lat) ? (vl_ctiles(pb) ? vl_blocks(pb, phase ? 1 : 2) : (rd.end - rd.base + 63) / 64) : 0;
   const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
   const size_t lds = latent_vl_lds(pb);
-  dim3 grid((unsigned)(wish + nlat + ncb + (phase ? 1 : 0)));
-#define L_(PP)                                                                                                  \
-  hipLaunchKernelGGL((k_overlap<PP>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep, tail, acc, \
-                     batch)
+  dim3 grid((unsigned)(wish + nlat + ncb + (phase ? 1 : 0)), (unsigned)pb.nchains);
+#define L_(PP)                                                                                                     \
+  do {                                                                                                             \
+    if (vl_ctiles(pb))                                                                                             \
+      hipLaunchKernelGGL((k_overlap<PP, false>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep,   \
+                         tail, acc, batch);                                                                        \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_overlap<PP, true>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep,    \
+                         tail, acc, batch);                                                                        \
+  } while (0)
   switch (pb.P) {
     case 1: L_(1); break;
     case 2: L_(2); break;

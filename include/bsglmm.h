@@ -189,11 +189,11 @@ int bsg_set_lane_split(bsg_ctx *ctx, int g);
 int bsg_get_lane_split(bsg_ctx *ctx, int *g); /* the split the next sweep uses */
 /* Overlapped sweeps (on by default; no counterpart in the reference, whose
  * sweep runs its kernels one after another, mcmc.cpp:311-394): full sweeps
- * of a one-chain context whose latent kernel runs at four lanes per voxel
- * (inversion draws, no WM image, P <= 6) run as two launches per sweep, each
- * colour's coefficient update beside the other colour's latent draws. Same
- * variates, same sums, same chain as with on = 0. bsg_get_overlap tells
- * whether the next bsg_sweep* call runs overlapped. */
+ * of a context whose latent kernel runs at four lanes per voxel (inversion
+ * draws, no WM image, P <= 6; any number of chains) run as two launches per
+ * sweep, each colour's coefficient update beside the other colour's latent
+ * draws. Same variates, same sums, same chains as with on = 0.
+ * bsg_get_overlap tells whether the next bsg_sweep* call runs overlapped. */
 int bsg_set_overlap(bsg_ctx *ctx, int on);
 int bsg_get_overlap(bsg_ctx *ctx, int *on);
 typedef struct bsg_batch bsg_batch;

@@ -51,14 +51,15 @@ def pair(name, nsubs, ncovar, g=0, nchains=1, wm=False):
     return L, out[0], out[1]
 
 
-def assert_same_state(A, B):
+def assert_same_state(A, B, nchains=1):
     A.synchronize()
     B.synchronize()
-    assert A.sweep_index() == B.sweep_index()
-    assert np.array_equal(A.get_spat_coef(), B.get_spat_coef())
-    assert np.array_equal(A.get_xtz(), B.get_xtz())
-    assert np.array_equal(A.get_prec(), B.get_prec())
-    assert np.array_equal(A.get_pair_sums(), B.get_pair_sums())
+    for c in range(nchains):
+        assert A.sweep_index(c) == B.sweep_index(c)
+        assert np.array_equal(A.get_spat_coef(c), B.get_spat_coef(c))
+        assert np.array_equal(A.get_xtz(c), B.get_xtz(c))
+        assert np.array_equal(A.get_prec(c), B.get_prec(c))
+        assert np.array_equal(A.get_pair_sums(c), B.get_pair_sums(c))
 
 
 # (N, P, lane split): P = 1..6 (means on FMA for P <= 2, one MFMA for P = 3..4,
@@ -98,11 +99,33 @@ def test_overlapped_accumulation_and_graph_equal_serial():
     assert_same_state(A, B)
 
 
+# several chains: the grid's second dimension, whole tiles (no pieces), one
+# Lambda draw and wait per chain; C2 with 3 chains at full size
+@pytest.mark.parametrize("name,nsubs,ncovar,nchains", [("c1", 300, 4, 2), ("c1", 200, 6, 3), ("c1", 90, 2, 2),
+                                                       ("c2", None, None, 3)])
+def test_overlapped_sweep_equals_serial_several_chains(name, nsubs, ncovar, nchains):
+    print(f"case overlap vs serial {name} N={nsubs} P={ncovar} chains={nchains}", flush=True)
+    L, A, B = pair(name, nsubs, ncovar, nchains=nchains)
+    assert A.overlap and not B.overlap
+    for S in (A, B):
+        S.accum_reset(ntypes=1, nbatches=2, batch_len=2)
+    A.sweep_accumulate(1, save_every=1)
+    A.sweep_accumulate(3, save_every=1)
+    B.sweep_accumulate(4, save_every=1)
+    assert_same_state(A, B, nchains)
+    for c in range(nchains):
+        ra, rb = A.accum_get(c), B.accum_get(c)
+        for k in ra:
+            assert np.array_equal(np.asarray(ra[k]), np.asarray(rb[k]), equal_nan=True), (c, k)
+    # the chains are distinct streams
+    assert not np.array_equal(A.get_spat_coef(0), A.get_spat_coef(1))
+
+
 def test_overlap_off_where_unsupported():
-    """Several chains, seven covariates, a WM image or a lane split other than
-    four run the serial order."""
+    """Seven covariates, a WM image or a lane split other than four run the
+    serial order."""
     print("case overlap unsupported forms", flush=True)
-    for kw in (dict(nchains=2), dict(wm=True)):
+    for kw in (dict(wm=True),):
         L, A, B = pair("c1", 300, 4, **kw)
         assert not A.overlap
     L, A, B = pair("c1", 300, 7)

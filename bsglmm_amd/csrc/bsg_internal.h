@@ -169,13 +169,14 @@ hipError_t launch_latent(const DevProblem &pb, bool fused_xtz, int zstore, uint3
                          int v0, int v1, int *bstat_parts = nullptr, int wish = 0, uint32_t wsweep = 0,
                          int *pieces = nullptr);
 bool latent_carries_wishart(const DevProblem &pb);
-// Overlapped sweep (one chain, colour tiled, Lambda drawn inside the launch):
+// Overlapped sweep (every chain; one chain runs colour tiles and pieces;
+// Lambda drawn inside the launch):
 // phase 0: [Wishart of sweep wsweep when wish] colour 1's latent draws of
 //          `sweep`, then colour 0's update of `sweep` (reading the new Lambda
-//          by its tags when wish);
-// phase 1: the Bartlett factor of `sweep`'s Lambda (for the next phase 0),
-//          colour 0's latent draws of sweep + 1 (when lat), then colour 1's
-//          update of `sweep` (pair partials).
+//          by its tags when wish); `lat` is ignored;
+// phase 1: colour 0's latent draws of sweep + 1 (when lat), colour 1's
+//          update of `sweep` (pair partials), then the Bartlett factor of
+//          `sweep`'s Lambda for the next phase 0.
 // tail: colour set whose leftover pieces the update adds (its own colour).
 bool overlap_supported(const DevProblem &pb);
 hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int lat, int wish, uint32_t wsweep,

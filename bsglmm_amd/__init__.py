@@ -98,6 +98,7 @@ def lib():
         "bsg_set_lane_split": (C.c_int, [vp, C.c_int]),
         "bsg_get_lane_split": (C.c_int, [vp, pi]),
         "bsg_set_overlap": (C.c_int, [vp, C.c_int]),
+        "bsg_test_overlap_wait": (C.c_int, [vp, C.c_int, C.c_int]),
         "bsg_get_overlap": (C.c_int, [vp, pi]),
         "bsg_batch_create": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
         "bsg_batch_destroy": (C.c_int, [vp]),

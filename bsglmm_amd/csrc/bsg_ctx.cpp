@@ -903,6 +903,9 @@ int bsg_synchronize(bsg_ctx *ctx)
     return BSG_ERR_NUM;
   }
   if (flags[2]) {
+    // reported once: the next sweeps start with the flag clear
+    const uint32_t zero = 0;
+    HIPCHK(hipMemcpy(ctx->pb.flags + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
     set_error("overlapped sweep: a colour-0 block waited past its bound for the CAR precision drawn in the "
               "same launch");
     return BSG_ERR_HIP;
@@ -1228,6 +1231,18 @@ int bsg_get_lane_split(bsg_ctx *ctx, int *g)
     return BSG_ERR_ARG;
   }
   *g = ctx->pb.lat_g > 0 ? ctx->pb.lat_g : bsg::latent_vl_groups(ctx->pb);
+  return BSG_OK;
+}
+
+int bsg_test_overlap_wait(bsg_ctx *ctx, int spin_limit, int tag)
+{
+  CTXCHK(ctx);
+  if (spin_limit < 0 || tag < 0) {
+    set_error("bsg_test_overlap_wait: negative argument");
+    return BSG_ERR_ARG;
+  }
+  ctx->pb.lam_spin = spin_limit;
+  ctx->pb.lam_tag = tag;
   return BSG_OK;
 }
 

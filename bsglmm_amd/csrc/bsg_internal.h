@@ -88,6 +88,10 @@ struct DevProblem {
   // carries its tag); cleared by the launch before
   unsigned long long *lamtag;  // [chain][P][P]
   double *wbart;               // [chain][P][P] Bartlett factor of the next overlapped Lambda draw
+  // bounded wait for the tagged Lambda: polls before giving up (0: 2^20), and
+  // the tag the draw writes (0: 1, the one the readers wait for). Set from the
+  // host only to test the bound (bsg_test_overlap_wait).
+  int lam_spin, lam_tag;
 };
 
 // Lanes per voxel of the Z-free latent kernels: enough lanes per voxel to give

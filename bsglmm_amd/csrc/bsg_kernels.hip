@@ -1222,8 +1222,9 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
       lam[i * P + j] = (float)v;
       if (tagged)
         __hip_atomic_store(pb.lamtag + (size_t)chain * P * P + i * P + j,
-                           (1ull << 32) | (unsigned long long)__float_as_uint((float)v), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+                           ((unsigned long long)(pb.lam_tag ? pb.lam_tag : 1) << 32) |
+                               (unsigned long long)__float_as_uint((float)v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1419,19 +1420,21 @@ __device__ __forceinline__ CoefOut<P> coef_fp64(CoefIn<P> in, const float *lam, 
 // register budget is shared with the latent draws.
 // The CAR precision drawn by another block of the same launch (k_overlap):
 // its entries carry tag 1 beside their bits in lamtag, so the block copies
-// them to LDS once every entry shows the tag. The wait is bounded; past the
-// bound flags[2] is set (the host reports it) and the block goes on.
+// them to LDS once every entry shows the tag. The wait is bounded (2^20
+// polls, or pb.lam_spin); past the bound flags[2] is set (the host reports
+// it) and the block goes on.
 constexpr int LAM_SPIN_MAX = 1 << 20;
 template <int P>
 __device__ __forceinline__ const float *lam_wait(const DevProblem &pb, const unsigned long long *lamw)
 {
   __shared__ float lam_s[P * P];
   if (threadIdx.x < P * P) {
+    const int spin = pb.lam_spin > 0 ? pb.lam_spin : LAM_SPIN_MAX;
     unsigned long long v = 0ull;
     for (int it = 0;; it++) {
       v = __hip_atomic_load(lamw + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if ((v >> 32) == 1ull) break;
-      if (it >= LAM_SPIN_MAX) {
+      if (it >= spin) {
         atomicOr(&pb.flags[2], 1u);
         break;
       }

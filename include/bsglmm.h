@@ -196,6 +196,12 @@ int bsg_get_lane_split(bsg_ctx *ctx, int *g); /* the split the next sweep uses *
  * bsg_get_overlap tells whether the next bsg_sweep* call runs overlapped. */
 int bsg_set_overlap(bsg_ctx *ctx, int on);
 int bsg_get_overlap(bsg_ctx *ctx, int *on);
+/* Test hook of the overlapped sweep's bounded wait (no counterpart in the
+ * reference): the colour-0 blocks give up after spin_limit polls (0: the
+ * default 2^20) and the CAR-precision draw writes `tag` beside its entries
+ * (0: the default 1, the only one the readers accept). With another tag every
+ * wait runs out, the kernels still finish, and bsg_synchronize reports it. */
+int bsg_test_overlap_wait(bsg_ctx *ctx, int spin_limit, int tag);
 typedef struct bsg_batch bsg_batch;
 /* Groups up to 32 contexts (independent analyses: own masks, N, P and chain
  * counts) so that each phase of a sweep is one kernel launch over all of

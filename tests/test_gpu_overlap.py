@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from bsglmm_amd import Layout, Sampler, synth
+from bsglmm_amd import Layout, Sampler, lib, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -119,6 +119,24 @@ def test_overlapped_sweep_equals_serial_several_chains(name, nsubs, ncovar, ncha
             assert np.array_equal(np.asarray(ra[k]), np.asarray(rb[k]), equal_nan=True), (c, k)
     # the chains are distinct streams
     assert not np.array_equal(A.get_spat_coef(0), A.get_spat_coef(1))
+
+
+def test_bounded_lambda_wait_reports_and_recovers():
+    """The colour-0 blocks' wait for the Lambda drawn beside them is bounded.
+    Forced from the host (bsg_test_overlap_wait): the draw writes a tag the
+    blocks never accept, every wait gives up after 8 polls, the kernels
+    finish, and bsg_synchronize reports it once; with the hook cleared the
+    context sweeps on."""
+    print("case overlap bounded wait forced from the host", flush=True)
+    L, A, B = pair("c1", 300, 4)
+    assert lib().bsg_test_overlap_wait(A._h, 8, 2) == 0
+    A.sweep(3)  # phases 0 of the second and third sweeps wait
+    with pytest.raises(Exception, match="past its bound"):
+        A.synchronize()
+    assert lib().bsg_test_overlap_wait(A._h, 0, 0) == 0
+    A.sweep(2)
+    A.synchronize()
+    assert np.all(np.isfinite(A.get_spat_coef()))
 
 
 def test_overlap_off_where_unsupported():

@@ -67,13 +67,16 @@ def test_accumulator_matches_host_snapshots():
     assert S.accum_get(0)["ndraws"] == 6
 
 
-@pytest.mark.parametrize("z_store,method", [(0, 0), (2, 1)])
-def test_posterior_means_match_reference_cpu_chain(z_store, method):
+@pytest.mark.parametrize("z_store,method,g", [(0, 0, 0), (2, 1, 0), (2, 0, 4)])
+def test_posterior_means_match_reference_cpu_chain(z_store, method, g):
     """z_store 0: every draw by inversion (k_latent); z_store 2 with method 1:
-    Box-Muller candidates with the exact fallback (k_latent_vl)."""
+    Box-Muller candidates with the exact fallback (k_latent_vl); z_store 2
+    with method 0 at four lanes per voxel: the overlapped sweep (k_overlap,
+    colour tiles and pieces, the Lambda draw inside its first launch)."""
     f = np.load(os.path.join(GOLD, "ref_chain_c1.npz"))
     burnin, draws, nb = int(f["burnin"]), int(f["draws"]), int(f["batches"])
-    print(f"case c1 chain {burnin}+{draws} sweeps vs reference CPU chain, z_store={z_store}", flush=True)
+    print(f"case c1 chain {burnin}+{draws} sweeps vs reference CPU chain, z_store={z_store} method={method} G={g}",
+          flush=True)
     d = synth.generate("c1")
     nx, ny, nz = d["grid"]
     L = Layout(d["mask"], nx, ny, nz)
@@ -81,6 +84,9 @@ def test_posterior_means_match_reference_cpu_chain(z_store, method):
     # the reference CPU path's semantics: df = TOTVOX + P, truncation at 0
     S = Sampler(L, d["covar"], d["data"], seed=0xC0FFEE, xtz_mode=2, z_store=z_store, latent_method=method,
                 df_offset=int(f["df_offset"]), trunc_eps=float(f["trunc_eps"]))
+    if g:
+        S.lane_split = g
+        assert S.overlap
     S.sweep(burnin)
     S.accum_reset(ntypes=1, nbatches=nb, batch_len=draws // nb)
     S.sweep_accumulate(draws, 1)

@@ -1623,21 +1623,21 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
 
 // ---------------------------------------------------------------------------
 // k_overlap: the sweep of each chain (the grid's second dimension) as two
-// launches, each colour's update beside the other colour's latent draws. Z_j and X'Z_j depend on b_j alone,
-// and each colour's update reads only the other colour's coefficients, so
+// launches, each colour's update beside the other colour's latent draws.
+// Z_j and X'Z_j depend on b_j alone, and each colour's update reads only the
+// other colour's coefficients, so
 //   phase 0: [block 0: Lambda of sweep wsweep when wish] colour 1's latent
 //            draws of `sweep` (nlat blocks), colour 0's update of `sweep`;
 //   phase 1: colour 0's latent draws of sweep + 1 (nlat blocks, 0 after the
 //            last sweep), colour 1's update of `sweep` with its pair
 //            partials, [last block: the Bartlett factor of this sweep's
 //            Lambda, drawn ahead for the next phase 0; the Lambda tags cleared];
-// the kernel boundaries order everything else. Latent blocks come before
-// the update blocks so that they all start at once (and keep their block
-// index, hence their XCD, from sweep to sweep); the update blocks take the
-// slots that the short leftover pieces free. Colour 0's update needs the
-// Lambda that block 0 of the same launch draws: it reads it by its tags
-// (lam_wait), with no fence. Every variate and sum is the one the serial
-// sweep forms with colour tiles.
+// the kernel boundaries order everything else. The latent draws are the
+// critical path, so their blocks come first and all start at once; the
+// update blocks take the slots that the short leftover pieces and the first
+// finished tiles free. Colour 0's update needs the Lambda that block 0 of
+// the same launch draws: it reads it by its tags (lam_wait), with no fence.
+// Every variate and sum is the one the serial sweep forms.
 // ---------------------------------------------------------------------------
 // MC: several chains (whole tiles); else one chain (colour tiles and pieces),
 // compiled apart so that each keeps its own registers

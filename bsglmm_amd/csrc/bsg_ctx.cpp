@@ -897,18 +897,20 @@ int bsg_synchronize(bsg_ctx *ctx)
   HIPCHK(hipStreamSynchronize(ctx->stream2));
   uint32_t flags[4];
   HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
-  if (flags[1]) {
-    set_error("coefficient update: " + std::to_string(flags[1]) +
-              " voxel(s) had a non positive definite posterior precision");
-    return BSG_ERR_NUM;
-  }
   if (flags[2]) {
-    // reported once: the next sweeps start with the flag clear
+    // first: a wait that gave up leaves colour 0 with an incomplete precision,
+    // which can also show as the failure below. Reported once: the next
+    // sweeps start with the flag clear
     const uint32_t zero = 0;
     HIPCHK(hipMemcpy(ctx->pb.flags + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
     set_error("overlapped sweep: a colour-0 block waited past its bound for the CAR precision drawn in the "
               "same launch");
     return BSG_ERR_HIP;
+  }
+  if (flags[1]) {
+    set_error("coefficient update: " + std::to_string(flags[1]) +
+              " voxel(s) had a non positive definite posterior precision");
+    return BSG_ERR_NUM;
   }
   return BSG_OK;
 }

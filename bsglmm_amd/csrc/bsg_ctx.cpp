@@ -899,10 +899,10 @@ int bsg_synchronize(bsg_ctx *ctx)
   HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[2]) {
     // first: a wait that gave up leaves colour 0 with an incomplete precision,
-    // which can also show as the failure below. Reported once: the next
-    // sweeps start with the flag clear
-    const uint32_t zero = 0;
-    HIPCHK(hipMemcpy(ctx->pb.flags + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    // which can also show as the failure below; both are reported here, once,
+    // and the next sweeps start with the flags clear
+    const uint32_t zero[2] = {0, 0};
+    HIPCHK(hipMemcpy(ctx->pb.flags + 1, zero, sizeof(zero), hipMemcpyHostToDevice));
     set_error("overlapped sweep: a colour-0 block waited past its bound for the CAR precision drawn in the "
               "same launch");
     return BSG_ERR_HIP;

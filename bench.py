@@ -209,10 +209,10 @@ def uneven_batch(steps):
         L = Layout(d["mask"], *d["grid"])
         S = Sampler(L, d["covar"], d["data"], seed=0xC5C5 + k, nchains=4, xtz_mode=2, z_store=2)
         S.sweep(3)
-        g, ms = S.autotune(reps=3)
+        g, ms = S.autotune(reps=10)
         P, N = d["covar"].shape
         rows.append({"analysis": k, "V": L.totvox, "N": N, "P": P, "chains": 4, "lane_split_G": g,
-                     "latent_ms_by_G": {str(G): (None if m < 0 else round(float(m), 4)) for G, m in zip((1, 2, 4, 8), ms)}})
+                     "sweep_ms_by_G": {str(G): (None if m < 0 else round(float(m), 4)) for G, m in zip((1, 2, 4, 8), ms)}})
         samplers.append(S)
         del d
     draws = sum(4 * r["N"] * r["V"] for r in rows)
@@ -231,8 +231,8 @@ def uneven_batch(steps):
             "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
             "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat,
             "concurrent_streams_draws_per_s": draws * steps / conc, "batched_over_concurrent_streams": conc / bat,
-            "sum_of_tuned_latent_ms": sum(min(v for v in r["latent_ms_by_G"].values() if v is not None)
-                                          for r in rows),
+            "sum_of_tuned_sweep_ms": sum(min(v for v in r["sweep_ms_by_G"].values() if v is not None)
+                                         for r in rows),
             "batch_sweep_ms": 1e3 * bat / steps,
             "form": "FP32, Z-free fused X'Z (the BF16-split MFMA X'Z needs Z in HBM and is slower, "
                     "see forms.separate_xtz_mfma_bf16x2)"}

@@ -280,8 +280,9 @@ class Sampler:
         m = np.ascontiguousarray(m, dtype=np.float32)
         _check(lib().bsg_set_xtz(self._h, chain, _ptr(m, C.c_float)))
 
-    def autotune(self, reps=3):
-        """Lane split G of the Z-free latent kernel chosen by timing; (G, ms for G = 1, 2, 4, 8)."""
+    def autotune(self, reps=10):
+        """Lane split G of the Z-free sweep chosen by timing whole sweeps (the
+        chain state is restored); (G, ms per sweep for G = 1, 2, 4, 8)."""
         g = C.c_int()
         ms = np.empty(4, np.float64)
         _check(lib().bsg_autotune(self._h, int(reps), C.byref(g), _ptr(ms, C.c_double)))

@@ -1267,57 +1267,89 @@ int bsg_get_overlap(bsg_ctx *ctx, int *on)
 }
 
 // ---- autotuning of the Z-free latent kernel ----------------------------
+// Whole sweeps at each lane split, overlapped where the split allows it:
+// the split also decides whether the latent launch can carry the Wishart
+// draw and whether the sweep overlaps, which can matter more than the latent
+// kernel's own time. Every candidate starts from the same saved state, and
+// the state (coefficients, X'Z and its pieces, Lambda, pair sums, b, flags,
+// sweep index) is put back afterwards, so the chain is not changed.
 int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
 {
   CTXCHK(ctx);
   if (ctx->opts.z_store != 2 || reps < 1) {
-    set_error("bsg_autotune: tunes the Z-free latent kernel (z_store 2); reps >= 1");
+    set_error("bsg_autotune: tunes the Z-free sweep (z_store 2); reps >= 1");
     return BSG_ERR_ARG;
   }
+  bsg::DevProblem &pb = ctx->pb;
+  const size_t C = ctx->C, V = ctx->L.totvox, P = ctx->P;
+  struct Part {
+    void *p;
+    size_t bytes;
+  };
+  const Part parts[] = {{pb.beta, 4 * C * V * P}, {pb.M, 4 * C * V * P},   {pb.lam, 4 * C * P * P},
+                        {pb.lam64, 8 * C * P * P}, {pb.tsum, 8 * C * P * P}, {pb.bwm, 4 * C},
+                        {pb.flags, 16},            {pb.mpart, pb.mpart ? 4 * C * pb.mp_stride : 0}};
+  size_t total = 0;
+  for (const Part &q : parts) total += (q.bytes + 255) & ~(size_t)255;
+  char *save = nullptr;
+  if (hipMalloc((void **)&save, total) != hipSuccess) {
+    set_error("bsg_autotune: hipMalloc of the saved state failed");
+    return BSG_ERR_NOMEM;
+  }
+  auto copy = [&](bool out) {
+    size_t off = 0;
+    for (const Part &q : parts) {
+      if (q.bytes &&
+          hipMemcpyAsync(out ? save + off : q.p, out ? q.p : save + off, q.bytes, hipMemcpyDeviceToDevice,
+                         ctx->stream) != hipSuccess)
+        return false;
+      off += (q.bytes + 255) & ~(size_t)255;
+    }
+    return true;
+  };
+  const uint32_t sw = ctx->sweep;
+  const int tp = ctx->tail_pending, keep = pb.lat_g;
+  auto restore = [&]() {
+    ctx->sweep = sw;
+    ctx->tail_pending = tp;
+    return copy(false);
+  };
   const int cand[4] = {1, 2, 4, 8};
   double best = 1e300;
   int bg = 0;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  const int keep = ctx->pb.lat_g;
-  int rc = BSG_OK, pieces = ctx->tail_pending;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = copy(true) && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess ? BSG_OK : BSG_ERR_HIP;
   for (int k = 0; k < 4 && rc == BSG_OK; k++) {
     const int G = cand[k];
     double ms = -1.0;
-    if (G == 1 || G <= ctx->NQ) {  // more lanes per voxel than subject quads would idle lanes
-      ctx->pb.lat_g = G;
-      // draws at the current sweep index: the state the next sweep reads is not changed
-      if (bsg::launch_latent(ctx->pb, true, 2, ctx->sweep, ctx->stream, 0, ctx->L.totvox) != hipSuccess ||
-          hipEventRecord(e0, ctx->stream) != hipSuccess) {
+    {
+      pb.lat_g = G;
+      float f = 0.0f;
+      if (!restore() || sweep_run(ctx, 1, nullptr) != BSG_OK || !restore() ||
+          hipEventRecord(e0, ctx->stream) != hipSuccess || sweep_run(ctx, reps, nullptr) != BSG_OK ||
+          hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&f, e0, e1) != hipSuccess) {
         rc = BSG_ERR_HIP;
         break;
       }
-      for (int r = 0; r < reps; r++)
-        if (bsg::launch_latent(ctx->pb, true, 2, ctx->sweep, ctx->stream, 0, ctx->L.totvox, nullptr, 0, 0,
-                               &pieces) != hipSuccess)
-          rc = BSG_ERR_HIP;
-      float f = 0.0f;
-      if (rc == BSG_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                           hipEventElapsedTime(&f, e0, e1) != hipSuccess))
-        rc = BSG_ERR_HIP;
       ms = f / reps;
-      if (rc == BSG_OK && ms < best) {
+      if (ms < best) {
         best = ms;
         bg = G;
       }
     }
     if (ms_per_g) ms_per_g[k] = ms;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (!restore() || hipStreamSynchronize(ctx->stream) != hipSuccess) rc = BSG_ERR_HIP;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(save);
   if (rc != BSG_OK) {
-    ctx->pb.lat_g = keep;
-    set_error("bsg_autotune: latent launch failed");
+    pb.lat_g = keep;
+    set_error("bsg_autotune: a timed sweep failed");
     return rc;
   }
-  ctx->pb.lat_g = bg;
-  ctx->tail_pending = pieces;  // the pieces the last timed launch left
+  pb.lat_g = bg;
   if (best_g) *best_g = bg;
   return BSG_OK;
 }

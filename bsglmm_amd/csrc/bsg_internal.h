@@ -94,15 +94,19 @@ struct DevProblem {
   int lam_spin, lam_tag;
 };
 
-// Lanes per voxel of the Z-free latent kernels: enough lanes per voxel to give
-// the chip many waves, few enough that every lane keeps several quads. Past 16
-// quads the inversion path keeps G = 4, where its means run on the matrix cores.
+// Lanes per voxel of the Z-free latent kernels. Inversion draws take G = 4 at
+// every N: there the means run on the matrix cores, the latent launch carries
+// the Wishart draw and the sweep overlaps (at small N this doubles the sweep
+// rate, the kernel's own time aside). The accept-or-fallback draws take
+// enough lanes per voxel to give the chip many waves, few enough that every
+// lane keeps several quads.
 __host__ __device__ inline int vl_groups(const DevProblem &pb)
 {
   if (pb.lat_g > 0) return pb.lat_g;
+  if (pb.latent_method == 0) return 4;
   if (pb.NQ <= 4) return 1;
   if (pb.NQ <= 16) return 2;
-  if (pb.NQ <= 128 || pb.latent_method == 0) return 4;
+  if (pb.NQ <= 128) return 4;
   return 8;
 }
 

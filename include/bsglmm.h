@@ -176,11 +176,13 @@ int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count);     /* FP64 Cholesky 
 
 /* ---- tile shapes and batches of analyses (no counterpart in the reference:
  * it runs one analysis per process) ------------------------------------- */
-/* Times the Z-free latent kernel (z_store 2) of this context at each lane
- * split G in {1, 2, 4, 8} lanes per voxel (voxels per workgroup = 256 / G,
- * subject tile = N / G), reps launches each at the current sweep index (the
- * chain state is not changed), and keeps the fastest for later sweeps.
- * best_g receives it; ms_per_g[4] (optional) the mean times, -1 if skipped. */
+/* Times whole Z-free sweeps (z_store 2) of this context at each lane split G
+ * in {1, 2, 4, 8} lanes per voxel (voxels per workgroup = 256 / G, subject
+ * tile = N / G), reps sweeps each from the same state, overlapped where the
+ * split allows it (the split also decides that), restores the chain state
+ * (coefficients, X'Z, Lambda, pair sums, b, sweep index), and keeps the
+ * fastest split for later sweeps. best_g receives it; ms_per_g[4] (optional)
+ * the mean sweep times. */
 int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g);
 /* Sets the lane split G of the Z-free latent kernel (1, 2, 4 or 8; 0 = chosen
  * from N), e.g. a split bsg_autotune found in an earlier process. Sweeps are

@@ -57,21 +57,36 @@ def test_batch_equals_members_run_alone():
     B.close()
 
 
-def test_autotune_keeps_the_chain_state():
-    print("case autotune c5_3 (N=250, P=5)", flush=True)
-    d = synth.generate("c5_3")
+@pytest.mark.parametrize("name,nchains", [("c5_3", 2), ("c1", 1)])
+def test_autotune_keeps_the_chain_state(name, nchains):
+    """The autotuner times whole sweeps at every lane split from one saved
+    state and puts the state back: coefficients, X'Z, Lambda, pair sums and
+    the sweep index are unchanged, and the next sweeps equal those of an
+    untouched twin pinned to the chosen split."""
+    print(f"case autotune {name} chains={nchains}", flush=True)
+    d = synth.generate(name)
     L = Layout(d["mask"], *d["grid"])
-    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=2, xtz_mode=2, z_store=2)
-    S.sweep(3)
-    before = [S.get_spat_coef(c) for c in range(2)], S.sweep_index()
+    S, T = (Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2) for _ in range(2))
+    for X in (S, T):
+        X.sweep(3)
+        X.synchronize()
+
+    def state(X):
+        return [(X.get_spat_coef(c), X.get_xtz(c), X.get_prec(c), X.get_pair_sums(c)) for c in range(nchains)]
+
+    before, idx = state(S), S.sweep_index()
     g, ms = S.autotune(reps=2)
-    assert g in (1, 2, 4, 8) and ms[[1, 2, 4, 8].index(g)] == ms[ms > 0].min()
-    assert S.sweep_index() == before[1]
-    for c in range(2):
-        assert np.array_equal(S.get_spat_coef(c), before[0][c])
+    assert g in (1, 2, 4, 8) and ms[[1, 2, 4, 8].index(g)] == ms.min()
+    assert S.sweep_index() == idx
+    for a, b in zip(state(S), before):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    T.lane_split = g
     S.sweep(2)  # sweeps run with the tuned split
-    S.synchronize()
-    assert np.all(np.isfinite(S.get_spat_coef(0)))
+    T.sweep(2)
+    for a, b in zip(state(S), state(T)):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
 
 
 def test_batch_rejects_unsupported_members():

@@ -1357,8 +1357,10 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
 // ---- batches of independent analyses ------------------------------------
 struct bsg_batch {
   std::vector<bsg_ctx *> m;
-  hipStream_t sa = nullptr, sb = nullptr;
-  hipEvent_t e_c0 = nullptr, e_c1 = nullptr;
+  // one stream per covariate count present: the members of one P share
+  // every launch, and members of different P are independent, so their
+  // launches overlap
+  hipStream_t sp[bsg::MAX_P + 1] = {};
   bsg::DevProblem *d_pbs = nullptr;
   bsg::WorkItem *d_items = nullptr;
   bsg::Accum *d_accs = nullptr;
@@ -1372,15 +1374,13 @@ struct bsg_batch {
 
   ~bsg_batch()
   {
-    if (sa) hipStreamSynchronize(sa);
-    if (sb) hipStreamSynchronize(sb);
+    for (hipStream_t s : sp)
+      if (s) hipStreamSynchronize(s);
     if (d_pbs) hipFree(d_pbs);
     if (d_items) hipFree(d_items);
     if (d_accs) hipFree(d_accs);
-    if (e_c0) hipEventDestroy(e_c0);
-    if (e_c1) hipEventDestroy(e_c1);
-    if (sb) hipStreamDestroy(sb);
-    if (sa) hipStreamDestroy(sa);
+    for (hipStream_t s : sp)
+      if (s) hipStreamDestroy(s);
   }
 };
 
@@ -1445,10 +1445,8 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     set_error("bsg_batch_create: hipMalloc failed");
     return BSG_ERR_NOMEM;
   }
-  HIPCHK(hipStreamCreateWithFlags(&b->sa, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&b->sb, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&b->e_c0, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&b->e_c1, hipEventDisableTiming));
+  for (int P = 1; P <= bsg::MAX_P; P++)
+    if (b->cnt[0][P] + b->cnt[4][P] > 0) HIPCHK(hipStreamCreateWithFlags(&b->sp[P], hipStreamNonBlocking));
   HIPCHK(hipMemcpy(b->d_pbs, pbs.data(), sizeof(bsg::DevProblem) * n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_items, all.data(), sizeof(bsg::WorkItem) * all.size(), hipMemcpyHostToDevice));
   *out = b.release();
@@ -1461,22 +1459,15 @@ int bsg_batch_destroy(bsg_batch *b)
   return BSG_OK;
 }
 
-// Sweeps of every analysis and chain, one launch per phase: latent draws and
-// X'Z, colour 0, colour 1 (+ pair partials), Wishart on the second stream.
+// Sweeps of every analysis and chain, one launch per phase and covariate
+// count P: latent draws and X'Z, colour 0, colour 1 (+ pair partials),
+// Wishart, in that order on the stream of P.
 static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
 {
-  hipStream_t sa = b->sa, sb = b->sb;
   const bsg::WorkItem *it = b->d_items;
-  bool wish_pending = false;
   bsg::BatchIndex bidx{};
   for (int i = 0; i < nsweeps; i++) {
     const uint32_t sw = b->m[0]->sweep;
-    for (int P = 1; P <= bsg::MAX_P; P++) {
-      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds[P], b->bmf, false, sw,
-                                      sa));
-      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[4][P], b->cnt[4][P], b->lat_lds[P], false, true, sw, sa));
-    }
-    if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, b->e_c1, 0));
     // a save sweep: the colour updates accumulate each member's voxels
     const bool save = accum_every > 0 && (i + 1) % accum_every == 0;
     if (save) {
@@ -1487,20 +1478,20 @@ static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
       }
     }
     const bsg::Accum *accs = save ? b->d_accs : nullptr;
-    for (int P = 1; P <= bsg::MAX_P; P++)
-      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, sa, accs, bidx));
-    for (int P = 1; P <= bsg::MAX_P; P++)
-      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, sa, accs, bidx));
-    HIPCHK(hipEventRecord(b->e_c0, sa));
-    HIPCHK(hipStreamWaitEvent(sb, b->e_c0, 0));
-    for (int P = 1; P <= bsg::MAX_P; P++)
-      HIPCHK(bsg::launch_wishart_multi(P, b->d_pbs, it + b->off[3][P], b->cnt[3][P], sw, sb));
-    HIPCHK(hipEventRecord(b->e_c1, sb));
-    wish_pending = true;
+    for (int P = 1; P <= bsg::MAX_P; P++) {
+      hipStream_t st = b->sp[P];
+      if (!st) continue;
+      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds[P], b->bmf, false, sw,
+                                      st));
+      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[4][P], b->cnt[4][P], b->lat_lds[P], false, true, sw, st));
+      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, st, accs, bidx));
+      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, st, accs, bidx));
+      HIPCHK(bsg::launch_wishart_multi(P, b->d_pbs, it + b->off[3][P], b->cnt[3][P], sw, st));
+    }
     for (bsg_ctx *c : b->m) c->sweep++;
   }
-  if (wish_pending) HIPCHK(hipStreamWaitEvent(sa, b->e_c1, 0));
-  HIPCHK(hipStreamSynchronize(sa));
+  for (hipStream_t st : b->sp)
+    if (st) HIPCHK(hipStreamSynchronize(st));
   return BSG_OK;
 }
 

@@ -15,9 +15,10 @@ SEED = 0xBA7C4ED
 
 
 def members(seed_shift=0):
-    """Three analyses with different masks, N, P and chain counts."""
+    """Four analyses with different masks, N and chain counts; three covariate
+    counts, so three streams, one of them shared by two analyses (P = 3)."""
     out = []
-    for (name, n, p, chains) in (("c1", 50, 3, 2), ("c1", 7, 2, 1), ("c5_0", 120, 5, 3)):
+    for (name, n, p, chains) in (("c1", 50, 3, 2), ("c1", 7, 2, 1), ("c5_0", 120, 5, 3), ("c5_1", 30, 3, 1)):
         d = synth.generate(name, nsubs=n, ncovar=p)
         L = Layout(d["mask"], *d["grid"])
         out.append((d, L, chains))
@@ -30,7 +31,7 @@ def make(ms):
 
 
 def test_batch_equals_members_run_alone():
-    print("case batch of 3 analyses (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3)", flush=True)
+    print("case batch of 4 analyses (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3, c5_1 N=30 P=3)", flush=True)
     ms = members()
     solo, grouped = make(ms), make(ms)
     B = Batch(grouped)

@@ -208,7 +208,8 @@ typedef struct bsg_batch bsg_batch;
 /* Groups up to 32 contexts (independent analyses: own masks, N, P and chain
  * counts) so that each phase of a sweep is one kernel launch over all of
  * them with the same covariate count (one launch per P present, so each has
- * its own register allocation). Members must use the Z-free form (xtz_mode 2, z_store 2), have no WM
+ * its own register allocation, and each P on its own stream, so that the
+ * launches of different P overlap). Members must use the Z-free form (xtz_mode 2, z_store 2), have no WM
  * image and share the latent method and sweep index; each keeps its own
  * device state and lane split, and must outlive the batch. */
 int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out);

@@ -1464,6 +1464,15 @@ int bsg_batch_destroy(bsg_batch *b)
 // Wishart, in that order on the stream of P.
 static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
 {
+  // the members' own queued work first; every member at the same sweep index,
+  // since the batch draws every member's Philox words from one index
+  for (bsg_ctx *c : b->m) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->sweep != b->m[0]->sweep) {
+      set_error("bsg_batch_sweep: members are at different sweep indices (a member was swept on its own)");
+      return BSG_ERR_ARG;
+    }
+  }
   const bsg::WorkItem *it = b->d_items;
   bsg::BatchIndex bidx{};
   for (int i = 0; i < nsweeps; i++) {

@@ -214,7 +214,9 @@ typedef struct bsg_batch bsg_batch;
  * device state and lane split, and must outlive the batch. */
 int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out);
 int bsg_batch_destroy(bsg_batch *batch);
-/* nsweeps sweeps of every member; returns when they are done */
+/* nsweeps sweeps of every member; returns when they are done. Waits for work
+ * queued on the members' own streams first; BSG_ERR_ARG if the members'
+ * sweep indices differ (a member was swept on its own since the last call). */
 int bsg_batch_sweep(bsg_batch *batch, int nsweeps);
 /* as bsg_sweep_accumulate for every member (bsg_accum_reset each first) */
 int bsg_batch_sweep_accumulate(bsg_batch *batch, int nsweeps, int save_every);

@@ -99,3 +99,22 @@ def test_batch_rejects_unsupported_members():
     A.sweep(1)
     with pytest.raises(Exception):  # sweep indices differ
         Batch([A, Sampler(L, d["covar"], d["data"], seed=2, xtz_mode=2, z_store=2)])
+
+
+def test_batch_refuses_members_swept_on_their_own():
+    """The batch draws every member's Philox words from one sweep index: a
+    member swept alone after the batch was formed is refused, not reused."""
+    print("case batch after a member swept alone", flush=True)
+    d = synth.generate("c1", nsubs=9, ncovar=2)
+    L = Layout(d["mask"], *d["grid"])
+    A = Sampler(L, d["covar"], d["data"], seed=1, xtz_mode=2, z_store=2)
+    Bm = Sampler(L, d["covar"], d["data"], seed=2, xtz_mode=2, z_store=2)
+    B = Batch([A, Bm])
+    B.sweep(2)
+    A.sweep(1)
+    with pytest.raises(Exception, match="sweep indices"):
+        B.sweep(1)
+    Bm.sweep(1)  # back at one index: the batch runs again
+    B.sweep(1)
+    assert A.sweep_index() == Bm.sweep_index() == 4
+    B.close()

@@ -116,6 +116,28 @@ static int dev_alloc(bsg_ctx *ctx, T **p, size_t count)
   return BSG_OK;
 }
 
+// Posterior-accumulator batch of the next saved draw: draw k goes to batch
+// k / batch_len while that is below nbatches; later draws add to the sums but
+// to no batch (-1), so every completed batch holds exactly batch_len draws.
+static int acc_batch_index(const bsg_ctx *ctx)
+{
+  const int b = ctx->acc_count / ctx->acc_batch_len;
+  return b < ctx->acc.nbatches ? b : -1;
+}
+
+// A captured sweep graph holds the problem (lane split, wait hooks) by value:
+// drop it whenever one of those changes, so the next graph call captures anew.
+static void drop_graph(bsg_ctx *ctx)
+{
+  if (!ctx->graph_exec && !ctx->graph) return;
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_k = 0;
+}
+
 extern "C" {
 
 const char *bsg_last_error(void) { return bsg::g_last_error.c_str(); }
@@ -307,6 +329,11 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   // b is drawn each sweep only when a WM image is given (mcmc.cpp:344-345, UPDATE_WM)
   pb.wm_update = wm ? 1 : 0;
   pb.latent_method = o.latent_method;
+  if (o.z_store == 2 && o.latent_method == 0 && bsg::latent_vl_lds_bytes(pb) > 65536) {
+    set_error("bsg_create: z_store 2 stages X in LDS: 16 * 4 ceil(N/16) * P bytes must fit in 64 KiB "
+              "(N * P <= 16384); use z_store 0 or 1");
+    return BSG_ERR_ARG;
+  }
   if (o.latent_method == 1 && (N > 32768 || bsg::latent_vl_lds_bytes(pb) > 65536)) {
     set_error("bsg_create: latent_method 1 needs N <= 32768 and ceil(N/4) * P * 16 bytes of X plus the "
               "fallback queues in 64 KiB of LDS");
@@ -551,8 +578,7 @@ static int accum_launch(bsg_ctx *ctx, hipStream_t st)
     set_error("posterior accumulation: call bsg_accum_reset first");
     return BSG_ERR_ARG;
   }
-  const int batch = std::min(ctx->acc_count / ctx->acc_batch_len, ctx->acc.nbatches - 1);
-  HIPCHK(bsg::launch_accum(ctx->pb, ctx->acc, batch, st));
+  HIPCHK(bsg::launch_accum(ctx->pb, ctx->acc, acc_batch_index(ctx), st));
   ctx->acc_count++;
   return BSG_OK;
 }
@@ -632,7 +658,7 @@ static int enqueue_overlap(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off
   for (int i = 0; i < k; i++) {
     const uint32_t off = off0 + (uint32_t)i;
     const bool save = accum_every > 0 && (i + 1) % accum_every == 0;
-    const int batch = save ? std::min(ctx->acc_count / ctx->acc_batch_len, ctx->acc.nbatches - 1) : 0;
+    const int batch = save ? acc_batch_index(ctx) : 0;
     const bsg::Accum *ac = save ? &ctx->acc : nullptr;
     if (ev) HIPCHK(hipEventRecord(ev[2 * i], sa));
     HIPCHK(bsg::launch_overlap(pb, 0, off, 1, i > 0 ? 1 : 0, off - 1, tail0, sa, ac, batch));
@@ -663,7 +689,7 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
     // save sweeps accumulate inside the colour updates (mcmc.cpp:584-651 saves
     // after the sweep; each voxel's final b of the sweep is what it adds)
     const bool save = accum_every > 0 && (i + 1) % accum_every == 0;
-    const int batch = save ? std::min(ctx->acc_count / ctx->acc_batch_len, ctx->acc.nbatches - 1) : 0;
+    const int batch = save ? acc_batch_index(ctx) : 0;
     const int rc = enqueue_sweep(ctx, ctx->pb, ctx->sweep, lat_ev ? lat_ev + 2 * i : nullptr, wish_pending, w_inline,
                                  save ? &ctx->acc : nullptr, batch);
     if (rc != BSG_OK) return rc;
@@ -908,8 +934,11 @@ int bsg_synchronize(bsg_ctx *ctx)
     return BSG_ERR_HIP;
   }
   if (flags[1]) {
+    // reported once: the next sweeps start with the count clear
+    const uint32_t zero = 0;
+    HIPCHK(hipMemcpy(ctx->pb.flags + 1, &zero, sizeof(zero), hipMemcpyHostToDevice));
     set_error("coefficient update: " + std::to_string(flags[1]) +
-              " voxel(s) had a non positive definite posterior precision");
+              " voxel(s) had a non positive definite posterior precision and kept their coefficients");
     return BSG_ERR_NUM;
   }
   return BSG_OK;
@@ -1221,6 +1250,7 @@ int bsg_set_lane_split(bsg_ctx *ctx, int g)
     set_error("bsg_set_lane_split: G must be 0, 1, 2, 4 or 8");
     return BSG_ERR_ARG;
   }
+  if (ctx->pb.lat_g != g) drop_graph(ctx);
   ctx->pb.lat_g = g;
   return BSG_OK;
 }
@@ -1243,6 +1273,7 @@ int bsg_test_overlap_wait(bsg_ctx *ctx, int spin_limit, int tag)
     set_error("bsg_test_overlap_wait: negative argument");
     return BSG_ERR_ARG;
   }
+  drop_graph(ctx);
   ctx->pb.lam_spin = spin_limit;
   ctx->pb.lam_tag = tag;
   return BSG_OK;
@@ -1349,6 +1380,7 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
     set_error("bsg_autotune: a timed sweep failed");
     return rc;
   }
+  if (pb.lat_g != bg || bg != keep) drop_graph(ctx);
   pb.lat_g = bg;
   if (best_g) *best_g = bg;
   return BSG_OK;
@@ -1482,7 +1514,7 @@ static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
     if (save) {
       for (size_t a = 0; a < b->m.size(); a++) {
         bsg_ctx *c = b->m[a];
-        bidx.b[a] = std::min(c->acc_count / c->acc_batch_len, c->acc.nbatches - 1);
+        bidx.b[a] = acc_batch_index(c);
         c->acc_count++;
       }
     }

@@ -1285,18 +1285,20 @@ __device__ __forceinline__ void accum_voxel(const DevProblem &pb, const Accum &a
   const double bw = (double)pb.bwm[chain] * (double)pb.wm[s];
   const size_t base = (size_t)chain * pb.V + s;
   const int K = P + ac.ntypes;
-  float *bat = ac.batch + (((size_t)chain * ac.nbatches + batch) * pb.V + s) * K;
+  // batch < 0: the draw lies past the last batch and adds to the sums only
+  const bool inb = batch >= 0;
+  float *bat = ac.batch + (((size_t)chain * ac.nbatches + (inb ? batch : 0)) * pb.V + s) * K;
 #pragma unroll
   for (int c = 0; c < P; c++) {
     const double v = (double)b[c];
     ac.sum[base * P + c] += v;
     ac.sq[base * P + c] += v * v;
-    bat[c] += b[c];
+    if (inb) bat[c] += b[c];
   }
   for (int t = 0; t < ac.ntypes && t < P; t++) {
     const double pr = phi_as((double)b[t] + bw);
     ac.prb[base * ac.ntypes + t] += pr;
-    bat[P + t] += (float)pr;
+    if (inb) bat[P + t] += (float)pr;
   }
   ac.bwm[base] += bw;
 }
@@ -1321,8 +1323,8 @@ __device__ __forceinline__ void accum_body(const DevProblem &pb, const Accum &ac
 // b = U^-T (U^-1 r + z): one factorisation and two triangular solves, no
 // explicit inverse. PREC (colour 1 of a full sweep) also accumulates the pair
 // terms of each voxel with its six colour-0 neighbours -- every neighbour pair
-// has exactly one colour-1 member -- and the last block to finish draws the
-// CAR precision (no spinning: a completion counter elects it).
+// has exactly one colour-1 member -- into one FP64 partial per block, which
+// the CAR-precision draw (wishart_block) sums in block order.
 // ---------------------------------------------------------------------------
 template <int P, typename T>
 __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], const T (&z)[P],

@@ -114,8 +114,11 @@ int bsg_update_coef(bsg_ctx *ctx, int colour);
  * device Wishart draw; advances the sweep index */
 int bsg_update_car_prec(bsg_ctx *ctx);
 /* waits for the context's stream; reports kernel-side numerical failures
- * (BSG_ERR_NUM), and BSG_ERR_HIP if an overlapped sweep's colour-0 blocks
- * waited past their bound for the CAR precision drawn beside them */
+ * (BSG_ERR_NUM: voxels whose posterior precision was not positive definite
+ * in FP64 either kept their coefficients; the reference propagates NaNs
+ * silently, covarGPU.cu:374), and BSG_ERR_HIP if an overlapped sweep's
+ * colour-0 blocks waited past their bound for the CAR precision drawn beside
+ * them. Each failure is reported once; the next sweeps start with it clear. */
 int bsg_synchronize(bsg_ctx *ctx);
 /* bsg_sweep with HIP events around every latent kernel: latent_ms receives
  * their summed durations, total_ms (optional) the whole span. Synchronous.
@@ -134,7 +137,8 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6]);
  * beta and beta^2, of the lesion probability Phi(beta_t + b WM) for the first
  * ntypes coefficients (compute_prbGPU, covarGPU.cu:1814-1824; Phi by
  * A&S 26.2.17 as covar.cpp:327-348) and of b WM; FP32 sums over nbatches
- * batches of batch_len draws for batch-means Monte-Carlo errors. */
+ * batches of batch_len draws for batch-means Monte-Carlo errors (draws past
+ * nbatches * batch_len add to the sums but to no batch). */
 int bsg_accum_reset(bsg_ctx *ctx, int ntypes, int nbatches, int batch_len);
 /* adds the current draw of every chain */
 int bsg_accum_add(bsg_ctx *ctx);
@@ -143,7 +147,8 @@ int bsg_sweep_accumulate(bsg_ctx *ctx, int nsweeps, int save_every);
 /* Posterior summaries of one chain in mask-rank order, any output may be NULL:
  * mean, var [P][TOTVOX] (variance with the n-1 denominator, mcmc.cpp:871-882),
  * prb [ntypes][TOTVOX], bwm_mean [TOTVOX], mcse [P+ntypes][TOTVOX] (batch
- * means over the completed batches). ndraws receives the number of draws. */
+ * means over the completed batches, each of batch_len draws). ndraws receives
+ * the number of draws. */
 int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *var, double *prb,
                   double *bwm_mean, double *mcse);
 

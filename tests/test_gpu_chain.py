@@ -5,8 +5,9 @@ path (tests/golden/ref_chain_c1.npz, made by tests/golden/make_ref_chain.py).
 The two chains use different random streams (the reference's host KISS
 versus Philox) and different exact truncated-normal samplers (Robert 1995
 versus inversion), so agreement is statistical: every posterior mean must
-agree within 4 combined Monte-Carlo standard errors, up to the handful of
-exceedances expected among ~28,000 comparisons.
+agree within 4 combined Monte-Carlo standard errors, up to the exceedances a
+Student-t null allows among the 28,224 comparisons (the binomial 1e-4 upper
+quantile of the count at P(|t_49| > 4): 17, against 6.0 expected).
 """
 import os
 
@@ -25,6 +26,15 @@ def phi_as(x):
     poly = t * (0.319381530 + t * (-0.356563782 + t * (1.781477937 + t * (-1.821255978 + t * 1.330274429))))
     tail = 0.39894228 * np.exp(-0.5 * x * x) * poly
     return np.where(x >= 0, 1.0 - tail, tail)
+
+
+def exceedance_bound(n, nu, z=4.0, q=1e-4):
+    """Smallest k with P(count > k) < q when each of n comparisons exceeds |z|
+    with probability P(|t_nu| > z): the MCSEs are batch-means estimates with
+    nu degrees of freedom, so |z| > 4 is likelier than under a normal."""
+    from scipy import stats
+    p0 = 2.0 * stats.t.sf(z, nu)
+    return int(stats.binom.ppf(1.0 - q, n, p0)), n * p0
 
 
 def rank_view(sc, L):
@@ -67,6 +77,41 @@ def test_accumulator_matches_host_snapshots():
     assert S.accum_get(0)["ndraws"] == 6
 
 
+@pytest.mark.parametrize("nbatches,batch_len,ndraws", [(4, 3, 17), (10, 4, 10), (3, 5, 15)])
+def test_mcse_uses_completed_batches_only(nbatches, batch_len, ndraws):
+    """Batch-means MCSE over the completed batches of batch_len draws: draws
+    past nbatches * batch_len (and those of an unfinished batch) add to the
+    means but to no batch. Both the separate accumulation pass (k_accum) and
+    the save sweeps' fused accumulation are checked against host batch means
+    of the GPU's own draws."""
+    print(f"case c1 MCSE nbatches={nbatches} batch_len={batch_len} draws={ndraws}", flush=True)
+    d = synth.generate("c1", nsubs=150)
+    L = Layout(d["mask"], *d["grid"])
+    A, F = (Sampler(L, d["covar"], d["data"], seed=91, xtz_mode=2, z_store=2) for _ in range(2))
+    for X in (A, F):
+        X.sweep(3)
+        X.accum_reset(ntypes=1, nbatches=nbatches, batch_len=batch_len)
+    snaps = []
+    for _ in range(ndraws):
+        A.sweep(1)
+        A.accum_add()
+        snaps.append(rank_view(A.get_spat_coef(), L))
+    F.sweep_accumulate(ndraws, 1)
+    b = np.array(snaps)
+    x = np.concatenate([b, phi_as(b[:, :1])], axis=1)  # draws x (P + 1) x V
+    nb = min(ndraws // batch_len, nbatches)
+    for X in (A, F):
+        out = X.accum_get(0)
+        assert out["ndraws"] == ndraws
+        assert np.allclose(out["mean"], b.mean(0), rtol=1e-6, atol=1e-6)
+        if nb > 1:
+            bm = x[:nb * batch_len].reshape(nb, batch_len, *x.shape[1:]).mean(1)
+            mcse = bm.std(0, ddof=1) / np.sqrt(nb)
+            assert np.allclose(out["mcse"], mcse, rtol=1e-3, atol=1e-6)
+        else:
+            assert np.all(out["mcse"] == 0.0)
+
+
 @pytest.mark.parametrize("z_store,method,g", [(0, 0, 0), (2, 1, 0), (2, 0, 4)])
 def test_posterior_means_match_reference_cpu_chain(z_store, method, g):
     """z_store 0: every draw by inversion (k_latent); z_store 2 with method 1:
@@ -94,9 +139,11 @@ def test_posterior_means_match_reference_cpu_chain(z_store, method, g):
     out = S.accum_get(0)
     ours = np.vstack([out["mean"], out["prb"]])
     z = (ours - f["mean"]) / np.sqrt(out["mcse"] ** 2 + f["mcse"].astype(np.float64) ** 2)
-    frac = float(np.mean(np.abs(z) > 4.0))
-    print(f"exceedance fraction {frac:.2e}, median |z| {np.median(np.abs(z)):.3f}", flush=True)
-    assert frac < 2e-3, frac
+    count = int(np.sum(np.abs(z) > 4.0))
+    bound, expect = exceedance_bound(z.size, nb - 1)
+    print(f"{count} of {z.size} beyond 4 MCSE (null expectation {expect:.1f}, bound {bound}), "
+          f"median |z| {np.median(np.abs(z)):.3f}", flush=True)
+    assert count <= bound, count
     assert np.median(np.abs(z)) < 1.2  # about 0.67 for a standard normal z
     assert S.fallback_count() == 0
     assert out["mean"].shape == (P, L.totvox)

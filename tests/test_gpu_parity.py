@@ -412,3 +412,24 @@ def test_graph_replays_equal_direct_sweeps(z_store, xtz_mode, wm, nsubs):
             assert np.array_equal(G.get_spat_coef(c), A.get_spat_coef(c))
             assert np.array_equal(G.get_prec(c), A.get_prec(c))
             assert G.get_wm_coef(c) == A.get_wm_coef(c)
+
+
+def test_graph_recaptured_after_lane_split_change():
+    """A captured sweep graph holds the lane split by value: after the split
+    changes (here 2 -> 8 -> 1, none of them overlapped, so the graph's form key
+    stays the same), graph sweeps must still equal direct sweeps."""
+    print("case c1 N=150 graph after lane-split changes", flush=True)
+    d = synth.generate("c1", nsubs=150)
+    L = Layout(d["mask"], *d["grid"])
+    A = Sampler(L, d["covar"], d["data"], seed=SEED, xtz_mode=2, z_store=2)
+    G = Sampler(L, d["covar"], d["data"], seed=SEED, xtz_mode=2, z_store=2)
+    for g in (2, 8, 1):
+        A.lane_split = g
+        G.lane_split = g
+        assert not G.overlap
+        A.sweep(8)
+        G.sweep_graph(8, 8)
+        A.synchronize()
+        G.synchronize()
+        assert np.array_equal(A.get_spat_coef(), G.get_spat_coef()), g
+        assert np.array_equal(A.get_prec(), G.get_prec()), g

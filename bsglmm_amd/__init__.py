@@ -88,6 +88,7 @@ def lib():
         "bsg_measure_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, pd, pd]),
         "bsg_measure_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, pd, pd]),
         "bsg_coef_means": (C.c_int, [vp, C.c_int, pd]),
+        "bsg_coef_means_noalloc": (C.c_int, [vp, C.c_int, pd]),
         "bsg_loglik": (C.c_int, [vp, C.c_int, C.c_int, pd]),
         "bsg_nifti_resolve": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
         "bsg_nifti_info": (C.c_int, [C.c_char_p, pi, pi]),

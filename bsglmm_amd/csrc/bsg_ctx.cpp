@@ -252,6 +252,8 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     set_error("bsg_create: no HIP device");
     return BSG_ERR_HIP;
   }
+  // the allocation-free entry points' scratch on this device (once per device)
+  HIPCHK(bsg::scratch_init());
 
   std::unique_ptr<bsg_ctx> ctx(new (std::nothrow) bsg_ctx);
   if (!ctx) {
@@ -501,6 +503,37 @@ int bsg_coef_means(bsg_ctx *ctx, int chain, double *mean)
     for (int b = 0; b < CM_BLOCKS; b++) acc += h[(size_t)b * P + c];
     mean[c] = acc / (double)ctx->L.totvox;
   }
+  return BSG_OK;
+}
+
+int bsg_coef_means_noalloc(bsg_ctx *ctx, int chain, double *mean)
+{
+  CTXCHK(ctx);
+  if (chain < 0 || chain >= ctx->C || !mean) {
+    set_error("bsg_coef_means_noalloc: bad chain or NULL output");
+    return BSG_ERR_ARG;
+  }
+  const int P = ctx->P, V = ctx->L.totvox;
+  // partials [CM_BLOCKS][P], then the P folded sums
+  if ((size_t)CM_BLOCKS * P + P > bsg::SCRATCH_DOUBLES) {
+    set_error("bsg_coef_means_noalloc: " + std::to_string(CM_BLOCKS) + " x (P + 1) doubles exceed the " +
+              std::to_string(bsg::SCRATCH_DOUBLES) + "-double scratch");
+    return BSG_ERR_ARG;
+  }
+  double *scr = bsg::scratch_ptr();
+  if (!scr) {
+    set_error("bsg_coef_means_noalloc: no scratch on the current device (create the context with it current)");
+    return BSG_ERR_HIP;
+  }
+  // the same kernel and block partials as bsg_coef_means, for this chain alone
+  bsg::DevProblem one = ctx->pb;
+  one.beta = ctx->pb.beta + (size_t)chain * V * P;
+  one.nchains = 1;
+  HIPCHK(bsg::launch_cmeans(one, scr, CM_BLOCKS, ctx->stream));
+  HIPCHK(bsg::launch_cmeans_fold(scr, CM_BLOCKS, P, scr + (size_t)CM_BLOCKS * P, ctx->stream));
+  HIPCHK(hipMemcpyAsync(mean, scr + (size_t)CM_BLOCKS * P, sizeof(double) * P, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < P; c++) mean[c] /= (double)V;
   return BSG_OK;
 }
 

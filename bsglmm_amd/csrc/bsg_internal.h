@@ -221,6 +221,17 @@ hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *it
                                 hipStream_t st);
 // plane means: part[chain][blocks][P] FP64 block partials
 hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st);
+// out[c] = sum of part[b][c] over b < blocks in block order (one chain)
+hipError_t launch_cmeans_fold(const double *part, int blocks, int P, double *out, hipStream_t st);
+
+// Allocation-free scratch: a fixed device array of SCRATCH_DOUBLES doubles
+// (64 KiB) per device. scratch_init takes its address on the current device
+// (once per device; bsg_create calls it); scratch_ptr returns that address,
+// or null before scratch_init ran on the current device.
+constexpr size_t SCRATCH_DOUBLES = 8192;
+constexpr int MAX_DEVICES = 64;
+hipError_t scratch_init();
+double *scratch_ptr();
 // per-subject log-likelihood: part [chain][tiles][N][T], out [chain][N][T]
 int loglik_tiles(const DevProblem &pb);
 hipError_t launch_loglik(const DevProblem &pb, int T, double *part, double *out, hipStream_t st);

@@ -19,6 +19,7 @@
 #include "bsg_internal.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace bsg {
 
@@ -2208,6 +2209,61 @@ hipError_t launch_loglik(const DevProblem &pb, int T, double *part, double *out,
   const int NT = pb.N * T;
   hipLaunchKernelGGL(k_llreduce, dim3((unsigned)((NT + 255) / 256), (unsigned)pb.nchains), dim3(256), 0, st, tiles,
                      NT, (const double *)part, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free scratch (bsg_coef_means_noalloc): a fixed device array for
+// callers that cannot allocate. Each device holds its own copy until the
+// process ends. Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in): scratch_init takes the array's address on the
+// current device once through hipGetSymbolAddress, and kernels receive that
+// address. Every call on a device shares the array, so calls that overlap on
+// one device race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MAX_DEVICES] = {};
+
+hipError_t scratch_init()
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return hipSuccess;
+  void *p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[dev] = static_cast<double *>(p);
+  return hipSuccess;
+}
+
+double *scratch_ptr()
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
+
+// out[c] = sum over b < blocks of part[b P + c], in block order (the order the
+// host adds k_cmeans's partials in bsg_coef_means)
+__global__ __launch_bounds__(64) void k_cmeans_fold(const double *part, int blocks, int P, double *out)
+{
+  const int c = threadIdx.x;
+  if (c >= P) return;
+  double acc = 0.0;
+  for (int b = 0; b < blocks; b++) acc += part[(size_t)b * P + c];
+  out[c] = acc;
+}
+
+hipError_t launch_cmeans_fold(const double *part, int blocks, int P, double *out, hipStream_t st)
+{
+  if (P < 1 || P > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cmeans_fold, dim3(1), dim3(64), 0, st, part, blocks, P, out);
   return hipGetLastError();
 }
 

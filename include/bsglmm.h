@@ -156,6 +156,15 @@ int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *va
 /* alphaMean (covarGPU.cu:563-583): mean of each coefficient plane over the
  * mask, double[P] (FP64 sums). Synchronous. */
 int bsg_coef_means(bsg_ctx *ctx, int chain, double *mean);
+/* No counterpart in the reference: bsg_coef_means for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). Same kernel, same block partials and the same order of sums, so
+ * the same result bit for bit; the partials live in a fixed 64 KiB device
+ * array per GPU instead of memory allocated on first use, and nothing is
+ * allocated. Limit: 256 (P + 1) doubles <= 8192. Every call on one GPU shares
+ * that array (each call overwrites all it reads): calls that overlap on one
+ * GPU race. Synchronous. */
+int bsg_coef_means_noalloc(bsg_ctx *ctx, int chain, double *mean);
 /* compute_predictGPU (covarGPU.cu:1826-1957, MODEL 1): for every subject i and
  * subtype t < ntypes, ll[i*ntypes + t] = sum_j y_ij log(p + 1e-35) +
  * (1 - y_ij) log(1 - p + 1e-35) with p = Phi(b_t,j + b WM_j +

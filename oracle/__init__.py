@@ -63,6 +63,7 @@ def orc():
             "orc_pair_sums": (None, [C.POINTER(Problem), pf, pd]),
             "orc_wm_stat": (None, [C.POINTER(Problem), pf, pd, pd, pd]),
             "orc_wm_draw": (C.c_double, [C.c_double, C.c_double, pu32, C.c_uint32, C.c_uint32]),
+            "orc_cpu_wm_stat": (None, [C.POINTER(Problem), pf, pf, pf, C.c_float, C.c_float, pf, pf]),
             "orc_wishart": (None, [C.c_int, pd, C.c_int, pu32, C.c_uint32, C.c_uint32, pd, pd, pd]),
             "orc_gamma": (C.c_double, [C.c_double, pu32, C.c_uint32, C.c_uint32, C.c_uint32]),
             "orc_chol": (C.c_int, [pd, C.c_int]),
@@ -214,6 +215,16 @@ class Oracle:
         S, W = C.c_double(), C.c_double()
         orc().orc_wm_stat(C.byref(self.pb), _p(sc, C.c_float), _p(z, C.c_double), C.byref(S), C.byref(W))
         return S.value, W.value
+
+    def cpu_wm_stat(self, spat_coef, z, prior_mean=0.0, prior_prec=1e-6):
+        """(mean, var) of b from the reference CPU path's statistic (updateBeta,
+        mcmc.cpp:1099-1136: float Kahan sums in raster order) for float z[N][TOTVOX]."""
+        sc = np.ascontiguousarray(spat_coef, dtype=np.float32)
+        z = np.ascontiguousarray(z, dtype=np.float32)
+        m, v = C.c_float(), C.c_float()
+        orc().orc_cpu_wm_stat(C.byref(self.pb), _p(sc, C.c_float), _p(z, C.c_float), _p(self.wm, C.c_float),
+                              prior_mean, prior_prec, C.byref(m), C.byref(v))
+        return m.value, v.value
 
     def wm_draw(self, S, W, sweep):
         """b ~ N(S/W, 1/W) from the Philox normal of (0, sweep, BWM, chain) (covarGPU.cu:711-717)."""

@@ -377,6 +377,48 @@ void orc_wm_stat(const orc_problem *pb, const float *spat_coef, const double *z,
   *W = (double)N * w;
 }
 
+/* mcmc.cpp:1099-1136 (CPU path, NSUBTYPES covariates then the rest: every
+ * covariate once): voxels in raster order of the padded grid (k, j, i with i
+ * fastest), Kahan-compensated float sums of WM^2 and of WM_j t_j, where t_j is
+ * a plain float sum over subjects of Z_ij - sum_c b_c,j x_c,i. */
+void orc_cpu_wm_stat(const orc_problem *pb, const float *spat_coef, const float *z, const float *wm,
+                     float prior_mean, float prior_prec, float *mean_out, float *var_out)
+{
+  int N = pb->nsubs, P = pb->ncovar, V = pb->totvox, VP = pb->totvoxp;
+  int sx = pb->nrow + 2, sxy = (pb->nrow + 2) * (pb->ncol + 2);
+  float mean = 0.0f, var = 0.0f, c = 0.0f, cm = 0.0f;
+  for (int k = 1; k <= pb->ndepth; k++)
+    for (int j = 1; j <= pb->ncol; j++)
+      for (int i = 1; i <= pb->nrow; i++) {
+        int idx = i + sx * j + sxy * k;
+        int id = pb->host_idx[idx];
+        if (id < 0)
+          continue;
+        int idsc = pb->host_idx_sc[idx];
+        float w = wm[id];
+        float y = w * w - c;
+        float t = var + y;
+        c = (t - var) - y;
+        var = t;
+        float tmp = 0.0f;
+        for (int s = 0; s < N; s++) {
+          tmp += z[(size_t)s * V + id];
+          for (int a = 0; a < P; a++)
+            tmp -= spat_coef[(size_t)a * VP + idsc] * pb->covar[(size_t)a * N + s];
+        }
+        tmp *= w;
+        float ym = tmp - cm;
+        float tm = mean + ym;
+        cm = (tm - mean) - ym;
+        mean = tm;
+      }
+  var *= (float)N;
+  var += prior_prec;
+  var = 1.0f / var;
+  *mean_out = var * (mean + prior_mean * prior_prec);
+  *var_out = var;
+}
+
 /* covarGPU.cu:711-717: var = 1/W, mean = S var, b = mean + sqrt(var) n */
 double orc_wm_draw(double S, double W, const uint32_t key[2], uint32_t sweep, uint32_t chain)
 {

@@ -121,6 +121,13 @@ void orc_wishart(int p, const double *t, int df, const uint32_t key[2], uint32_t
 void orc_wm_stat(const orc_problem *pb, const float *spat_coef, const double *z, double *S, double *W);
 /* b = S/W + sqrt(1/W) n, n from Philox (0, sweep, ORC_TAG_BWM, chain) by Box-Muller */
 double orc_wm_draw(double S, double W, const uint32_t key[2], uint32_t sweep, uint32_t chain);
+/* The reference CPU path's WM-coefficient statistic (updateBeta,
+ * mcmc.cpp:1099-1140, which cannot be compiled without CUDA): raster order
+ * over the padded grid, float Kahan sums, z float[N][TOTVOX], wm float[TOTVOX].
+ * Returns the posterior mean and variance of b before its normal draw
+ * (var = 1 / (N sum WM^2 + prior_prec), mean = var (S + prior_mean prior_prec)). */
+void orc_cpu_wm_stat(const orc_problem *pb, const float *spat_coef, const float *z, const float *wm,
+                     float prior_mean, float prior_prec, float *mean, float *var);
 /* Marsaglia-Tsang gamma(a,1), a >= 1, from Philox stream (row, sweep, chain) */
 double orc_gamma(double a, const uint32_t key[2], uint32_t row, uint32_t sweep, uint32_t chain);
 

@@ -273,6 +273,41 @@ def test_wm_stat_closed_form_and_draw_law():
     assert st.kstest(zs, "norm").pvalue > 1e-3
 
 
+def test_cpu_wm_stat_closed_form_and_fp64_agreement():
+    """orc_cpu_wm_stat (the reference CPU path's updateBeta statistic,
+    mcmc.cpp:1099-1136, float Kahan sums in raster order): with
+    Z = x'b_j + c WM_j the mean is c W / (W + prior_prec) and the variance
+    1 / (W + prior_prec); with random Z it agrees with the FP64 statistic
+    (orc_wm_stat) to float accuracy, and a prior mean shifts the mean by
+    var * prior_mean * prior_prec."""
+    from bsglmm_amd import synth
+    d = synth.generate("c1", nsubs=20, ncovar=3)
+    nx, ny, nz = d["grid"]
+    lay = oracle.layout(d["mask"], nx, ny, nz)
+    rng = np.random.default_rng(8)
+    V, N = lay["totvox"], 20
+    wm = rng.uniform(0.1, 1.0, V).astype(np.float32)
+    O = oracle.Oracle(lay, d["covar"], d["data"], wm=wm, seed=5)
+    beta = rng.standard_normal((3, V)).astype(np.float32)
+    hi, hs = lay["host_idx"], lay["host_idx_sc"]
+    ids = np.flatnonzero(hi >= 0)
+    sc = np.zeros((3, lay["totvoxp"]), np.float32)
+    sc[:, hs[ids]] = beta[:, hi[ids]]
+    W = N * np.sum(wm.astype(np.float64) ** 2)
+    c = 0.81
+    z = ((d["covar"].astype(np.float64).T @ beta.astype(np.float64)) + c * wm[None, :]).astype(np.float32)
+    m, v = O.cpu_wm_stat(sc, z)
+    assert v == pytest.approx(1.0 / (W + 1e-6), rel=1e-6)
+    assert m == pytest.approx(c * W / (W + 1e-6), abs=2e-4)
+    zr = rng.standard_normal((N, V)).astype(np.float32)
+    m, v = O.cpu_wm_stat(sc, zr)
+    S, W2 = O.wm_stat(sc, zr.astype(np.float64))
+    assert m == pytest.approx(S / (W2 + 1e-6), abs=1e-4 * np.sqrt(np.sum(np.abs(zr)) / W2))
+    m2, _ = O.cpu_wm_stat(sc, zr, prior_mean=3.0, prior_prec=0.5)
+    m3, v3 = O.cpu_wm_stat(sc, zr, prior_mean=0.0, prior_prec=0.5)
+    assert m2 - m3 == pytest.approx(v3 * 1.5, rel=1e-4)
+
+
 @pytest.mark.parametrize("mu,y", [(-2.0, 0), (-2.0, 1), (0.3, 0), (0.3, 1), (3.5, 0), (-0.5, 1)])
 def test_accept_or_fallback_sampler_is_the_truncated_normal(mu, y):
     """orc_update_z method 1 (Box-Muller candidate kept inside the interval

@@ -72,6 +72,7 @@ def lib():
         "bsg_sweep_timed": (C.c_int, [vp, C.c_int, pd, pd]),
         "bsg_accum_reset": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "bsg_accum_add": (C.c_int, [vp]),
+        "bsg_accum_shape": (C.c_int, [vp, pi, pi, pi]),
         "bsg_sweep_accumulate": (C.c_int, [vp, C.c_int, C.c_int]),
         "bsg_accum_get": (C.c_int, [vp, C.c_int, pi, pd, pd, pd, pd, pd]),
         "bsg_get_spat_coef": (C.c_int, [vp, C.c_int, pf]),
@@ -85,6 +86,8 @@ def lib():
         "bsg_get_sweep_index": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint32)]),
         "bsg_set_sweep_index": (C.c_int, [vp, C.c_uint32]),
         "bsg_get_fallback_count": (C.c_int, [vp, C.POINTER(C.c_uint32)]),
+        "bsg_checkpoint_write": (C.c_int, [vp, C.c_char_p]),
+        "bsg_checkpoint_read": (C.c_int, [vp, C.c_char_p]),
         "bsg_measure_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, pd, pd]),
         "bsg_measure_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, pd, pd]),
         "bsg_coef_means": (C.c_int, [vp, C.c_int, pd]),
@@ -226,7 +229,9 @@ class Sampler:
 
     def accum_get(self, chain=0):
         """dict(ndraws, mean[P][V], var[P][V], prb[T][V], bwm[V], mcse[P+T][V]) in mask-rank order."""
-        T = getattr(self, "_ntypes", 1)
+        t = C.c_int()
+        _check(lib().bsg_accum_shape(self._h, C.byref(t), None, None))
+        T = t.value
         out = dict(mean=np.empty((self.P, self.V)), var=np.empty((self.P, self.V)),
                    prb=np.empty((max(T, 1), self.V)), bwm=np.empty(self.V),
                    mcse=np.empty((self.P + T, self.V)))
@@ -360,6 +365,15 @@ class Sampler:
         v = C.c_uint32()
         _check(lib().bsg_get_fallback_count(self._h, C.byref(v)))
         return v.value
+
+    def checkpoint(self, path):
+        """Writes the whole chain state (save_iter, mcmc.cpp:1265-1296, exact)."""
+        _check(lib().bsg_checkpoint_write(self._h, os.fsencode(path)))
+
+    def restore(self, path):
+        """Reads a checkpoint of the same problem (restart_iter, mcmc.cpp:1298-1336);
+        the chains then continue bit for bit."""
+        _check(lib().bsg_checkpoint_read(self._h, os.fsencode(path)))
 
     def close(self):
         if self._h:

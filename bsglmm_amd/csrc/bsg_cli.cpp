@@ -39,6 +39,63 @@ void usage(const char *prog)
          IMAGE_DIR, DEFAULT_WM);
   printf("  MaxIter - Number of iterations (defaults to 100,000)\n");
   printf("  BurnIn  - Number of burn-in iterations (defaults to 50,000)\n");
+  printf("Options after these (this build):\n");
+  printf("  -checkpoint K  every K iterations write last_state.bsg and last_run.dat (exact state) and\n");
+  printf("                 last_parms.dat / last_spatCoef.dat (the reference's save_iter files)\n");
+  printf("  -stop S        stop after iteration S, writing the checkpoint\n");
+  printf("  -restart       continue the run saved in last_state.bsg / last_run.dat\n");
+}
+
+// ---- checkpoints of a run (save_iter / restart_iter, mcmc.cpp:1265-1336) ----
+// last_state.bsg holds the library's chain state (bsg_checkpoint_write);
+// last_run.dat this program's own counters: the next iteration, the number of
+// saves, the DIC and Qhat sums, the run's iteration counts and the seed.dat
+// triple it will write at the end.
+const char *CKPT_STATE = "last_state.bsg";
+const char *CKPT_RUN = "last_run.dat";
+const char RUN_MAGIC[8] = {'B', 'S', 'G', 'R', 'U', 'N', '0', '1'};
+
+struct RunState {
+  int next_iter = 0, totsav = 0, maxiter = 0, burnin = 0, nsubs = 0, ntypes = 0;
+  double ED = 0.0;
+  unsigned long seed_out[3] = {0, 0, 0};
+  std::vector<std::vector<double>> Qhat;
+};
+
+bool write_run_state(const RunState &st)
+{
+  FILE *f = fopen(CKPT_RUN, "wb");
+  if (!f) return false;
+  int32_t hdr[6] = {st.next_iter, st.totsav, st.maxiter, st.burnin, st.nsubs, st.ntypes};
+  bool ok = fwrite(RUN_MAGIC, 1, 8, f) == 8 && fwrite(hdr, sizeof(hdr), 1, f) == 1 &&
+            fwrite(&st.ED, sizeof(double), 1, f) == 1 && fwrite(st.seed_out, sizeof(st.seed_out), 1, f) == 1;
+  for (int i = 0; ok && i < st.nsubs; i++)
+    ok = fwrite(st.Qhat[i].data(), sizeof(double), st.ntypes, f) == (size_t)st.ntypes;
+  return (fclose(f) == 0) && ok;
+}
+
+bool read_run_state(RunState &st)
+{
+  FILE *f = fopen(CKPT_RUN, "rb");
+  if (!f) return false;
+  char magic[8];
+  int32_t hdr[6];
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, RUN_MAGIC, 8) == 0 && fread(hdr, sizeof(hdr), 1, f) == 1 &&
+            fread(&st.ED, sizeof(double), 1, f) == 1 && fread(st.seed_out, sizeof(st.seed_out), 1, f) == 1;
+  if (ok) {
+    st.next_iter = hdr[0];
+    st.totsav = hdr[1];
+    st.maxiter = hdr[2];
+    st.burnin = hdr[3];
+    st.nsubs = hdr[4];
+    st.ntypes = hdr[5];
+    ok = st.nsubs > 0 && st.ntypes > 0 && st.nsubs <= (1 << 24) && st.ntypes <= BSG_MAX_COVAR;
+  }
+  if (ok) st.Qhat.assign(st.nsubs, std::vector<double>(st.ntypes, 0.0));
+  for (int i = 0; ok && i < st.nsubs; i++)
+    ok = fread(st.Qhat[i].data(), sizeof(double), st.ntypes, f) == (size_t)st.ntypes;
+  fclose(f);
+  return ok;
 }
 
 // Failures of the library print its message and end the program with the
@@ -310,6 +367,28 @@ void write_qhat(const Run &r, const char *path, const std::vector<std::vector<do
 
 extern "C" int bsg_bincar_main(int argc, char **argv)
 {
+  // positional arguments (main.cu:82-112), then this build's options
+  int npos = 1;
+  while (npos < argc && !(argv[npos][0] == '-' && argv[npos][1] >= 'a' && argv[npos][1] <= 'z')) npos++;
+  int ckpt_every = 0, stop_at = -1;
+  bool restart = false;
+  for (int k = npos; k < argc; k++) {
+    if (strcmp(argv[k], "-checkpoint") == 0 && k + 1 < argc) {
+      ckpt_every = atoi(argv[++k]);
+    } else if (strcmp(argv[k], "-stop") == 0 && k + 1 < argc) {
+      stop_at = atoi(argv[++k]);
+    } else if (strcmp(argv[k], "-restart") == 0) {
+      restart = true;
+    } else {
+      usage(argv[0]);
+      return 1;
+    }
+  }
+  if (ckpt_every < 0) {
+    usage(argv[0]);
+    return 1;
+  }
+  argc = npos;
   if (argc != 7 && argc != 9) {
     usage(argv[0]);
     return 1;
@@ -366,6 +445,9 @@ extern "C" int bsg_bincar_main(int argc, char **argv)
 
   bsg_layout *lay = nullptr;
   LIBCHK(bsg_layout_create(r.msk.data(), r.nrow, r.ncol, r.ndepth, &lay), 1);
+  int lcounts[4];
+  LIBCHK(bsg_layout_counts(lay, lcounts), 1);
+  const int totvoxp = lcounts[1];
   bsg_opts o;
   bsg_opts_default(&o);
   o.seed = bsg_devseed(seed);  // main.cu:413
@@ -390,12 +472,74 @@ extern "C" int bsg_bincar_main(int argc, char **argv)
   std::vector<float> precf((size_t)P * P);
   double ED = 0.0;
   int totsav = 0;
-  const auto t_start = std::chrono::steady_clock::now();
   int iter = 0;
+  if (restart) {  // restart_iter (mcmc.cpp:1298-1336), exact
+    RunState st;
+    if (!read_run_state(st) || st.nsubs != N || st.ntypes != T) {
+      printf("-restart: %s is missing or is not a checkpoint of this study\n", CKPT_RUN);
+      return 1;
+    }
+    if (st.maxiter != maxiter || st.burnin != burnin) {
+      printf("-restart: the checkpoint is of a run with MaxIter %d and BurnIn %d\n", st.maxiter, st.burnin);
+      return 1;
+    }
+    LIBCHK(bsg_checkpoint_read(ctx, CKPT_STATE), 1);
+    iter = st.next_iter;
+    totsav = st.totsav;
+    ED = st.ED;
+    Qhat = st.Qhat;
+    for (int k = 0; k < 3; k++) seed[k] = st.seed_out[k];
+    printf("RESTART at iteration %d (%d saves so far)\n", iter, totsav);
+  }
+  // writes the checkpoint files of the run, the next iteration being `next_iter`
+  auto checkpoint = [&](int next_iter) -> int {
+    LIBCHK(bsg_checkpoint_write(ctx, CKPT_STATE), 1);
+    RunState st;
+    st.next_iter = next_iter;
+    st.totsav = totsav;
+    st.maxiter = maxiter;
+    st.burnin = burnin;
+    st.nsubs = N;
+    st.ntypes = T;
+    st.ED = ED;
+    for (int k = 0; k < 3; k++) st.seed_out[k] = seed[k];
+    st.Qhat = Qhat;
+    if (!write_run_state(st)) {
+      printf("cannot write %s\n", CKPT_RUN);
+      return 1;
+    }
+    // the reference's own files (save_iter, mcmc.cpp:1265-1286): b, the plane
+    // means, Lambda as text; the coefficients as float[NCOV][TOTVOXp]
+    float b = 0.0f;
+    LIBCHK(bsg_get_wm_coef(ctx, 0, &b), 1);
+    LIBCHK(bsg_coef_means(ctx, 0, means.data()), 1);
+    LIBCHK(bsg_get_prec(ctx, 0, precf.data()), 1);
+    FILE *fp = fopen("last_parms.dat", "w");
+    if (fp) {
+      fprintf(fp, "%f ", b);
+      for (int c = 0; c < P; c++) fprintf(fp, "%f ", means[c]);
+      for (int a = 0; a < P; a++)
+        for (int c = 0; c < P; c++) fprintf(fp, "%f ", precf[c + a * P]);
+      fprintf(fp, "\n");
+      fclose(fp);
+    }
+    std::vector<float> sc((size_t)P * totvoxp);
+    LIBCHK(bsg_get_spat_coef(ctx, 0, sc.data()), 1);
+    fp = fopen("last_spatCoef.dat", "wb");
+    if (fp) {
+      fwrite(sc.data(), sizeof(float), sc.size(), fp);
+      fclose(fp);
+    }
+    return 0;
+  };
+  const auto t_start = std::chrono::steady_clock::now();
+  const int iter0 = iter;
+  auto is_ckpt = [&](int it) { return (ckpt_every > 0 && it > 0 && it % ckpt_every == 0) || it == stop_at; };
   while (iter <= maxiter) {
-    // iterations up to the next report or save run as one batch of sweeps
+    // iterations up to the next report, save or checkpoint run as one batch of sweeps
     int next = iter;
-    while (next <= maxiter && next % prn_iter != 0 && !(next > burnin && save_iter > 0 && next % save_iter == 0))
+    while (next <= maxiter && next % prn_iter != 0 && !(next > burnin && save_iter > 0 && next % save_iter == 0) &&
+           !is_ckpt(next))
       next++;
     if (next > iter) {
       LIBCHK(bsg_sweep(ctx, next - iter), 1);
@@ -459,12 +603,22 @@ extern "C" int bsg_bincar_main(int argc, char **argv)
         printf("Time to compute_predictGPU: %f ms\n\n",
                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }
+    if (is_ckpt(iter)) {
+      LIBCHK(bsg_synchronize(ctx), 1);
+      if (checkpoint(iter + 1)) return 1;
+      if (iter == stop_at) {
+        printf("stopped after iteration %d; checkpoint in %s and %s (continue with -restart)\n", iter, CKPT_STATE,
+               CKPT_RUN);
+        bsg_destroy(ctx);
+        return 0;
+      }
+    }
     iter++;
   }
   LIBCHK(bsg_synchronize(ctx), 1);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   printf("TOTSAV = %d\n", totsav);
-  printf("%d sweeps in %.2f s (%.1f sweeps/s)\n", maxiter + 1, secs, (maxiter + 1) / secs);
+  printf("%d sweeps in %.2f s (%.1f sweeps/s)\n", maxiter + 1 - iter0, secs, (maxiter + 1 - iter0) / secs);
 
   write_qhat(r, "Qhat.dat", Qhat);
   {

@@ -886,6 +886,16 @@ int bsg_accum_reset(bsg_ctx *ctx, int ntypes, int nbatches, int batch_len)
   return BSG_OK;
 }
 
+int bsg_accum_shape(bsg_ctx *ctx, int *ntypes, int *nbatches, int *batch_len)
+{
+  CTXCHK(ctx);
+  const bool on = ctx->acc.sum != nullptr;
+  if (ntypes) *ntypes = on ? ctx->acc.ntypes : 0;
+  if (nbatches) *nbatches = on ? ctx->acc.nbatches : 0;
+  if (batch_len) *batch_len = on ? ctx->acc_batch_len : 0;
+  return BSG_OK;
+}
+
 int bsg_accum_add(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
@@ -1193,6 +1203,207 @@ int bsg_set_sweep_index(bsg_ctx *ctx, uint32_t sweep)
 {
   CTXCHK(ctx);
   ctx->sweep = sweep;
+  return BSG_OK;
+}
+
+// ---- checkpoint / resume (the reference's save_iter / restart_iter,
+// mcmc.cpp:1265-1336, made exact) ------------------------------------------
+namespace {
+
+constexpr char CKPT_MAGIC[8] = {'B', 'S', 'G', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t CKPT_VERSION = 1;
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  int32_t V, N, P, C, n0, lat_g, df, acc_present;
+  uint32_t key0, key1, sweep, pad0;
+  float trunc_eps, pivot_min;
+  uint32_t flags[4];
+  int32_t ntypes, nbatches, batch_len, acc_count;
+  uint64_t layout_hash;
+};
+
+// FNV-1a, 64 bit
+uint64_t fnv(uint64_t h, const void *p, size_t n)
+{
+  const unsigned char *c = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < n; i++) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+uint64_t layout_hash(const bsg::Layout &L)
+{
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(h, L.rank_of_sorted.data(), sizeof(int) * L.rank_of_sorted.size());
+  h = fnv(h, L.nbr6.data(), sizeof(int) * L.nbr6.size());
+  return h;
+}
+
+// the device arrays of the chain state, in file order
+struct Part {
+  void *dev;
+  size_t bytes;
+};
+
+std::vector<Part> state_parts(bsg_ctx *ctx, bool with_acc)
+{
+  const size_t C = ctx->C, V = ctx->L.totvox, P = ctx->P;
+  const bsg::DevProblem &pb = ctx->pb;
+  std::vector<Part> v = {{pb.beta, 4 * C * V * P}, {pb.M, 4 * C * V * P},     {pb.lam, 4 * C * P * P},
+                         {pb.lam64, 8 * C * P * P}, {pb.tsum, 8 * C * P * P}, {pb.bwm, 4 * C}};
+  if (with_acc) {
+    const bsg::Accum &a = ctx->acc;
+    const size_t T = a.ntypes, NB = a.nbatches;
+    v.push_back({a.sum, 8 * C * V * P});
+    v.push_back({a.sq, 8 * C * V * P});
+    if (T) v.push_back({a.prb, 8 * C * V * T});
+    v.push_back({a.bwm, 8 * C * V});
+    v.push_back({a.batch, 4 * C * NB * V * (P + T)});
+  }
+  return v;
+}
+
+}  // namespace
+
+int bsg_checkpoint_write(bsg_ctx *ctx, const char *path)
+{
+  CTXCHK(ctx);
+  if (!path) {
+    set_error("bsg_checkpoint_write: NULL path");
+    return BSG_ERR_ARG;
+  }
+  if (ctx->tail_pending) {
+    set_error("bsg_checkpoint_write: a latent step's X'Z is still in pieces; finish the sweep first");
+    return BSG_ERR_ARG;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream2));
+  CkptHeader h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, CKPT_MAGIC, 8);
+  h.version = CKPT_VERSION;
+  h.V = ctx->L.totvox;
+  h.N = ctx->N;
+  h.P = ctx->P;
+  h.C = ctx->C;
+  h.n0 = ctx->L.n[0];
+  h.lat_g = ctx->pb.lat_g;
+  h.df = ctx->pb.df;
+  h.key0 = ctx->pb.key0;
+  h.key1 = ctx->pb.key1;
+  h.sweep = ctx->sweep;
+  h.trunc_eps = ctx->pb.trunc_eps;
+  h.pivot_min = ctx->pb.pivot_min;
+  HIPCHK(hipMemcpy(h.flags, ctx->pb.flags, sizeof(h.flags), hipMemcpyDeviceToHost));
+  h.acc_present = ctx->acc.sum ? 1 : 0;
+  if (h.acc_present) {
+    h.ntypes = ctx->acc.ntypes;
+    h.nbatches = ctx->acc.nbatches;
+    h.batch_len = ctx->acc_batch_len;
+    h.acc_count = ctx->acc_count;
+  }
+  h.layout_hash = layout_hash(ctx->L);
+  FILE *f = fopen(path, "wb");
+  if (!f) {
+    set_error(std::string("bsg_checkpoint_write: cannot open ") + path);
+    return BSG_ERR_ARG;
+  }
+  uint64_t sum = fnv(1469598103934665603ull, &h, sizeof(h));
+  bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+  std::vector<char> buf;
+  for (const Part &q : state_parts(ctx, h.acc_present != 0)) {
+    if (!ok) break;
+    buf.resize(q.bytes);
+    if (hipMemcpy(buf.data(), q.dev, q.bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      fclose(f);
+      set_error("bsg_checkpoint_write: device read failed");
+      return BSG_ERR_HIP;
+    }
+    sum = fnv(sum, buf.data(), q.bytes);
+    ok = fwrite(buf.data(), 1, q.bytes, f) == q.bytes;
+  }
+  ok = ok && fwrite(&sum, sizeof(sum), 1, f) == 1;
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) {
+    set_error(std::string("bsg_checkpoint_write: write to ") + path + " failed");
+    return BSG_ERR_ARG;
+  }
+  return BSG_OK;
+}
+
+int bsg_checkpoint_read(bsg_ctx *ctx, const char *path)
+{
+  CTXCHK(ctx);
+  if (!path) {
+    set_error("bsg_checkpoint_read: NULL path");
+    return BSG_ERR_ARG;
+  }
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    set_error(std::string("bsg_checkpoint_read: cannot open ") + path);
+    return BSG_ERR_ARG;
+  }
+  CkptHeader h;
+  auto fail = [&](const std::string &msg) {
+    fclose(f);
+    set_error("bsg_checkpoint_read: " + msg);
+    return BSG_ERR_ARG;
+  };
+  if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != CKPT_VERSION)
+    return fail(std::string(path) + " is not a checkpoint of this version");
+  if (h.V != ctx->L.totvox || h.N != ctx->N || h.P != ctx->P || h.C != ctx->C || h.n0 != ctx->L.n[0] ||
+      h.layout_hash != layout_hash(ctx->L))
+    return fail("the checkpoint is of another problem (mask, N, P or chain count differ)");
+  if (h.df != ctx->pb.df || h.trunc_eps != ctx->pb.trunc_eps || h.pivot_min != ctx->pb.pivot_min)
+    return fail("the checkpoint was written under other sampler options (Wishart df, truncation or pivot_min)");
+  if (h.acc_present && (h.ntypes < 0 || h.ntypes > h.P || h.nbatches < 1 || h.nbatches > 1024 || h.batch_len < 1 ||
+                        h.acc_count < 0))
+    return fail("bad accumulator header");
+  if (h.lat_g != 0 && h.lat_g != 1 && h.lat_g != 2 && h.lat_g != 4 && h.lat_g != 8) return fail("bad lane split");
+  // read everything, check the sum, then touch the context
+  uint64_t sum = fnv(1469598103934665603ull, &h, sizeof(h));
+  std::vector<std::vector<char>> data;
+  {
+    const size_t C = h.C, V = h.V, P = h.P, T = h.ntypes, NB = h.nbatches;
+    std::vector<size_t> sizes = {4 * C * V * P, 4 * C * V * P, 4 * C * P * P, 8 * C * P * P, 8 * C * P * P, 4 * C};
+    if (h.acc_present) {
+      sizes.insert(sizes.end(), {8 * C * V * P, 8 * C * V * P});
+      if (T) sizes.push_back(8 * C * V * T);
+      sizes.insert(sizes.end(), {8 * C * V, 4 * C * NB * V * (P + T)});
+    }
+    for (size_t n : sizes) {
+      data.emplace_back(n);
+      if (fread(data.back().data(), 1, n, f) != n) return fail("the file is truncated");
+      sum = fnv(sum, data.back().data(), n);
+    }
+  }
+  uint64_t stored = 0;
+  if (fread(&stored, sizeof(stored), 1, f) != 1 || stored != sum) return fail("checksum mismatch");
+  fclose(f);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream2));
+  if (h.acc_present) {
+    const int rc = bsg_accum_reset(ctx, h.ntypes, h.nbatches, h.batch_len);
+    if (rc != BSG_OK) return rc;
+  } else {
+    ctx->free_accum();
+  }
+  const std::vector<Part> parts = state_parts(ctx, h.acc_present != 0);
+  for (size_t k = 0; k < parts.size(); k++)
+    HIPCHK(hipMemcpy(parts[k].dev, data[k].data(), parts[k].bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->pb.flags, h.flags, sizeof(h.flags), hipMemcpyHostToDevice));
+  drop_graph(ctx);  // a captured graph holds the key and the split by value
+  ctx->pb.key0 = h.key0;
+  ctx->pb.key1 = h.key1;
+  ctx->pb.lat_g = h.lat_g;
+  ctx->sweep = h.sweep;
+  ctx->acc_count = h.acc_present ? h.acc_count : 0;
+  ctx->tail_pending = 0;
+  ctx->bstat_parts = 0;
   return BSG_OK;
 }
 

@@ -140,6 +140,9 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6]);
  * batches of batch_len draws for batch-means Monte-Carlo errors (draws past
  * nbatches * batch_len add to the sums but to no batch). */
 int bsg_accum_reset(bsg_ctx *ctx, int ntypes, int nbatches, int batch_len);
+/* the accumulators' shape as bsg_accum_reset (or a checkpoint read) set it;
+ * zeros when none are allocated. Any pointer may be NULL. */
+int bsg_accum_shape(bsg_ctx *ctx, int *ntypes, int *nbatches, int *batch_len);
 /* adds the current draw of every chain */
 int bsg_accum_add(bsg_ctx *ctx);
 /* nsweeps sweeps, adding every save_every-th draw (mcmc.cpp:584: SAVE_ITER) */
@@ -187,6 +190,21 @@ int bsg_get_pair_sums(bsg_ctx *ctx, int chain, double *t);    /* double[P][P], l
 int bsg_get_sweep_index(bsg_ctx *ctx, int chain, uint32_t *sweep);
 int bsg_set_sweep_index(bsg_ctx *ctx, uint32_t sweep);        /* all chains */
 int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count);     /* FP64 Cholesky refactors */
+
+/* ---- checkpoint / resume: save_iter / restart_iter (mcmc.cpp:1265-1336,
+ * RESTART main.cu:119, 152), exact. The file holds the whole chain state of
+ * every chain: coefficients, X'Z, Lambda (FP32 and FP64), pair sums, b, the
+ * sweep index, the Philox key, the lane split, the kernel-side counts and, if
+ * allocated, the posterior accumulators. Z is not saved (the reference saves
+ * it, mcmc.cpp:1288-1294): every draw of Z depends only on the current
+ * coefficients, so the next sweep redraws it. A context created for the same
+ * problem (mask, N, P, chains, Wishart df, truncation, pivot_min) and read
+ * from the file continues the chains bit for bit: k sweeps, a checkpoint and
+ * n - k sweeps in a new process equal n sweeps in one. Write only between
+ * full sweeps; read replaces the key with the saved one. A checksum guards
+ * the file. */
+int bsg_checkpoint_write(bsg_ctx *ctx, const char *path);
+int bsg_checkpoint_read(bsg_ctx *ctx, const char *path);
 
 /* ---- tile shapes and batches of analyses (no counterpart in the reference:
  * it runs one analysis per process) ------------------------------------- */

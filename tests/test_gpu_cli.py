@@ -173,3 +173,41 @@ def test_loglik_and_coef_means_against_fp64():
     # closed form: b = 0 and all coefficients 0 give p = 1/2 everywhere
     S2 = Sampler(L, d["covar"], d["data"], seed=3)
     assert np.allclose(S2.loglik(2), L.totvox * np.log(0.5), rtol=1e-6)
+
+
+def test_bincar_checkpoint_and_restart_equal_one_run(tmp_path):
+    """A run stopped after iteration 4321 (-checkpoint 1000 -stop 4321) and
+    continued with -restart writes every output file the uninterrupted run
+    writes, with the same contents: maps, Qhat, DIC and seed.dat."""
+    print("case BinCar c1 N=40 NTypes=2 NCov=3 WM 10200/200: one run vs stop at 4321 + restart", flush=True)
+    one, two = tmp_path / "one", tmp_path / "two"
+    one.mkdir()
+    two.mkdir()
+    study(one)
+    study(two)
+    args = [BINCAR, "2", "3", "1", "design.txt", "1", "wm.nii", "10200", "200"]
+    r = subprocess.run(args, cwd=one, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-800:]
+    r = subprocess.run(args + ["-checkpoint", "1000", "-stop", "4321"], cwd=two, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-800:]
+    assert "stopped after iteration 4321" in r.stdout
+    assert not (two / "DIC.dat").exists()
+    for f in ("last_state.bsg", "last_run.dat", "last_parms.dat", "last_spatCoef.dat"):
+        assert (two / f).exists(), f
+    npar = len((two / "last_parms.dat").read_text().split())
+    assert npar == 1 + 3 + 9  # b, the plane means, Lambda (save_iter, mcmc.cpp:1265-1286)
+    r = subprocess.run(args + ["-restart"], cwd=two, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-800:]
+    assert "RESTART at iteration 4322" in r.stdout and "TOTSAV = 10000" in r.stdout
+    outs = sorted(p.name for p in one.iterdir() if p.is_file() and p.suffix in (".gz", ".dat"))
+    assert {"DIC.dat", "Qhat.dat", "seed.dat", "spatCoef_age.nii.gz", "bWM.nii.gz"} <= set(outs)
+    for name in outs:
+        a, b = one / name, two / name
+        if name.endswith(".nii.gz"):
+            assert np.array_equal(B.nifti_read(str(a)), B.nifti_read(str(b))), name
+        else:
+            assert a.read_text() == b.read_text(), name
+    # a restart needs the same run lengths
+    r = subprocess.run(args[:-2] + ["10400", "200", "-restart"], cwd=two, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "MaxIter" in r.stdout

@@ -309,7 +309,7 @@ class Sampler:
     def overlap(self):
         """True when the next sweep runs overlapped: two launches per sweep,
         each colour's coefficient update beside the other colour's latent
-        draws (four lanes per voxel, no WM image, P <= 6). The chains are
+        draws (four lanes per voxel, no WM image). The chains are
         the same either way. Assign False to run the serial order."""
         on = C.c_int()
         _check(lib().bsg_get_overlap(self._h, C.byref(on)))

@@ -69,8 +69,9 @@ __device__ __forceinline__ double u01d(uint32_t x)
 // Deterministic block reductions in FP64 (fixed order, so sums repeat bit for bit).
 // ---------------------------------------------------------------------------
 // Block sum of NE doubles in fixed order; thread e < NE of the block gets entry e.
+// (count < NE: only entries e < count are written)
 template <int NE>
-__device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst)
+__device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst, int count = NE)
 {
   __shared__ double red[4][NE];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -81,27 +82,35 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst)
     if (lane == 0) red[wv][e] = v;
   }
   __syncthreads();
-  if (threadIdx.x < NE) {
+  if (threadIdx.x < NE && (int)threadIdx.x < count) {
     const int e = threadIdx.x;
     dst[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
   }
 }
 
 // Sum of the per-block partials in a fixed order: thread t adds blocks
-// t, t + 256, ... of every entry, then one block reduction of all NE entries
-// (block_sum_store). T (shared memory) holds the sums for every thread on return.
+// t, t + 256, ... of an entry, then a block reduction (block_sum_store), for
+// all entries at once up to P = 6 and in chunks of 6 above (each entry's sum
+// is formed alone, so the chunking changes no result; it keeps the
+// accumulators within a latent kernel's register budget at P = 7, 8). T (shared memory) holds the sums for
+// every thread on return.
 template <int NE>
 __device__ void sum_partials(int nparts, const double *part, double *T)
 {
-  double acc[NE];
+  constexpr int CH = NE <= 21 ? NE : 6;  // one chunk up to P = 6
+#pragma unroll 1
+  for (int e0 = 0; e0 < NE; e0 += CH) {
+    double acc[CH];
 #pragma unroll
-  for (int e = 0; e < NE; e++) acc[e] = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += 256) {
+    for (int e = 0; e < CH; e++) acc[e] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += 256) {
 #pragma unroll
-    for (int e = 0; e < NE; e++) acc[e] += part[(size_t)b * NE + e];
+      for (int e = 0; e < CH; e++)
+        if (e0 + e < NE) acc[e] += part[(size_t)b * NE + e0 + e];
+    }
+    block_sum_store<CH>(acc, T + e0, NE - e0);
+    __syncthreads();
   }
-  block_sum_store<NE>(acc, T);
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -903,8 +912,8 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_vl_mfm(DevProble
 {
   const uint32_t base = *pb.sweep_base;
   const int chain = blockIdx.y;
-  // P <= 6: the draw's registers spill only in its own code under the bound
-  if (P <= 6 && wish && blockIdx.x == 0) {
+  // the draw keeps its FP64 matrices in LDS, so it fits the latent kernel's registers
+  if (wish && blockIdx.x == 0) {
     constexpr int NE = P * (P + 1) / 2;
     wishart_block<P>(pb, chain, (pb.V - pb.n0 + 255) / 256, pb.partials + (size_t)chain * pb.max_parts * NE,
                      wsweep + base);
@@ -1124,9 +1133,15 @@ __device__ double gamma_mt(double a, uint32_t row, uint32_t sweep, uint32_t chai
   return d;
 }
 
+// Loops of the Wishart algebra: unrolled (matrices in registers) up to P = 6,
+// rolled above (matrices in LDS, few registers).
+template <int P>
+constexpr int wish_unroll() { return P <= 6 ? 8 : 1; }
+
 template <int P>
 __device__ void chol_lower(double (&a)[P][P])
 {
+#pragma unroll wish_unroll<P>()
   for (int j = 0; j < P; j++) {
     double d = a[j][j];
     for (int k = 0; k < j; k++) d -= a[j][k] * a[j][k];
@@ -1159,6 +1174,62 @@ __device__ __forceinline__ double bartlett_entry(const DevProblem &pb, int chain
   return val;
 }
 
+// S = (T + I)^-1, its Cholesky factor L, B = L A and Lambda = B B' in FP64
+// (cholesky.cpp:245-286 steps), on one lane; T packed lower triangle.
+template <int P>
+__device__ __forceinline__ void wishart_algebra(const DevProblem &pb, int chain, const double *T,
+                                                const double (&A)[P][P], bool tagged, double (&S)[P][P],
+                                                double (&inv)[P][P], double (&B)[P][P], double (&y)[P])
+{
+  for (int a = 0, e = 0; a < P; a++)
+    for (int c = 0; c <= a; c++, e++) {
+      S[a][c] = T[e];
+      S[c][a] = T[e];
+      pb.tsum[(size_t)chain * P * P + a * P + c] = T[e];
+      pb.tsum[(size_t)chain * P * P + c * P + a] = T[e];
+    }
+  for (int a = 0; a < P; a++) S[a][a] += 1.0;
+  chol_lower<P>(S);
+#pragma unroll wish_unroll<P>()
+  for (int col = 0; col < P; col++) {
+    for (int i = 0; i < P; i++) {
+      double v = (i == col) ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) v -= S[i][k] * y[k];
+      y[i] = v / S[i][i];
+    }
+    for (int i = P - 1; i >= 0; i--) {
+      double v = y[i];
+      for (int k = i + 1; k < P; k++) v -= S[k][i] * inv[k][col];
+      inv[i][col] = v / S[i][i];
+    }
+  }
+  chol_lower<P>(inv);
+#pragma unroll wish_unroll<P>()
+  for (int i = 0; i < P; i++)
+    for (int j = 0; j < P; j++) {
+      double v = 0.0;
+      if (j <= i)
+        for (int k = j; k <= i; k++) v += inv[i][k] * A[k][j];
+      B[i][j] = v;
+    }
+  float *lam = pb.lam + (size_t)chain * P * P;
+  double *lam64 = pb.lam64 + (size_t)chain * P * P;
+#pragma unroll wish_unroll<P>()
+  for (int i = 0; i < P; i++)
+    for (int j = 0; j < P; j++) {
+      const int K = i < j ? i : j;
+      double v = 0.0;
+      for (int k = 0; k <= K; k++) v += B[i][k] * B[j][k];
+      lam64[i * P + j] = v;
+      lam[i * P + j] = (float)v;
+      if (tagged)
+        __hip_atomic_store(pb.lamtag + (size_t)chain * P * P + i * P + j,
+                           ((unsigned long long)(pb.lam_tag ? pb.lam_tag : 1) << 32) |
+                               (unsigned long long)__float_as_uint((float)v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Block of 256 threads: partial sums, Bartlett variates, FP64 algebra.
 // tagged: Lambda is also written to lamtag, each entry with tag 1 beside its
 // bits, for blocks of the same launch that wait for it (k_overlap).
@@ -1179,54 +1250,16 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   if (lane != 0) return;
-  double S[P][P];
-  for (int a = 0, e = 0; a < P; a++)
-    for (int c = 0; c <= a; c++, e++) {
-      S[a][c] = T[e];
-      S[c][a] = T[e];
-      pb.tsum[(size_t)chain * P * P + a * P + c] = T[e];
-      pb.tsum[(size_t)chain * P * P + c * P + a] = T[e];
-    }
-  for (int a = 0; a < P; a++) S[a][a] += 1.0;
-  chol_lower<P>(S);
-  double inv[P][P];
-  for (int col = 0; col < P; col++) {
-    double y[P];
-    for (int i = 0; i < P; i++) {
-      double v = (i == col) ? 1.0 : 0.0;
-      for (int k = 0; k < i; k++) v -= S[i][k] * y[k];
-      y[i] = v / S[i][i];
-    }
-    for (int i = P - 1; i >= 0; i--) {
-      double v = y[i];
-      for (int k = i + 1; k < P; k++) v -= S[k][i] * inv[k][col];
-      inv[i][col] = v / S[i][i];
-    }
+  // the FP64 algebra runs on one lane: in registers up to P = 6, with its
+  // matrices in LDS above (few registers, so that the kernels that carry this
+  // draw keep their register budget)
+  if constexpr (P <= 6) {
+    double S[P][P], inv[P][P], B[P][P], y[P];
+    wishart_algebra<P>(pb, chain, T, A, tagged, S, inv, B, y);
+  } else {
+    __shared__ double S[P][P], inv[P][P], B[P][P], y[P];
+    wishart_algebra<P>(pb, chain, T, A, tagged, S, inv, B, y);
   }
-  chol_lower<P>(inv);
-  double B[P][P];
-  for (int i = 0; i < P; i++)
-    for (int j = 0; j < P; j++) {
-      double v = 0.0;
-      if (j <= i)
-        for (int k = j; k <= i; k++) v += inv[i][k] * A[k][j];
-      B[i][j] = v;
-    }
-  float *lam = pb.lam + (size_t)chain * P * P;
-  double *lam64 = pb.lam64 + (size_t)chain * P * P;
-  for (int i = 0; i < P; i++)
-    for (int j = 0; j < P; j++) {
-      const int K = i < j ? i : j;
-      double v = 0.0;
-      for (int k = 0; k <= K; k++) v += B[i][k] * B[j][k];
-      lam64[i * P + j] = v;
-      lam[i * P + j] = (float)v;
-      if (tagged)
-        __hip_atomic_store(pb.lamtag + (size_t)chain * P * P + i * P + j,
-                           ((unsigned long long)(pb.lam_tag ? pb.lam_tag : 1) << 32) |
-                               (unsigned long long)__float_as_uint((float)v),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // SIDE = false: after k_pairs (PAIR_BLOCKS partials). SIDE = true: after the
@@ -1380,9 +1413,9 @@ __device__ __forceinline__ bool ul_draw(const T (&Q)[P][P], const T (&r)[P], con
 }
 
 // FP64 refactorisation of one voxel's update (its FP32 pivot fell below
-// pivot_min): inputs and result by value, in a function of its own, so that
-// its registers do not count against the FP32 path that nearly every voxel
-// takes. ok = 0: the precision is not positive definite.
+// pivot_min), P <= 6: inputs and result by value, in a function of its own,
+// so that its registers do not count against the FP32 path that nearly every
+// voxel takes. ok = 0: the precision is not positive definite.
 template <int P>
 struct CoefIn {
   float m[P], sv[P], z[P];
@@ -1414,6 +1447,93 @@ __device__ __forceinline__ CoefOut<P> coef_fp64(CoefIn<P> in, const float *lam, 
 #pragma unroll
   for (int c = 0; c < P; c++) o.out[c] = (float)od[c];
   return o;
+}
+
+// FP64 refactorisation at P = 7, 8: the arithmetic of ul_draw<P, double>
+// (the same operations in the same order), with every matrix and vector in an
+// LDS area of the wave, one lane at a time. Such voxels are rare; keeping this
+// path out of registers keeps the FP32 path (and the latent draws that share
+// its kernels) within their register budget. ok = false: the precision is not
+// positive definite.
+template <int P>
+constexpr int fp64_area() { return 2 * P * P + 6 * P; }  // Q, U, dinv, r, z, w, out
+
+template <int P>
+__device__ __noinline__ bool ul_draw_f64_lds(double *a)
+{
+  double *Q = a, *U = a + P * P, *dinv = U + P * P, *r = dinv + P, *z = r + P, *w = z + P, *out = w + P;
+#pragma unroll 1
+  for (int k = P - 1; k >= 0; k--) {
+    double d = Q[k * P + k];
+    for (int j = k + 1; j < P; j++) d -= U[k * P + j] * U[k * P + j];
+    if (!(d >= 0.0)) return false;
+    const double ukk = sqrt(d);
+    const double inv = 1.0 / ukk;
+    U[k * P + k] = ukk;
+    dinv[k] = inv;
+#pragma unroll 1
+    for (int i = 0; i < k; i++) {
+      double v = Q[i * P + k];
+      for (int j = k + 1; j < P; j++) v -= U[i * P + j] * U[k * P + j];
+      U[i * P + k] = v * dinv[k];
+    }
+  }
+#pragma unroll 1
+  for (int i = P - 1; i >= 0; i--) {
+    double v = r[i];
+    for (int j = i + 1; j < P; j++) v -= U[i * P + j] * w[j];
+    w[i] = v * dinv[i];
+  }
+#pragma unroll 1
+  for (int i = 0; i < P; i++) w[i] += z[i];
+#pragma unroll 1
+  for (int i = 0; i < P; i++) {
+    double v = w[i];
+    for (int j = 0; j < i; j++) v -= U[j * P + i] * out[j];
+    out[i] = v * dinv[i];
+  }
+  return true;
+}
+
+// Every lane of the wave with ok32 = false redoes its voxel in FP64 in turn
+// (wave-uniform loop over those lanes): Q = X'X + n_j Lambda and
+// r = m + Lambda s_j formed in FP64 from the FP32 inputs, then
+// ul_draw_f64_lds. On success out receives the FP64 draw rounded to FP32;
+// otherwise out keeps `prev` (the voxel's coefficients before the update) and
+// flags[1] counts the voxel. flags[0] counts every refactored voxel.
+template <int P>
+__device__ __forceinline__ void coef_fp64_redo(bool ok32, bool active, const float (&m)[P], const float (&sv)[P],
+                                               const float (&z)[P], float nn, const float *lam, const float *xtx,
+                                               const float *prev, uint32_t *flags, float (&out)[P])
+{
+  __shared__ double area[4][fp64_area<P>()];
+  double *a = area[(threadIdx.x >> 6) & 3];
+  uint64_t need = __ballot(!ok32);
+  while (need) {
+    const int l = __builtin_ctzll(need);
+    need &= need - 1;
+    if ((int)(threadIdx.x & 63) == l) {
+      double *Q = a, *r = a + 2 * P * P + P, *zz = r + P, *od = r + 3 * P;
+      for (int i = 0; i < P; i++) {
+        double acc = (double)m[i];
+        for (int j = 0; j < P; j++) {
+          const double lij = (double)lam[i * P + j];
+          acc += lij * (double)sv[j];
+          Q[i * P + j] = (double)xtx[i * P + j] + (double)nn * lij;
+        }
+        r[i] = acc;
+        zz[i] = (double)z[i];
+      }
+      if (active) atomicAdd(&flags[0], 1u);
+      if (ul_draw_f64_lds<P>(a)) {
+        for (int c = 0; c < P; c++) out[c] = (float)od[c];
+      } else {
+        // not positive definite: keep the previous coefficients
+        if (active) atomicAdd(&flags[1], 1u);
+        for (int c = 0; c < P; c++) out[c] = prev[c];
+      }
+    }
+  }
 }
 
 // Block `blk` of colour `colour` (256 voxels of its list) for chain `chain`.
@@ -1540,27 +1660,32 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   }
   float out[P];
   float *dst = pb.beta + ((size_t)chain * pb.V + s) * P;
-  if (!ul_draw<P, float>(Q, r, z, out, pb.pivot_min)) {
-    // FP64 refactorisation of this voxel (FP32 pivot below pivot_min)
-    CoefIn<P> in;
+  const bool ok32 = ul_draw<P, float>(Q, r, z, out, pb.pivot_min);
+  // FP64 refactorisation of the voxels whose FP32 pivot fell below pivot_min
+  if constexpr (P <= 6) {
+    if (!ok32) {
+      CoefIn<P> in;
 #pragma unroll
-    for (int i = 0; i < P; i++) {
-      in.m[i] = m[i];
-      in.sv[i] = sv[i];
-      in.z[i] = z[i];
+      for (int i = 0; i < P; i++) {
+        in.m[i] = m[i];
+        in.sv[i] = sv[i];
+        in.z[i] = z[i];
+      }
+      in.nn = (float)nn;
+      if (active) atomicAdd(&pb.flags[0], 1u);
+      const CoefOut<P> o = coef_fp64<P>(in, lam, pb.xtx);
+      if (o.ok) {
+#pragma unroll
+        for (int c = 0; c < P; c++) out[c] = o.out[c];
+      } else {
+        // not positive definite: keep the previous coefficients
+        if (active) atomicAdd(&pb.flags[1], 1u);
+#pragma unroll
+        for (int c = 0; c < P; c++) out[c] = dst[c];
+      }
     }
-    in.nn = (float)nn;
-    if (active) atomicAdd(&pb.flags[0], 1u);
-    const CoefOut<P> o = coef_fp64<P>(in, lam, pb.xtx);
-    if (o.ok) {
-#pragma unroll
-      for (int c = 0; c < P; c++) out[c] = o.out[c];
-    } else {
-      // not positive definite: keep the previous coefficients
-      if (active) atomicAdd(&pb.flags[1], 1u);
-#pragma unroll
-      for (int c = 0; c < P; c++) out[c] = dst[c];
-    }
+  } else {
+    if (__ballot(!ok32)) coef_fp64_redo<P>(ok32, active, m, sv, z, (float)nn, lam, pb.xtx, dst, pb.flags, out);
   }
   if (active) {
 #pragma unroll
@@ -1954,7 +2079,7 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
   int nlb = (int)((tiles + per - 1) / per);
   const bool wmu = pb.wm_update != 0, bmf = pb.latent_method == 1;
   if (G != 1 && G != 2 && G != 4 && G != 8) return hipErrorInvalidValue;
-  if (wish && (G != 4 || bmf || pb.P > 6)) return hipErrorInvalidValue;
+  if (wish && (G != 4 || bmf)) return hipErrorInvalidValue;
   // colour tiles (and pieces) of the inversion path at G = 4
   const int cs = ctile_set(pb, G);
   if (cs) nlb = vl_blocks(pb, cs);
@@ -1984,7 +2109,7 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
 bool latent_carries_wishart(const DevProblem &pb)
 {
   const int G = pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb);
-  return G == 4 && pb.P <= 6 && pb.latent_method == 0 && latent_vl_lds(pb) <= 65536;
+  return G == 4 && pb.latent_method == 0 && latent_vl_lds(pb) <= 65536;
 }
 
 // Voxel ranges: latent and X'Z launches cover the sorted voxels [v0, v1), so a
@@ -2120,7 +2245,7 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 }
 
 // The overlapped sweep: Z-free inversion draws at four lanes per voxel, b
-// fixed, Lambda drawn inside phase 0 (P <= 6, as latent_carries_wishart), both
+// fixed, Lambda drawn inside phase 0 (as latent_carries_wishart), both
 // colours present; any number of chains (the grid's second dimension; one
 // chain runs colour tiles and pieces). Phase 1 draws the Bartlett factor of
 // its sweep's Lambda ahead, for the phase 0 after it.
@@ -2150,15 +2275,7 @@ hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int l
       hipLaunchKernelGGL((k_overlap<PP, true>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep,    \
                          tail, acc, batch);                                                                        \
   } while (0)
-  switch (pb.P) {
-    case 1: L_(1); break;
-    case 2: L_(2); break;
-    case 3: L_(3); break;
-    case 4: L_(4); break;
-    case 5: L_(5); break;
-    case 6: L_(6); break;
-    default: return hipErrorInvalidValue;
-  }
+  BSG_DISPATCH_P(pb.P, L_)
 #undef L_
   return hipGetLastError();
 }

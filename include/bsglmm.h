@@ -221,12 +221,14 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g);
  * exact at every split; the split changes FP32 summation order only. */
 int bsg_set_lane_split(bsg_ctx *ctx, int g);
 int bsg_get_lane_split(bsg_ctx *ctx, int *g); /* the split the next sweep uses */
-/* Overlapped sweeps (on by default; no counterpart in the reference, whose
- * sweep runs its kernels one after another, mcmc.cpp:311-394): full sweeps
- * of a context whose latent kernel runs at four lanes per voxel (inversion
- * draws, no WM image, P <= 6; any number of chains) run as two launches per
- * sweep, each colour's coefficient update beside the other colour's latent
- * draws. Same variates, same sums, same chains as with on = 0.
+/* Overlapped sweeps (no counterpart in the reference, whose sweep runs its
+ * kernels one after another, mcmc.cpp:311-394): full sweeps of a context
+ * whose latent kernel runs at four lanes per voxel (inversion draws, no WM
+ * image; any number of chains) run as two launches per sweep, each colour's
+ * coefficient update beside the other colour's latent draws. Same variates,
+ * same sums, same chains as with on = 0. On by default for P <= 6; at P = 7
+ * and 8 it is supported but off by default (the colour updates then spill
+ * under the latent kernel's register budget and the serial order is faster).
  * bsg_get_overlap tells whether the next bsg_sweep* call runs overlapped. */
 int bsg_set_overlap(bsg_ctx *ctx, int on);
 int bsg_get_overlap(bsg_ctx *ctx, int *on);

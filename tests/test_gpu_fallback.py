@@ -72,12 +72,15 @@ def check_against_oracle(S, O, scs, prec0, sweep, nchains, tol):
     return worst
 
 
-@pytest.mark.parametrize("form,nchains", [("serial_z_stored", 1), ("serial_z_free", 1), ("overlap", 1),
-                                          ("overlap", 3)])
-def test_fp64_refactor_every_voxel(form, nchains):
+# P <= 6 redoes a voxel in registers (coef_fp64), P = 7, 8 in an LDS area of
+# the wave (coef_fp64_redo); both the same FP64 arithmetic
+@pytest.mark.parametrize("form,nchains,ncovar", [("serial_z_stored", 1, 4), ("serial_z_free", 1, 4), ("overlap", 1, 4),
+                                                 ("overlap", 3, 4), ("serial_z_free", 1, 8), ("overlap", 1, 8),
+                                                 ("overlap", 2, 7), ("serial_z_stored", 1, 7)])
+def test_fp64_refactor_every_voxel(form, nchains, ncovar):
     """pivot_min = 1e6: every voxel of both colours takes the FP64 branch."""
-    print(f"case FP64 fallback form={form} chains={nchains} (c1 N=150 P=4)", flush=True)
-    d, lay, L = setup(150, 4)
+    print(f"case FP64 fallback form={form} chains={nchains} (c1 N=150 P={ncovar})", flush=True)
+    d, lay, L = setup(150, ncovar)
     zs = 0 if form == "serial_z_stored" else 2
     xm = 0 if form == "serial_z_stored" else 2
     S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=xm, z_store=zs, pivot_min=1e6)
@@ -111,9 +114,9 @@ def test_fp64_refactor_in_a_batch():
     """k_coef_multi: every voxel of every member refactored in FP64 inside a
     batch; each member against the oracle, and the batch against the members
     swept alone."""
-    print("case FP64 fallback in a batch (c1 N=150 P=4 x2 chains, c1 N=60 P=6)", flush=True)
+    print("case FP64 fallback in a batch (c1 N=150 P=4 x2 chains, c1 N=60 P=6, c1 N=40 P=8)", flush=True)
     rows = []
-    for k, (n, p, ch) in enumerate(((150, 4, 2), (60, 6, 1))):
+    for k, (n, p, ch) in enumerate(((150, 4, 2), (60, 6, 1), (40, 8, 1))):
         d, lay, L = setup(n, p)
         mk = lambda: Sampler(L, d["covar"], d["data"], seed=SEED + k, nchains=ch, xtz_mode=2, z_store=2,
                              pivot_min=1e6)

@@ -26,7 +26,7 @@ def spat_coef_from_ranks(beta, lay):
 
 
 def pair(name, nsubs, ncovar, g=0, nchains=1, wm=False):
-    kw = {} if name == "c2" else dict(nsubs=nsubs, ncovar=ncovar)
+    kw = {} if name in ("c2", "c3") else dict(nsubs=nsubs, ncovar=ncovar)
     d = synth.generate(name, **kw)
     nx, ny, nz = d["grid"]
     lay = oracle.layout(d["mask"], nx, ny, nz)
@@ -62,12 +62,14 @@ def assert_same_state(A, B, nchains=1):
         assert np.array_equal(A.get_pair_sums(c), B.get_pair_sums(c))
 
 
-# (N, P, lane split): P = 1..6 (means on FMA for P <= 2, one MFMA for P = 3..4,
-# two for P = 5, 6); leftover tiles in 4 pieces at N >= 57, none at N = 13
-# (one subject round); C2 at full size (13 and 12 leftover tiles in 16 pieces)
+# (N, P, lane split): P = 1..8 (means on FMA for P <= 2, one MFMA for P = 3..4,
+# two for P = 5..8); leftover tiles in 4 pieces at N >= 57, none at N = 13
+# (one subject round); C2 at full size (13 and 12 leftover tiles in 16 pieces);
+# the C3 shape (N = 1,000, P = 8) at full size
 @pytest.mark.parametrize("name,nsubs,ncovar,g", [("c1", 300, 4, 0), ("c1", 200, 6, 0), ("c1", 57, 5, 4),
                                                  ("c1", 13, 3, 4), ("c1", 90, 2, 0), ("c1", 150, 1, 0),
-                                                 ("c2", None, None, 0)])
+                                                 ("c1", 120, 7, 0), ("c1", 200, 8, 0), ("c2", None, None, 0),
+                                                 ("c3", None, None, 0)])
 def test_overlapped_sweep_equals_serial(name, nsubs, ncovar, g):
     print(f"case overlap vs serial {name} N={nsubs} P={ncovar} G={g}", flush=True)
     L, A, B = pair(name, nsubs, ncovar, g)
@@ -102,7 +104,7 @@ def test_overlapped_accumulation_and_graph_equal_serial():
 # several chains: the grid's second dimension, whole tiles (no pieces), one
 # Lambda draw and wait per chain; C2 with 3 chains at full size
 @pytest.mark.parametrize("name,nsubs,ncovar,nchains", [("c1", 300, 4, 2), ("c1", 200, 6, 3), ("c1", 90, 2, 2),
-                                                       ("c2", None, None, 3)])
+                                                       ("c1", 100, 8, 2), ("c1", 60, 7, 3), ("c2", None, None, 3)])
 def test_overlapped_sweep_equals_serial_several_chains(name, nsubs, ncovar, nchains):
     print(f"case overlap vs serial {name} N={nsubs} P={ncovar} chains={nchains}", flush=True)
     L, A, B = pair(name, nsubs, ncovar, nchains=nchains)
@@ -140,14 +142,11 @@ def test_bounded_lambda_wait_reports_and_recovers():
 
 
 def test_overlap_off_where_unsupported():
-    """Seven covariates, a WM image or a lane split other than four run the
-    serial order."""
+    """A WM image or a lane split other than four run the serial order."""
     print("case overlap unsupported forms", flush=True)
     for kw in (dict(wm=True),):
         L, A, B = pair("c1", 300, 4, **kw)
         assert not A.overlap
-    L, A, B = pair("c1", 300, 7)
-    assert not A.overlap
     L, A, B = pair("c1", 300, 4, g=8)
     assert not A.overlap
     A.lane_split = 4

@@ -161,8 +161,8 @@ def test_z_free_fused_sweep(nsubs, ncovar, nchains, method):
 
 # Lane splits of the inversion path: G = 4 forms the means on the matrix
 # cores with one MFMA for P = 3, 4, two for P = 5..8 and none for P <= 2; one
-# chain at G = 4 also splits the leftover tiles (carrying the Wishart draw in
-# the latent launch up to P = 6, on the second stream past it).
+# chain at G = 4 also splits the leftover tiles (and the latent launch carries
+# the previous sweep's Wishart draw).
 @pytest.mark.parametrize("nsubs,ncovar,g,nchains", [(300, 6, 1, 1), (300, 6, 2, 1), (300, 6, 4, 2), (300, 6, 8, 1),
                                                     (200, 8, 4, 1), (200, 8, 4, 2), (90, 2, 4, 2), (57, 5, 4, 1),
                                                     (13, 3, 4, 2), (150, 7, 4, 1)])
@@ -351,7 +351,7 @@ def test_wm_coefficient_sweep(z_store, xtz_mode, method, nsubs):
     assert abs(b_gpu - s_orc / w_orc) < 6.0 / np.sqrt(w_orc)
 
 
-@pytest.mark.parametrize("nsubs,ncovar,nchains", [(150, 4, 2), (90, 6, 1), (300, 3, 1)])
+@pytest.mark.parametrize("nsubs,ncovar,nchains", [(150, 4, 2), (90, 6, 1), (300, 3, 1), (200, 8, 1), (100, 7, 2)])
 def test_wishart_carried_by_next_latent_launch(nsubs, ncovar, nchains):
     """At four lanes per voxel block 0 of the Z-free latent launch of sweep t+1
     draws the CAR precision of sweep t, and the last sweep of a call draws it in

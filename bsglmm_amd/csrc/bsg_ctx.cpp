@@ -223,7 +223,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     return BSG_ERR_ARG;
   }
   if (ncovar < 1 || ncovar > BSG_MAX_COVAR) {
-    set_error("bsg_create: number of covariates must be in 1..8");
+    set_error("bsg_create: number of covariates must be in 1.." + std::to_string(BSG_MAX_COVAR));
     return BSG_ERR_ARG;
   }
   if (nsubs < 1) {
@@ -247,6 +247,11 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   }
   if (o.latent_method < 0 || o.latent_method > 1 || (o.latent_method == 1 && o.z_store != 2)) {
     set_error("bsg_create: latent_method must be 0, or 1 with z_store 2");
+    return BSG_ERR_ARG;
+  }
+  if (ncovar > bsg::MAX_P && (o.z_store != 2 || o.xtz_mode != 2 || o.latent_method != 0)) {
+    set_error("bsg_create: more than " + std::to_string(bsg::MAX_P) +
+              " covariates run the Z-free fused form only (xtz_mode 2, z_store 2, latent_method 0)");
     return BSG_ERR_ARG;
   }
   int ndev = 0;
@@ -334,7 +339,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   // b is drawn each sweep only when a WM image is given (mcmc.cpp:344-345, UPDATE_WM)
   pb.wm_update = wm ? 1 : 0;
   pb.latent_method = o.latent_method;
-  if (o.z_store == 2 && o.latent_method == 0 && bsg::latent_vl_lds_bytes(pb) > 65536) {
+  if (P <= bsg::MAX_P && o.z_store == 2 && o.latent_method == 0 && bsg::latent_vl_lds_bytes(pb) > 65536) {
     set_error("bsg_create: z_store 2 stages X in LDS: 16 * 4 ceil(N/16) * P bytes must fit in 64 KiB "
               "(N * P <= 16384); use z_store 0 or 1");
     return BSG_ERR_ARG;
@@ -371,6 +376,8 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(xtxd, (size_t)P * P);
   ALLOC(xsumd, (size_t)P);
   ALLOC(wmd, (size_t)V);
+  float4 *xqd = nullptr;
+  if (P > bsg::MAX_P) ALLOC(xqd, (size_t)ctx->NQ * P);
   ALLOC(pb.Z, o.z_store == 2 ? (size_t)1 : (size_t)C * V * NS);
   ALLOC(pb.beta, (size_t)C * V * P);
   ALLOC(pb.M, (size_t)C * V * P);
@@ -398,12 +405,12 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
       return BSG_ERR_HIP;
     }
     pb.ncu = ncu;
-    pb.ctile = C == 1 ? 1 : 0;
+    pb.ctile = (C == 1 && P <= bsg::MAX_P) ? 1 : 0;
     const int R = (N + 15) / 16;
     int mp = 0;
     for (int c = 0; c < 2; c++) {
       const int tiles = (L.n[c] + 63) / 64, tail = tiles % ncu;
-      const int K = (tail > 0 && C == 1) ? std::min(std::min(ncu / tail, R), 8) : 0;
+      const int K = (tail > 0 && pb.ctile) ? std::min(std::min(ncu / tail, R), 8) : 0;
       pb.tl_left[c] = K > 1 ? tail : 0;
       pb.tl_k[c] = K > 1 ? K : 1;
       if (K > 1) mp += tail * K * 64 * P;
@@ -421,6 +428,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.nbr6 = nbr6;
   pb.ybits = yb;
   pb.X = Xd;
+  pb.Xq = xqd;
   pb.xtx = xtxd;
   pb.xsum = xsumd;
   pb.wm = wmd;
@@ -429,6 +437,14 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemcpyAsync(nbr6, L.nbr6.data(), sizeof(int) * L.nbr6.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(yb, ybits.data(), sizeof(uint32_t) * ybits.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(Xd, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice, st));
+  std::vector<float> xq;
+  if (xqd) {  // [NQ][P] quads of X for the wide kernels, zero past N
+    xq.assign((size_t)ctx->NQ * P * 4, 0.0f);
+    for (int q = 0; q < ctx->NQ; q++)
+      for (int c = 0; c < P; c++)
+        for (int l = 0; l < 4; l++) xq[((size_t)q * P + c) * 4 + l] = X[(size_t)c * NS + 4 * q + l];
+    HIPCHK(hipMemcpyAsync(xqd, xq.data(), sizeof(float) * xq.size(), hipMemcpyHostToDevice, st));
+  }
   HIPCHK(hipMemcpyAsync(xtxd, xtx.data(), sizeof(float) * xtx.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(xsumd, xsum.data(), sizeof(float) * P, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(wmd, wms.data(), sizeof(float) * V, hipMemcpyHostToDevice, st));
@@ -582,6 +598,10 @@ int bsg_update_wm_coef(bsg_ctx *ctx)
 int bsg_compute_xtz(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
+  if (ctx->opts.z_store == 2) {
+    set_error("bsg_compute_xtz: this context does not keep Z (z_store 2); bsg_update_latent forms X'Z");
+    return BSG_ERR_ARG;
+  }
   HIPCHK(bsg::launch_xtz(ctx->pb, ctx->opts.xtz_mode, ctx->stream, 0, ctx->L.totvox));
   return BSG_OK;
 }
@@ -1681,6 +1701,10 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     bsg_ctx *c = members[a];
     if (!c) {
       set_error("bsg_batch_create: NULL context");
+      return BSG_ERR_ARG;
+    }
+    if (c->P > bsg::MAX_P) {
+      set_error("bsg_batch_create: batches take members of at most " + std::to_string(bsg::MAX_P) + " covariates");
       return BSG_ERR_ARG;
     }
     if (c->opts.z_store != 2 || c->opts.xtz_mode != 2 || c->pb.wm_update || c->L.n[1] == 0) {

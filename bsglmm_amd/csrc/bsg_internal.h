@@ -21,7 +21,8 @@ constexpr uint32_t TAG_FALLBACK = 6; // index = voxel * NQ + quad, sub = subject
 
 constexpr int PAIR_BLOCKS = 512;    // fixed partial count per chain: deterministic sums
 constexpr int BSTAT_SLOTS = 4096;   // per-chain partial slots of the WM-coefficient statistic
-constexpr int MAX_P = 8;
+constexpr int MAX_P = 8;      // covariates of the register-resident kernels (and of batches)
+constexpr int PW_MAX = 20;    // covariates of the wide kernels (the reference's limit, covarGPU.cu:65)
 
 // Host copy of the reference layout (main.cu:189-394) plus the colour-sorted
 // order used on the device: sorted index s runs over INDX[0].hostVox, then
@@ -49,6 +50,7 @@ struct DevProblem {
   const int *nbr6;         // [V][6]
   const uint32_t *ybits;   // [V][NW], bit i of row s = (y_is == 1)
   const float *X;          // [P][NS], zero past N
+  const float4 *Xq;        // [NQ][P] quads of X, zero past N (wide problems, P > MAX_P)
   const float *xtx;        // [P][P]
   const float *xsum;       // [P]  column sums of X
   const float *wm;         // [V]

@@ -1133,15 +1133,9 @@ __device__ double gamma_mt(double a, uint32_t row, uint32_t sweep, uint32_t chai
   return d;
 }
 
-// Loops of the Wishart algebra: unrolled (matrices in registers) up to P = 6,
-// rolled above (matrices in LDS, few registers).
-template <int P>
-constexpr int wish_unroll() { return P <= 6 ? 8 : 1; }
-
 template <int P>
 __device__ void chol_lower(double (&a)[P][P])
 {
-#pragma unroll wish_unroll<P>()
   for (int j = 0; j < P; j++) {
     double d = a[j][j];
     for (int k = 0; k < j; k++) d -= a[j][k] * a[j][k];
@@ -1174,8 +1168,104 @@ __device__ __forceinline__ double bartlett_entry(const DevProblem &pb, int chain
   return val;
 }
 
+// ---------------------------------------------------------------------------
+// The Wishart algebra spread over the 64 lanes of one wave, matrices in LDS
+// (row-major P x P, P at run time up to PW_MAX): used above P = 6. Every
+// entry is formed by the operations of the one-lane form in the same order
+// (a right-looking Cholesky subtracts the products of an entry in the order
+// k = 0, 1, ... as the left-looking loop does; the inverse runs one column
+// per lane; the products one entry per lane), so the result does not depend
+// on the lane count.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// lower Cholesky factor in place, zeros above the diagonal (chol_lower)
+__device__ void chol_lower_wave(double *a, int P, int lane)
+{
+  for (int j = 0; j < P; j++) {
+    if (lane == 0) a[j * P + j] = sqrt(a[j * P + j]);
+    wave_sync();
+    for (int i = j + 1 + lane; i < P; i += 64) a[i * P + j] = a[i * P + j] / a[j * P + j];
+    wave_sync();
+    // the trailing entries (i, c), j < c <= i, subtract their term k = j
+    const int m = P - j - 1, cnt = m * (m + 1) / 2;
+    for (int e = lane; e < cnt; e += 64) {
+      int ii = 0;
+      while ((ii + 1) * (ii + 2) / 2 <= e) ii++;
+      const int i = j + 1 + ii, c = j + 1 + (e - ii * (ii + 1) / 2);
+      a[i * P + c] -= a[i * P + j] * a[c * P + j];
+    }
+    wave_sync();
+  }
+  for (int e = lane; e < P * P; e += 64)
+    if (e % P > e / P) a[e] = 0.0;
+  wave_sync();
+}
+
+// T packed lower triangle (row-major), A the Bartlett factor; S, inv, B and
+// Y (P x P each) are LDS work areas. Writes Lambda (FP32, FP64, and tagged
+// when `tagged`) and the pair sums.
+__device__ void wishart_algebra_wave(const DevProblem &pb, int chain, int P, const double *T, const double *A,
+                                     bool tagged, double *S, double *inv, double *B, double *Y, int lane)
+{
+  for (int e = lane; e < P * P; e += 64) {
+    const int i = e / P, j = e % P;
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const double t = T[hi * (hi + 1) / 2 + lo];
+    S[e] = i == j ? t + 1.0 : t;
+    pb.tsum[(size_t)chain * P * P + e] = t;
+  }
+  wave_sync();
+  chol_lower_wave(S, P, lane);
+  if (lane < P) {  // column `lane` of the inverse: forward, then back substitution
+    const int col = lane;
+    double *y = Y + (size_t)col * P;
+    for (int i = 0; i < P; i++) {
+      double v = (i == col) ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) v -= S[i * P + k] * y[k];
+      y[i] = v / S[i * P + i];
+    }
+    for (int i = P - 1; i >= 0; i--) {
+      double v = y[i];
+      for (int k = i + 1; k < P; k++) v -= S[k * P + i] * inv[k * P + col];
+      inv[i * P + col] = v / S[i * P + i];
+    }
+  }
+  wave_sync();
+  chol_lower_wave(inv, P, lane);
+  for (int e = lane; e < P * P; e += 64) {
+    const int i = e / P, j = e % P;
+    double v = 0.0;
+    if (j <= i)
+      for (int k = j; k <= i; k++) v += inv[i * P + k] * A[k * P + j];
+    B[e] = v;
+  }
+  wave_sync();
+  float *lam = pb.lam + (size_t)chain * P * P;
+  double *lam64 = pb.lam64 + (size_t)chain * P * P;
+  for (int e = lane; e < P * P; e += 64) {
+    const int i = e / P, j = e % P;
+    const int K = i < j ? i : j;
+    double v = 0.0;
+    for (int k = 0; k <= K; k++) v += B[i * P + k] * B[j * P + k];
+    lam64[e] = v;
+    lam[e] = (float)v;
+    if (tagged)
+      __hip_atomic_store(pb.lamtag + (size_t)chain * P * P + e,
+                         ((unsigned long long)(pb.lam_tag ? pb.lam_tag : 1) << 32) |
+                             (unsigned long long)__float_as_uint((float)v),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // S = (T + I)^-1, its Cholesky factor L, B = L A and Lambda = B B' in FP64
-// (cholesky.cpp:245-286 steps), on one lane; T packed lower triangle.
+// (cholesky.cpp:245-286 steps), on one lane with the matrices in registers
+// (P <= 6); T packed lower triangle.
 template <int P>
 __device__ __forceinline__ void wishart_algebra(const DevProblem &pb, int chain, const double *T,
                                                 const double (&A)[P][P], bool tagged, double (&S)[P][P],
@@ -1190,7 +1280,6 @@ __device__ __forceinline__ void wishart_algebra(const DevProblem &pb, int chain,
     }
   for (int a = 0; a < P; a++) S[a][a] += 1.0;
   chol_lower<P>(S);
-#pragma unroll wish_unroll<P>()
   for (int col = 0; col < P; col++) {
     for (int i = 0; i < P; i++) {
       double v = (i == col) ? 1.0 : 0.0;
@@ -1204,7 +1293,6 @@ __device__ __forceinline__ void wishart_algebra(const DevProblem &pb, int chain,
     }
   }
   chol_lower<P>(inv);
-#pragma unroll wish_unroll<P>()
   for (int i = 0; i < P; i++)
     for (int j = 0; j < P; j++) {
       double v = 0.0;
@@ -1214,7 +1302,6 @@ __device__ __forceinline__ void wishart_algebra(const DevProblem &pb, int chain,
     }
   float *lam = pb.lam + (size_t)chain * P * P;
   double *lam64 = pb.lam64 + (size_t)chain * P * P;
-#pragma unroll wish_unroll<P>()
   for (int i = 0; i < P; i++)
     for (int j = 0; j < P; j++) {
       const int K = i < j ? i : j;
@@ -1249,16 +1336,16 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  if (lane != 0) return;
-  // the FP64 algebra runs on one lane: in registers up to P = 6, with its
-  // matrices in LDS above (few registers, so that the kernels that carry this
-  // draw keep their register budget)
+  // the FP64 algebra: on one lane in registers up to P = 6; above, over the
+  // wave with its matrices in LDS (few registers, so that the kernels that
+  // carry this draw keep their register budget)
   if constexpr (P <= 6) {
+    if (lane != 0) return;
     double S[P][P], inv[P][P], B[P][P], y[P];
     wishart_algebra<P>(pb, chain, T, A, tagged, S, inv, B, y);
   } else {
-    __shared__ double S[P][P], inv[P][P], B[P][P], y[P];
-    wishart_algebra<P>(pb, chain, T, A, tagged, S, inv, B, y);
+    __shared__ double S[P * P], inv[P * P], B[P * P], Y[P * P];
+    wishart_algebra_wave(pb, chain, P, T, &A[0][0], tagged, S, inv, B, Y, lane);
   }
 }
 
@@ -1937,7 +2024,9 @@ __global__ __launch_bounds__(256) void k_loglik(DevProblem pb, int T, double *pa
   }
   if (active) {
     double *dst = part + (((size_t)chain * gridDim.x + tile) * pb.N + i) * T;
-    for (int t = 0; t < T; t++) dst[t] = acc[t];
+#pragma unroll
+    for (int t = 0; t < MAX_P; t++)
+      if (t < T) dst[t] = acc[t];
   }
 }
 
@@ -2009,6 +2098,493 @@ __global__ __launch_bounds__(256) void k_wishart_multi(const DevProblem *pbs, co
 }
 
 
+// ===========================================================================
+// Wide problems: MAX_P < P <= PW_MAX (20) covariates. The reference's GPU path
+// accepts P <= 20 (per-thread var[20*20], covarGPU.cu:65, 300-304). These
+// kernels take P at run time: per-voxel vectors sit in registers through
+// loops unrolled to PW_MAX under a uniform predicate (c < P), a voxel's P x P
+// factor sits in LDS. They carry the Z-free fused form in the serial order:
+// latent draws with X'Z (and the WM statistic), colour updates (with the
+// posterior accumulation on save sweeps), FP64 pair sums, the Wishart draw,
+// and the per-save summaries. The random counters are those of the other
+// kernels, so the draws are the same variates.
+// ===========================================================================
+
+// Latent draws + X'Z (+ the WM statistic with WMU), Z not kept. G lanes per
+// voxel split its subject quads into G contiguous ranges (as latent_vl_body);
+// X is read as [NQ][P] quads, staged in LDS when it fits (xlds), else from
+// global memory (every lane of a subject group reads the same address).
+template <bool WMU>
+__global__ __launch_bounds__(256) void k_latent_w(DevProblem pb, int G, uint32_t sweep, int xlds)
+{
+  sweep += *pb.sweep_base;
+  extern __shared__ float4 xsm[];
+  const int P = pb.P;
+  const float4 *xq = pb.Xq;
+  if (xlds) {
+    for (int i = threadIdx.x; i < pb.NQ * P; i += blockDim.x) xsm[i] = pb.Xq[i];
+    __syncthreads();
+    xq = xsm;
+  }
+  const int chain = blockIdx.y;
+  const int lgg = G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0;
+  const int VPW = 64 >> lgg;
+  const int lane = threadIdx.x & 63, vl = lane & (VPW - 1), g = lane >> (6 - lgg);
+  const int R = (pb.NQ + (1 << lgg) - 1) >> lgg;
+  const int q0 = g * R, q1 = min(q0 + R, pb.NQ);
+  const float b = pb.bwm[chain];
+  const float eps = pb.trunc_eps;
+  const int tiles = (pb.V + 4 * VPW - 1) / (4 * VPW);
+  double bpart = 0.0;
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const int s = (t * 4 + (threadIdx.x >> 6)) * VPW + vl;
+    const bool valid = s < pb.V;
+    const int sv = valid ? s : pb.V - 1;
+    const float wmv = pb.wm[sv];
+    const float bw = b * wmv;
+    const float *bp = pb.beta + ((size_t)chain * pb.V + sv) * P;
+    float bs[PW_MAX], acc[PW_MAX];
+#pragma unroll
+    for (int c = 0; c < PW_MAX; c++) {
+      bs[c] = c < P ? bp[c] : 0.0f;
+      acc[c] = 0.0f;
+    }
+    const uint32_t *yrow = pb.ybits + (size_t)sv * pb.NW;
+    const uint32_t cbase = (uint32_t)sv * (uint32_t)pb.NQ;
+    float zs = 0.0f;
+    for (int q = q0; q < q1; q++) {
+      const float4 *xr = xq + (size_t)q * P;
+      float mu[4] = {bw, bw, bw, bw};
+#pragma unroll
+      for (int c = 0; c < PW_MAX; c++)
+        if (c < P) {
+          const float4 x = xr[c];
+          mu[0] = fmaf(bs[c], x.x, mu[0]);
+          mu[1] = fmaf(bs[c], x.y, mu[1]);
+          mu[2] = fmaf(bs[c], x.z, mu[2]);
+          mu[3] = fmaf(bs[c], x.w, mu[3]);
+        }
+      const uint32_t yw = yrow[q >> 3] >> ((q & 7) * 4);
+      uint32_t rr[4];
+      philox10(cbase + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain, pb.key0, pb.key1, rr);
+      float z[4];
+      latent_quad(mu, yw, rr, eps, z);  // X is zero past N: padding subjects add nothing to m_j
+      if (WMU && 4 * q + 3 >= pb.N) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) z[l] = (4 * q + l < pb.N) ? z[l] : 0.0f;
+      }
+#pragma unroll
+      for (int c = 0; c < PW_MAX; c++)
+        if (c < P) {
+          const float4 x = xr[c];
+          acc[c] = fmaf(x.x, z[0], acc[c]);
+          acc[c] = fmaf(x.y, z[1], acc[c]);
+          acc[c] = fmaf(x.z, z[2], acc[c]);
+          acc[c] = fmaf(x.w, z[3], acc[c]);
+        }
+      if (WMU) zs += (z[0] + z[1]) + (z[2] + z[3]);
+    }
+    for (int off = VPW; off < 64; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < PW_MAX; c++)
+        if (c < P) acc[c] += __shfl_xor(acc[c], off, 64);
+      if (WMU) zs += __shfl_xor(zs, off, 64);
+    }
+    if (valid && g == 0) {
+      float *m = pb.M + ((size_t)chain * pb.V + s) * P;
+#pragma unroll
+      for (int c = 0; c < PW_MAX; c++)
+        if (c < P) m[c] = acc[c];
+      if (WMU) {
+        double r = (double)zs;
+        for (int c = 0; c < P; c++) r -= (double)pb.xsum[c] * (double)bs[c];
+        bpart += (double)wmv * r;
+      }
+    }
+  }
+  if (WMU) {
+    double bp1[1] = {bpart};
+    block_sum_store<1>(bp1, pb.bstat + (size_t)chain * BSTAT_SLOTS + blockIdx.x);
+  }
+}
+
+// Packed upper triangle, row-major: entry (i, j), i <= j, of a P x P matrix.
+__device__ __forceinline__ int upk(int i, int j, int P) { return i * P - (i * (i - 1)) / 2 + (j - i); }
+
+// Floats of one thread's LDS area in k_coef_w: the packed factor and a
+// vector, padded to an odd stride so that the threads' areas fall in
+// different banks.
+__host__ __device__ inline int coefw_stride(int P) { return (P * (P + 1) / 2 + P) | 1; }
+constexpr int COEFW_THREADS = 64;
+
+// Posterior accumulation of one voxel (accum_voxel with P at run time).
+__device__ __forceinline__ void accum_voxel_rt(const DevProblem &pb, const Accum &ac, int batch, int chain, int s,
+                                               const float *b)
+{
+  const int P = pb.P;
+  const double bw = (double)pb.bwm[chain] * (double)pb.wm[s];
+  const size_t base = (size_t)chain * pb.V + s;
+  const int K = P + ac.ntypes;
+  const bool inb = batch >= 0;
+  float *bat = ac.batch + (((size_t)chain * ac.nbatches + (inb ? batch : 0)) * pb.V + s) * K;
+  for (int c = 0; c < P; c++) {
+    const double v = (double)b[c];
+    ac.sum[base * P + c] += v;
+    ac.sq[base * P + c] += v * v;
+    if (inb) bat[c] += b[c];
+  }
+  for (int t = 0; t < ac.ntypes && t < P; t++) {
+    const double pr = phi_as((double)b[t] + bw);
+    ac.prb[base * ac.ntypes + t] += pr;
+    if (inb) bat[P + t] += (float)pr;
+  }
+  ac.bwm[base] += bw;
+}
+
+// The four normals of group g of voxel s's coefficient draw (coef_body's
+// Box-Muller on the hardware v_log / v_sqrt / v_sin / v_cos)
+__device__ __forceinline__ void coef_normals(const DevProblem &pb, int s, uint32_t sweep, int chain, int g,
+                                             float (&zz)[4])
+{
+  uint32_t rr[4];
+  philox10((uint32_t)s, sweep, TAG_COEF | ((uint32_t)g << 4), (uint32_t)chain, pb.key0, pb.key1, rr);
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01f(rr[2 * h])));
+    const float ang = u01f(rr[2 * h + 1]);
+    zz[2 * h] = rad * __builtin_amdgcn_cosf(ang);
+    zz[2 * h + 1] = rad * __builtin_amdgcn_sinf(ang);
+  }
+}
+
+// The coefficient update of one voxel in FP64, for one lane at a time (the
+// wave-uniform loop of k_coef_w): its FP32 inputs formed again from memory as
+// k_coef_w forms them (the neighbour sum s_j and m_j - b WM_j X'1), then the
+// factor and solves of k_coef_w's FP32 path in FP64 in the wave's LDS area
+// (packed factor, then a vector, which holds the result). Returns false if
+// the precision is not positive definite.
+__device__ bool coef_w_fp64(const DevProblem &pb, int s, uint32_t sweep, int chain, double *U)
+{
+  const int P = pb.P, NU = P * (P + 1) / 2;
+  double *t = U + NU;
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  const float *lam = pb.lam + (size_t)chain * P * P;
+  const float bw = pb.bwm[chain] * pb.wm[s];
+  const float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
+  int nn = 0;
+  for (int d = 0; d < 6; d++) nn += pb.nbr6[(size_t)s * 6 + d] >= 0 ? 1 : 0;
+  // U[j] (j < P) first holds s_j in FP32, summed in neighbour order
+  float *svf = reinterpret_cast<float *>(U + NU + P);
+  for (int c = 0; c < P; c++) svf[c] = 0.0f;
+  for (int d = 0; d < 6; d++) {
+    const int k = pb.nbr6[(size_t)s * 6 + d];
+    if (k >= 0)
+      for (int c = 0; c < P; c++) svf[c] += betc[(size_t)k * P + c];
+  }
+  for (int i = 0; i < P; i++) {
+    double acc = (double)fmaf(-bw, pb.xsum[i], mz[i]);
+    for (int j = 0; j < P; j++) acc += (double)lam[i * P + j] * (double)svf[j];
+    t[i] = acc;
+  }
+  for (int i = 0; i < P; i++)
+    for (int j = i; j < P; j++) U[upk(i, j, P)] = (double)pb.xtx[i * P + j] + (double)nn * (double)lam[i * P + j];
+  for (int k = P - 1; k >= 0; k--) {
+    double d = U[upk(k, k, P)];
+    for (int j = k + 1; j < P; j++) d -= U[upk(k, j, P)] * U[upk(k, j, P)];
+    if (!(d >= 0.0)) return false;
+    const double ukk = sqrt(d);
+    U[upk(k, k, P)] = ukk;
+    for (int i = 0; i < k; i++) {
+      double v = U[upk(i, k, P)];
+      for (int j = k + 1; j < P; j++) v -= U[upk(i, j, P)] * U[upk(k, j, P)];
+      U[upk(i, k, P)] = v / ukk;
+    }
+  }
+  for (int i = P - 1; i >= 0; i--) {
+    double v = t[i];
+    for (int j = i + 1; j < P; j++) v -= U[upk(i, j, P)] * t[j];
+    t[i] = v / U[upk(i, i, P)];
+  }
+  for (int g = 0; 4 * g < P; g++) {
+    float zz[4];
+    coef_normals(pb, s, sweep, chain, g, zz);
+    for (int h = 0; h < 4 && 4 * g + h < P; h++) t[4 * g + h] += (double)zz[h];
+  }
+  for (int i = 0; i < P; i++) {
+    double v = t[i];
+    for (int j = 0; j < i; j++) v -= U[upk(j, i, P)] * t[j];
+    t[i] = v / U[upk(i, i, P)];
+  }
+  return true;
+}
+
+// Coefficient update of one colour, one voxel per thread, 64 threads (one
+// wave) per block. The neighbour sum s_j, then Q = X'X + n_j Lambda (upper
+// triangle) factored in place as Q = U U' with U upper (ul_draw's scheme),
+// r = m + Lambda s_j solved through U and U' with the normals added between:
+// b = U^-T (U^-1 r + z). A pivot below pivot_min sends the voxel to the FP64
+// form (coef_w_fp64); if that fails too, the voxel keeps its coefficients
+// and flags[1] counts it.
+__global__ __launch_bounds__(64) void k_coef_w(DevProblem pb, int colour, uint32_t sweep, Accum ac, int batch)
+{
+  sweep += *pb.sweep_base;
+  const int P = pb.P, NU = P * (P + 1) / 2;
+  extern __shared__ float wsm[];
+  float *U = wsm + threadIdx.x * coefw_stride(P);
+  float *t = U + NU;
+  // the FP64 redo's area: packed factor, vector, and the FP32 neighbour sum
+  __shared__ double f64area[PW_MAX * (PW_MAX + 1) / 2 + PW_MAX + PW_MAX / 2];
+  const int chain = blockIdx.y;
+  const int q = blockIdx.x * COEFW_THREADS + threadIdx.x;
+  const int nc = colour ? pb.V - pb.n0 : pb.n0;
+  const bool active = q < nc;
+  const int s = (active ? q : nc - 1) + (colour ? pb.n0 : 0);
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  const float *lam = pb.lam + (size_t)chain * P * P;
+  int nb[6];
+#pragma unroll
+  for (int d = 0; d < 6; d++) nb[d] = pb.nbr6[(size_t)s * 6 + d];
+  float sv[PW_MAX], m[PW_MAX];
+  int nn = 0;
+#pragma unroll
+  for (int c = 0; c < PW_MAX; c++) sv[c] = 0.0f;
+#pragma unroll
+  for (int d = 0; d < 6; d++)
+    if (nb[d] >= 0) {
+      nn++;
+      const float *src = betc + (size_t)nb[d] * P;
+#pragma unroll
+      for (int c = 0; c < PW_MAX; c++)
+        if (c < P) sv[c] += src[c];
+    }
+  const float bw = pb.bwm[chain] * pb.wm[s];
+  const float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
+#pragma unroll
+  for (int c = 0; c < PW_MAX; c++) m[c] = c < P ? fmaf(-bw, pb.xsum[c], mz[c]) : 0.0f;
+  for (int i = 0; i < P; i++) {
+    float acc = m[0];
+#pragma unroll
+    for (int c = 0; c < PW_MAX; c++) acc = (c == i) ? m[c] : acc;
+#pragma unroll
+    for (int j = 0; j < PW_MAX; j++)
+      if (j < P) acc = fmaf(lam[i * P + j], sv[j], acc);
+    t[i] = acc;
+  }
+  for (int i = 0; i < P; i++)
+    for (int j = i; j < P; j++) U[upk(i, j, P)] = fmaf((float)nn, lam[i * P + j], pb.xtx[i * P + j]);
+  bool ok = true;
+  for (int k = P - 1; k >= 0; k--) {
+    float d = U[upk(k, k, P)];
+    for (int j = k + 1; j < P; j++) {
+      const float u = U[upk(k, j, P)];
+      d -= u * u;
+    }
+    if (!(d >= pb.pivot_min)) {
+      ok = false;
+      break;
+    }
+    // pivots are >= pivot_min (no denormals): hardware square root and reciprocal
+    const float ukk = __builtin_amdgcn_sqrtf(d);
+    const float inv = __builtin_amdgcn_rcpf(ukk);
+    U[upk(k, k, P)] = ukk;
+    for (int i = 0; i < k; i++) {
+      float v = U[upk(i, k, P)];
+      for (int j = k + 1; j < P; j++) v -= U[upk(i, j, P)] * U[upk(k, j, P)];
+      U[upk(i, k, P)] = v * inv;
+    }
+  }
+  float out[PW_MAX];
+  if (ok) {
+    for (int i = P - 1; i >= 0; i--) {
+      float v = t[i];
+      for (int j = i + 1; j < P; j++) v -= U[upk(i, j, P)] * t[j];
+      t[i] = v * __builtin_amdgcn_rcpf(U[upk(i, i, P)]);
+    }
+    for (int g = 0; 4 * g < P; g++) {
+      float zz[4];
+      coef_normals(pb, s, sweep, chain, g, zz);
+      for (int h = 0; h < 4 && 4 * g + h < P; h++) t[4 * g + h] += zz[h];
+    }
+    for (int i = 0; i < P; i++) {
+      float v = t[i];
+      for (int j = 0; j < i; j++) v -= U[upk(j, i, P)] * t[j];
+      t[i] = v * __builtin_amdgcn_rcpf(U[upk(i, i, P)]);
+    }
+#pragma unroll
+    for (int c = 0; c < PW_MAX; c++) out[c] = c < P ? t[c] : 0.0f;
+  }
+  float *dst = pb.beta + ((size_t)chain * pb.V + s) * P;
+  // FP64 refactorisation, one lane at a time in the wave's area
+  uint64_t need = __ballot(!ok);
+  while (need) {
+    const int l = __builtin_ctzll(need);
+    need &= need - 1;
+    if ((int)threadIdx.x == l) {
+      if (active) atomicAdd(&pb.flags[0], 1u);
+      const bool ok64 = coef_w_fp64(pb, s, sweep, chain, f64area);
+      if (!ok64 && active) atomicAdd(&pb.flags[1], 1u);  // not positive definite: keep the coefficients
+      const double *t64 = f64area + NU;
+#pragma unroll
+      for (int c = 0; c < PW_MAX; c++)
+        if (c < P) out[c] = ok64 ? (float)t64[c] : dst[c];
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < PW_MAX; c++)
+      if (c < P) dst[c] = out[c];
+    if (ac.sum) accum_voxel_rt(pb, ac, batch, chain, s, dst);  // a save sweep: this voxel's final b
+  }
+}
+
+// FP64 pair sums (k_pairs with P at run time): PAIR_BLOCKS blocks per chain,
+// each over a contiguous range of voxels in rounds of 64. A round stages the
+// float differences of each voxel's +x, +y, +z pairs in LDS (the a-side with
+// the reference's b^a_k != 0 test applied, covarGPU.cu:975); thread e owns
+// entry e = (a, c), c <= a, and adds the exact FP64 products of the round in
+// pair order. One partial per block and entry.
+constexpr int PAIRW_ROWS = 192;
+__global__ __launch_bounds__(256) void k_pairs_w(DevProblem pb)
+{
+  const int P = pb.P, NE = P * (P + 1) / 2;
+  __shared__ float dA[PAIRW_ROWS][PW_MAX], dC[PAIRW_ROWS][PW_MAX];
+  const int chain = blockIdx.y;
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  const int chunk = (pb.V + PAIR_BLOCKS - 1) / PAIR_BLOCKS;
+  const int s0 = blockIdx.x * chunk;
+  const int s1 = min(pb.V, s0 + chunk);
+  const int e = threadIdx.x;
+  int ea = 0, ec = 0;  // e = ea (ea + 1) / 2 + ec, ec <= ea
+  while ((ea + 1) * (ea + 2) / 2 <= e) ea++;
+  ec = e - ea * (ea + 1) / 2;
+  double acc = 0.0;
+  for (int base = s0; base < s1; base += PAIRW_ROWS / 3) {
+    if (threadIdx.x < PAIRW_ROWS) {
+      const int row = threadIdx.x, sidx = base + row / 3, d = row % 3;
+      const int k = sidx < s1 ? pb.nbr6[(size_t)sidx * 6 + 2 * d + 1] : -1;
+      for (int c = 0; c < P; c++) {
+        float da = 0.0f, dc = 0.0f;
+        if (k >= 0) {
+          const float bk = betc[(size_t)k * P + c];
+          dc = betc[(size_t)sidx * P + c] - bk;
+          da = bk != 0.0f ? dc : 0.0f;
+        }
+        dA[row][c] = da;
+        dC[row][c] = dc;
+      }
+    }
+    __syncthreads();
+    if (e < NE)
+      for (int row = 0; row < PAIRW_ROWS; row++) acc = fma((double)dA[row][ea], (double)dC[row][ec], acc);
+    __syncthreads();
+  }
+  if (e < NE) pb.partials[((size_t)chain * pb.max_parts + blockIdx.x) * NE + e] = acc;
+}
+
+// Bartlett entry idx = i P + j with P at run time (bartlett_entry)
+__device__ double bartlett_entry_rt(const DevProblem &pb, int chain, int idx, uint32_t sweep)
+{
+  const int P = pb.P, i = idx / P, j = idx % P;
+  double val = 0.0;
+  if (i == j) {
+    val = sqrt(2.0 * gamma_mt(0.5 * (double)(pb.df - i), (uint32_t)i, sweep, (uint32_t)chain, pb.key0, pb.key1));
+  } else if (j < i) {
+    uint32_t rr[4];
+    philox10((uint32_t)idx, sweep, TAG_WNORM, (uint32_t)chain, pb.key0, pb.key1, rr);
+    val = sqrt(-2.0 * log(u01d(rr[0]))) * cospi(2.0 * u01d(rr[1]));
+  }
+  return val;
+}
+
+// The CAR-precision draw with P at run time (wishart_block's steps): thread e
+// sums entry e of the PAIR_BLOCKS partials in block order, the Bartlett factor
+// is drawn across the block, then the first wave forms S = (T + I)^-1,
+// L = chol(S) and Lambda = (L A)(L A)' in FP64 with the matrices in LDS.
+__global__ __launch_bounds__(256) void k_wishart_w(DevProblem pb, uint32_t sweep)
+{
+  sweep += *pb.sweep_base;
+  const int P = pb.P, NE = P * (P + 1) / 2;
+  const int chain = blockIdx.x;
+  __shared__ double T[PW_MAX * (PW_MAX + 1) / 2];
+  __shared__ double A[PW_MAX * PW_MAX], S[PW_MAX * PW_MAX], inv[PW_MAX * PW_MAX], B[PW_MAX * PW_MAX],
+      Y[PW_MAX * PW_MAX];
+  const double *part = pb.partials + (size_t)chain * pb.max_parts * NE;
+  if ((int)threadIdx.x < NE) {
+    double acc = 0.0;
+    for (int b = 0; b < PAIR_BLOCKS; b++) acc += part[(size_t)b * NE + threadIdx.x];
+    T[threadIdx.x] = acc;
+  }
+  for (int idx = threadIdx.x; idx < P * P; idx += blockDim.x) A[idx] = bartlett_entry_rt(pb, chain, idx, sweep);
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  wishart_algebra_wave(pb, chain, P, T, A, false, S, inv, B, Y, threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_accum_w(DevProblem pb, Accum ac, int batch)
+{
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= pb.V) return;
+  const int chain = blockIdx.y;
+  accum_voxel_rt(pb, ac, batch, chain, s, pb.beta + ((size_t)chain * pb.V + s) * pb.P);
+}
+
+__global__ __launch_bounds__(256) void k_cmeans_w(DevProblem pb, double *part)
+{
+  const int chain = blockIdx.y, P = pb.P;
+  double acc[PW_MAX];
+#pragma unroll
+  for (int c = 0; c < PW_MAX; c++) acc[c] = 0.0;
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < pb.V; s += gridDim.x * 256)
+#pragma unroll
+    for (int c = 0; c < PW_MAX; c++)
+      if (c < P) acc[c] += (double)betc[(size_t)s * P + c];
+  block_sum_store<PW_MAX>(acc, part + ((size_t)chain * gridDim.x + blockIdx.x) * P, P);
+}
+
+// per-subject log-likelihood (k_loglik with P at run time)
+__global__ __launch_bounds__(256) void k_loglik_w(DevProblem pb, int T, double *part)
+{
+  const int P = pb.P;
+  const int chain = blockIdx.z;
+  const int tile = blockIdx.x;
+  const int i = blockIdx.y * 256 + threadIdx.x;
+  const bool active = i < pb.N;
+  const int ii = active ? i : 0;
+  float xi[PW_MAX];
+#pragma unroll
+  for (int c = 0; c < PW_MAX; c++) xi[c] = c < P ? pb.X[(size_t)c * pb.NS + ii] : 0.0f;
+  const float b = pb.bwm[chain];
+  const float *betc = pb.beta + (size_t)chain * pb.V * P;
+  double acc[PW_MAX];
+#pragma unroll
+  for (int t = 0; t < PW_MAX; t++) acc[t] = 0.0;
+  const int s1 = min(pb.V, (tile + 1) * 256);
+  for (int s = tile * 256; s < s1; s++) {
+    float bs[PW_MAX];
+#pragma unroll
+    for (int c = 0; c < PW_MAX; c++) bs[c] = c < P ? betc[(size_t)s * P + c] : 0.0f;
+    float rest = b * pb.wm[s];
+#pragma unroll
+    for (int c = 0; c < PW_MAX; c++)
+      if (c >= T && c < P) rest = fmaf(bs[c], xi[c], rest);
+    const bool y = (pb.ybits[(size_t)s * pb.NW + (ii >> 5)] >> (ii & 31)) & 1u;
+#pragma unroll
+    for (int t = 0; t < PW_MAX; t++) {
+      if (t < T) {
+        const float p = phi_as_f(bs[t] + rest);
+        acc[t] += (double)(y ? logf(p + 1e-35f) : logf(1.0f - p + 1e-35f));
+      }
+    }
+  }
+  if (active) {
+    double *dst = part + (((size_t)chain * gridDim.x + tile) * pb.N + i) * T;
+#pragma unroll
+    for (int t = 0; t < PW_MAX; t++)
+      if (t < T) dst[t] = acc[t];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -2037,12 +2613,62 @@ static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
     default: return hipErrorInvalidValue; \
   }
 
+// lanes per voxel of the wide latent kernel: a set split, else enough lanes
+// per voxel to give the chip many waves, few enough that every lane keeps
+// several quads
+static int wide_groups(const DevProblem &pb)
+{
+  if (pb.lat_g > 0) return pb.lat_g;
+  if (pb.NQ <= 4) return 1;
+  if (pb.NQ <= 16) return 2;
+  if (pb.NQ <= 128) return 4;
+  return 8;
+}
+
 // the lane split chosen from N (vl_groups without a set split)
 int latent_vl_groups(const DevProblem &pb)
 {
   DevProblem q = pb;
   q.lat_g = 0;
-  return vl_groups(q);
+  return pb.P > MAX_P ? wide_groups(q) : vl_groups(q);
+}
+
+// ---- launchers of the wide kernels (MAX_P < P <= PW_MAX) ----
+static hipError_t launch_latent_w(const DevProblem &pb, uint32_t sweep, hipStream_t st, int *parts)
+{
+  const int G = wide_groups(pb);
+  if (G != 1 && G != 2 && G != 4 && G != 8) return hipErrorInvalidValue;
+  const int VPW = 64 / G;
+  const long long tiles = ((long long)pb.V + 4 * VPW - 1) / (4 * VPW);
+  const int nlb = (int)std::min<long long>(tiles, BSTAT_SLOTS);
+  const size_t xbytes = (size_t)pb.NQ * pb.P * sizeof(float4);
+  const int xlds = xbytes <= 65536 ? 1 : 0;
+  dim3 grid((unsigned)nlb, (unsigned)pb.nchains);
+  if (pb.wm_update)
+    hipLaunchKernelGGL((k_latent_w<true>), grid, dim3(256), xlds ? xbytes : 0, st, pb, G, sweep, xlds);
+  else
+    hipLaunchKernelGGL((k_latent_w<false>), grid, dim3(256), xlds ? xbytes : 0, st, pb, G, sweep, xlds);
+  if (parts) *parts = pb.wm_update ? nlb : 0;
+  return hipGetLastError();
+}
+
+static hipError_t launch_pairs_w(const DevProblem &pb, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_pairs_w, dim3((unsigned)PAIR_BLOCKS, (unsigned)pb.nchains), dim3(256), 0, st, pb);
+  return hipGetLastError();
+}
+
+static hipError_t launch_coef_w(const DevProblem &pb, int colour, bool with_prec, uint32_t sweep, hipStream_t st,
+                                const Accum &ac, int batch)
+{
+  const int nc = colour ? pb.V - pb.n0 : pb.n0;
+  if (nc > 0) {
+    const size_t lds = (size_t)COEFW_THREADS * coefw_stride(pb.P) * sizeof(float);
+    dim3 grid((unsigned)((nc + COEFW_THREADS - 1) / COEFW_THREADS), (unsigned)pb.nchains);
+    hipLaunchKernelGGL(k_coef_w, grid, dim3(COEFW_THREADS), lds, st, pb, colour, sweep, ac, batch);
+  }
+  if (with_prec) return launch_pairs_w(pb, st);  // the pair sums after colour 1, as the colour-1 kernel forms them
+  return hipGetLastError();
 }
 
 // X staging of either body (the inversion path at G = 4 pads X to 4R quads,
@@ -2109,7 +2735,7 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
 bool latent_carries_wishart(const DevProblem &pb)
 {
   const int G = pb.lat_g > 0 ? pb.lat_g : latent_vl_groups(pb);
-  return G == 4 && pb.latent_method == 0 && latent_vl_lds(pb) <= 65536;
+  return pb.P <= MAX_P && G == 4 && pb.latent_method == 0 && latent_vl_lds(pb) <= 65536;
 }
 
 // Voxel ranges: latent and X'Z launches cover the sorted voxels [v0, v1), so a
@@ -2129,6 +2755,10 @@ hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t
   if (bstat_parts) *bstat_parts = 0;
   if (pieces) *pieces = 0;
   if (!set_range(pb, v0, v1)) return hipErrorInvalidValue;
+  if (pb.P > MAX_P) {  // wide problems: the Z-free fused form over every voxel
+    if (!fused || zstore != 2 || wish || v0 != 0 || v1 != pb.V || pb.latent_method != 0) return hipErrorInvalidValue;
+    return launch_latent_w(pb, sweep, st, bstat_parts);
+  }
   if (v0 == v1 && !wish) return hipSuccess;
   if (fused && zstore == 2) {
     // the Z-free form needs X in LDS; the WM statistic needs the whole voxel range
@@ -2180,6 +2810,7 @@ hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t
 hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st)
 {
   if (!pb.wm_update) return hipSuccess;
+  if (parts <= 0 && pb.P > MAX_P) return hipErrorInvalidValue;  // the wide latent kernel forms the partials
   if (parts <= 0) {
     if (pb.Z == nullptr || pb.NQ <= 0) return hipErrorInvalidValue;
     parts = std::min(BSTAT_SLOTS, (pb.V + 255) / 256);
@@ -2195,7 +2826,7 @@ hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream
 hipError_t launch_xtz(const DevProblem &pb0, int mode, hipStream_t st, int v0, int v1)
 {
   DevProblem pb = pb0;
-  if (!set_range(pb, v0, v1)) return hipErrorInvalidValue;
+  if (!set_range(pb, v0, v1) || pb.P > MAX_P) return hipErrorInvalidValue;
   if (v0 == v1) return hipSuccess;
   if (mode == 1) {
     const long long tiles = ((long long)(v1 - v0) + 15) / 16;
@@ -2229,6 +2860,7 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 {
   const Accum acc = ac ? *ac : Accum{};
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
+  if (pb.P > MAX_P) return launch_coef_w(pb, colour, with_prec, sweep, st, acc, batch);
   if (nc == 0) return with_prec ? hipErrorInvalidValue : hipSuccess;
   const int extra = (colour == 0 && (tail & 2)) ? vl_range(pb, 1).left : 0;
   dim3 grid((unsigned)((nc + 255) / 256 + extra), (unsigned)pb.nchains);
@@ -2251,7 +2883,8 @@ hipError_t launch_coef(const DevProblem &pb, int colour, bool with_prec, uint32_
 // its sweep's Lambda ahead, for the phase 0 after it.
 bool overlap_supported(const DevProblem &pb)
 {
-  return pb.zfree && pb.latent_method == 0 && !pb.wm_update && vl_groups(pb) == 4 && pb.n0 > 0 && pb.V > pb.n0 &&
+  return pb.P <= MAX_P && pb.zfree && pb.latent_method == 0 && !pb.wm_update && vl_groups(pb) == 4 && pb.n0 > 0 &&
+         pb.V > pb.n0 &&
          latent_carries_wishart(pb) && pb.lamtag != nullptr;
 }
 
@@ -2282,6 +2915,7 @@ hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int l
 
 hipError_t launch_pairs(const DevProblem &pb, hipStream_t st)
 {
+  if (pb.P > MAX_P) return launch_pairs_w(pb, st);
   dim3 grid((unsigned)PAIR_BLOCKS, (unsigned)pb.nchains);
 #define L_(PP) hipLaunchKernelGGL(k_pairs<PP>, grid, dim3(256), 0, st, pb)
   BSG_DISPATCH_P(pb.P, L_)
@@ -2292,6 +2926,10 @@ hipError_t launch_pairs(const DevProblem &pb, hipStream_t st)
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st)
 {
   dim3 grid((unsigned)pb.nchains);
+  if (pb.P > MAX_P) {  // always from the PAIR_BLOCKS partials of k_pairs_w
+    hipLaunchKernelGGL(k_wishart_w, grid, dim3(256), 0, st, pb, sweep);
+    return hipGetLastError();
+  }
 #define L_(PP)                                                                   \
   do {                                                                           \
     if (side)                                                                    \
@@ -2307,6 +2945,10 @@ hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipSt
 hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st)
 {
   dim3 grid((unsigned)blocks, (unsigned)pb.nchains);
+  if (pb.P > MAX_P) {
+    hipLaunchKernelGGL(k_cmeans_w, grid, dim3(256), 0, st, pb, part);
+    return hipGetLastError();
+  }
 #define L_(PP) hipLaunchKernelGGL(k_cmeans<PP>, grid, dim3(256), 0, st, pb, part)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
@@ -2320,9 +2962,13 @@ hipError_t launch_loglik(const DevProblem &pb, int T, double *part, double *out,
   if (T < 1 || T > pb.P) return hipErrorInvalidValue;
   const int tiles = loglik_tiles(pb);
   dim3 grid((unsigned)tiles, (unsigned)((pb.N + 255) / 256), (unsigned)pb.nchains);
+  if (pb.P > MAX_P) {
+    hipLaunchKernelGGL(k_loglik_w, grid, dim3(256), 0, st, pb, T, part);
+  } else {
 #define L_(PP) hipLaunchKernelGGL(k_loglik<PP>, grid, dim3(256), 0, st, pb, T, part)
-  BSG_DISPATCH_P(pb.P, L_)
+    BSG_DISPATCH_P(pb.P, L_)
 #undef L_
+  }
   const int NT = pb.N * T;
   hipLaunchKernelGGL(k_llreduce, dim3((unsigned)((NT + 255) / 256), (unsigned)pb.nchains), dim3(256), 0, st, tiles,
                      NT, (const double *)part, out);
@@ -2454,6 +3100,10 @@ size_t latent_vl_lds_bytes(const DevProblem &pb) { return latent_vl_lds(pb); }
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st)
 {
   dim3 grid((unsigned)((pb.V + 255) / 256), (unsigned)pb.nchains);
+  if (pb.P > MAX_P) {
+    hipLaunchKernelGGL(k_accum_w, grid, dim3(256), 0, st, pb, ac, batch);
+    return hipGetLastError();
+  }
 #define L_(PP) hipLaunchKernelGGL(k_accum<PP>, grid, dim3(256), 0, st, pb, ac, batch)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_

@@ -36,7 +36,10 @@ extern "C" {
 #define BSG_ERR_NOMEM (-3) /* host or device allocation failed */
 #define BSG_ERR_NUM (-4)   /* numerical failure reported by a kernel */
 
-#define BSG_MAX_COVAR 8    /* P <= 8 (the reference allows 20: covarGPU.cu:65) */
+#define BSG_MAX_COVAR 20   /* P <= 20, as the reference (covarGPU.cu:65, 300-304). P <= 8 runs the
+                              register-resident kernels in every form; 9 <= P <= 20 runs wide kernels
+                              (P at run time, the P x P factor in LDS) in the Z-free fused form
+                              (xtz_mode 2, z_store 2, latent_method 0), serial order, no batches */
 
 typedef struct bsg_layout bsg_layout;
 typedef struct bsg_ctx bsg_ctx;

@@ -20,7 +20,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbsglmm.so")
+# BSGLMM_LIB selects another build of the library (for example one built with
+# host sanitizers); the default is the in-tree build
+LIB_PATH = os.environ.get("BSGLMM_LIB") or os.path.join(HERE, "libbsglmm.so")
 
 _lib = None
 

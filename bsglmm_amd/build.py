@@ -35,18 +35,29 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False, jobs=None):
-    """Compile every source to an object, then link libbsglmm.so."""
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer (the device code is
+# built as usual). The library is linked without the sanitizer runtime, which
+# the process that loads it preloads, e.g.
+#   BSGLMM_LIB=<out> LD_PRELOAD=<clang>/lib/linux/libclang_rt.asan-x86_64.so \
+#   ASAN_OPTIONS=detect_leaks=0 python -m pytest tests/test_io.py tests/test_layout.py
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+
+
+def build(verbose=False, jobs=None, host_sanitize_out=None):
+    """Compile every source to an object, then link libbsglmm.so (or, with
+    host_sanitize_out, a host-sanitized library at that path, and no BinCar)."""
     cc = hipcc()
-    objdir = os.path.join(HERE, "_build")
+    objdir = os.path.join(HERE, "_build_hostsan" if host_sanitize_out else "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmd = [cc] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c",
-                                  os.path.join(CSRC, src), "-o", obj]
+        flags = HIP_FLAGS + (HOST_SANITIZE if host_sanitize_out else [])
+        cmd = [cc] + flags + ["-I", os.path.join(ROOT, "include"), "-c",
+                              os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -63,6 +74,10 @@ def build(verbose=False, jobs=None):
             print("\n".join(lines))
     if failed:
         raise RuntimeError("hipcc failed on: " + ", ".join(failed))
+    if host_sanitize_out:
+        subprocess.run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", host_sanitize_out] + objs + ["-lz"],
+                       check=True)
+        return host_sanitize_out
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     subprocess.run(cmd, check=True)
@@ -77,4 +92,7 @@ def build(verbose=False, jobs=None):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    out = None
+    if "--host-sanitize" in sys.argv:
+        out = os.path.abspath(sys.argv[sys.argv.index("--host-sanitize") + 1])
+    print(build(verbose="-v" in sys.argv, host_sanitize_out=out))

@@ -187,13 +187,17 @@ int bsg_layout_export(const bsg_layout *layout, int *host_idx, int *host_idx_sc,
     return BSG_ERR_ARG;
   }
   const bsg::Layout &L = layout->L;
-  if (host_idx) memcpy(host_idx, L.host_idx.data(), sizeof(int) * L.nsize);
-  if (host_idx_sc) memcpy(host_idx_sc, L.host_idx_sc.data(), sizeof(int) * L.nsize);
-  if (vox0) memcpy(vox0, L.vox[0].data(), sizeof(int) * L.n[0]);
-  if (vox1) memcpy(vox1, L.vox[1].data(), sizeof(int) * L.n[1]);
-  if (nbrs0) memcpy(nbrs0, L.nbrs[0].data(), L.n[0]);
-  if (nbrs1) memcpy(nbrs1, L.nbrs[1].data(), L.n[1]);
-  if (nbr6) memcpy(nbr6, L.nbr6.data(), sizeof(int) * L.nbr6.size());
+  // a colour list can be empty: copy only what exists (memcpy takes no null source)
+  auto put = [](void *dst, const void *src, size_t bytes) {
+    if (dst && bytes) memcpy(dst, src, bytes);
+  };
+  put(host_idx, L.host_idx.data(), sizeof(int) * L.nsize);
+  put(host_idx_sc, L.host_idx_sc.data(), sizeof(int) * L.nsize);
+  put(vox0, L.vox[0].data(), sizeof(int) * L.n[0]);
+  put(vox1, L.vox[1].data(), sizeof(int) * L.n[1]);
+  put(nbrs0, L.nbrs[0].data(), L.n[0]);
+  put(nbrs1, L.nbrs[1].data(), L.n[1]);
+  put(nbr6, L.nbr6.data(), sizeof(int) * L.nbr6.size());
   return BSG_OK;
 }
 

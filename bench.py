@@ -349,6 +349,10 @@ def uneven_batch(steps):
     t0 = time.perf_counter()
     B.sweep(steps)
     bat = time.perf_counter() - t0
+    B.sweep_graph(10, 10)
+    t0 = time.perf_counter()
+    B.sweep_graph(steps, 10)
+    gra = time.perf_counter() - t0
     B.close()
     for S in samplers:
         S.close()
@@ -360,7 +364,7 @@ def uneven_batch(steps):
             "concurrent_streams_draws_per_s": draws * steps / conc, "batched_over_concurrent_streams": conc / bat,
             "sum_of_tuned_sweep_ms": sum(min(v for v in r["sweep_ms_by_G"].values() if v is not None)
                                          for r in rows),
-            "batch_sweep_ms": 1e3 * bat / steps,
+            "batch_sweep_ms": 1e3 * bat / steps, "graph_batch_sweep_ms": 1e3 * gra / steps,
             "form": "FP32, Z-free fused X'Z (the BF16-split MFMA X'Z needs Z in HBM and is slower, "
                     "see forms.separate_xtz_mfma_bf16x2)"}
 
@@ -407,12 +411,27 @@ def small_batch(steps, count=24):
     t0 = time.perf_counter()
     B.sweep(steps)
     bat = time.perf_counter() - t0
+    # the same batch sweeps replayed from HIP graphs of 50 (one host launch per group)
+    B.sweep_graph(50, 50)
+    t0 = time.perf_counter()
+    B.sweep_graph(steps, 50)
+    gra = time.perf_counter() - t0
+    unified = B.one_launch_per_phase
+    # and one launch per phase and covariate count, each P on its own stream
+    B.set_launch_mode(2)
+    B.sweep(3)
+    t0 = time.perf_counter()
+    B.sweep(steps)
+    per_p = time.perf_counter() - t0
     B.close()
     for S in samplers:
         S.close()
     return {"analyses": count, "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
             "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat,
-            "concurrent_streams_draws_per_s": draws * steps / conc, "batched_over_concurrent_streams": conc / bat}
+            "concurrent_streams_draws_per_s": draws * steps / conc, "batched_over_concurrent_streams": conc / bat,
+            "graph_batch_sweeps_per_s": steps / gra, "graph_over_direct_batch": bat / gra,
+            "graph_over_one_by_one": seq / gra, "one_launch_per_phase": unified,
+            "per_p_launches_batch_sweeps_per_s": steps / per_p}
 
 
 def load_profile(path):

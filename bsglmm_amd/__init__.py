@@ -110,6 +110,9 @@ def lib():
         "bsg_batch_destroy": (C.c_int, [vp]),
         "bsg_batch_sweep": (C.c_int, [vp, C.c_int]),
         "bsg_batch_sweep_accumulate": (C.c_int, [vp, C.c_int, C.c_int]),
+        "bsg_batch_sweep_graph": (C.c_int, [vp, C.c_int, C.c_int]),
+        "bsg_batch_set_launch_mode": (C.c_int, [vp, C.c_int]),
+        "bsg_batch_get_launch_mode": (C.c_int, [vp, pi]),
         "bsg_kiss": (C.c_double, [C.POINTER(C.c_ulong)]),
         "bsg_devseed": (C.c_uint64, [C.POINTER(C.c_ulong)]),
     }
@@ -437,6 +440,21 @@ class Batch:
 
     def sweep_accumulate(self, n, save_every=1):
         _check(lib().bsg_batch_sweep_accumulate(self._h, int(n), int(save_every)))
+
+    def sweep_graph(self, n, per_graph=8):
+        """n batch sweeps replayed from a HIP graph of per_graph sweeps (same results as sweep)."""
+        _check(lib().bsg_batch_sweep_graph(self._h, int(n), int(per_graph)))
+
+    @property
+    def one_launch_per_phase(self):
+        """True when a batch sweep is four launches over every covariate count."""
+        v = C.c_int()
+        _check(lib().bsg_batch_get_launch_mode(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def set_launch_mode(self, mode):
+        """0: chosen from the draws per sweep; 1: one launch per phase; 2: one per phase and P."""
+        _check(lib().bsg_batch_set_launch_mode(self._h, int(mode)))
 
     def close(self):
         if self._h:

@@ -1672,18 +1672,46 @@ struct bsg_batch {
   // the coefficient phases
   int off[5][bsg::MAX_P + 1] = {}, cnt[5][bsg::MAX_P + 1] = {};
   size_t lat_lds[bsg::MAX_P + 1] = {};  // per P: the largest X staging of its members
+  // one launch per phase over every covariate count, on s0 (phases latent,
+  // colour 0, colour 1, Wishart): chosen for batches whose sweep is too small
+  // to keep the chip busy between launches (launch_mode 1, or 0 with few draws)
+  int off_all[4] = {}, cnt_all[4] = {};
+  size_t lds_all = 0;
+  long long draws = 0;  // latent draws per batch sweep
+  int launch_mode = 0;  // 0 chosen from the draws, 1 one launch per phase, 2 one per phase and P
+  bool unified() const { return launch_mode == 1 || (launch_mode == 0 && draws < (1LL << 28)); }
   bool bmf = false;
   std::vector<bsg::Accum> h_accs;
+  // graph-captured groups of batch sweeps (bsg_batch_sweep_graph): the
+  // members' problems again with their Philox sweep word read from gbase,
+  // which the graph's last node advances; s0 forks to and joins the P streams
+  bsg::DevProblem *d_pbs_g = nullptr;
+  uint32_t *gbase = nullptr;
+  uint32_t gstage = 0;
+  hipStream_t s0 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join[bsg::MAX_P + 1] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_k = 0;
 
   ~bsg_batch()
   {
     for (hipStream_t s : sp)
       if (s) hipStreamSynchronize(s);
+    if (s0) hipStreamSynchronize(s0);
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    if (graph) hipGraphDestroy(graph);
     if (d_pbs) hipFree(d_pbs);
+    if (d_pbs_g) hipFree(d_pbs_g);
+    if (gbase) hipFree(gbase);
     if (d_items) hipFree(d_items);
     if (d_accs) hipFree(d_accs);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    for (hipEvent_t e : ev_join)
+      if (e) hipEventDestroy(e);
     for (hipStream_t s : sp)
       if (s) hipStreamDestroy(s);
+    if (s0) hipStreamDestroy(s0);
   }
 };
 
@@ -1746,6 +1774,19 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
         if (pbs[w.a].P == P) all.push_back(w);
       b->cnt[k][P] = (int)all.size() - b->off[k][P];
     }
+  // the same items once more, all covariate counts together, per phase
+  for (int ph = 0; ph < 4; ph++) {
+    b->off_all[ph] = (int)all.size();
+    if (ph == 0) {
+      for (const bsg::WorkItem &w : lists[4]) all.push_back(w);
+      for (const bsg::WorkItem &w : lists[0]) all.push_back(w);
+    } else {
+      for (const bsg::WorkItem &w : lists[ph]) all.push_back(w);
+    }
+    b->cnt_all[ph] = (int)all.size() - b->off_all[ph];
+  }
+  for (int P = 1; P <= bsg::MAX_P; P++) b->lds_all = std::max(b->lds_all, b->lat_lds[P]);
+  for (bsg_ctx *c : b->m) b->draws += (long long)c->C * c->N * c->L.totvox;
   if (hipMalloc((void **)&b->d_pbs, sizeof(bsg::DevProblem) * n) != hipSuccess ||
       hipMalloc((void **)&b->d_items, sizeof(bsg::WorkItem) * all.size()) != hipSuccess ||
       hipMalloc((void **)&b->d_accs, sizeof(bsg::Accum) * n) != hipSuccess) {
@@ -1753,8 +1794,20 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     return BSG_ERR_NOMEM;
   }
   for (int P = 1; P <= bsg::MAX_P; P++)
-    if (b->cnt[0][P] + b->cnt[4][P] > 0) HIPCHK(hipStreamCreateWithFlags(&b->sp[P], hipStreamNonBlocking));
+    if (b->cnt[0][P] + b->cnt[4][P] > 0) {
+      HIPCHK(hipStreamCreateWithFlags(&b->sp[P], hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&b->ev_join[P], hipEventDisableTiming));
+    }
+  HIPCHK(hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
   HIPCHK(hipMemcpy(b->d_pbs, pbs.data(), sizeof(bsg::DevProblem) * n, hipMemcpyHostToDevice));
+  if (hipMalloc((void **)&b->d_pbs_g, sizeof(bsg::DevProblem) * n) != hipSuccess ||
+      hipMalloc((void **)&b->gbase, sizeof(uint32_t)) != hipSuccess) {
+    set_error("bsg_batch_create: hipMalloc failed");
+    return BSG_ERR_NOMEM;
+  }
+  for (bsg::DevProblem &q : pbs) q.sweep_base = b->gbase;
+  HIPCHK(hipMemcpy(b->d_pbs_g, pbs.data(), sizeof(bsg::DevProblem) * n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_items, all.data(), sizeof(bsg::WorkItem) * all.size(), hipMemcpyHostToDevice));
   *out = b.release();
   return BSG_OK;
@@ -1766,9 +1819,33 @@ int bsg_batch_destroy(bsg_batch *b)
   return BSG_OK;
 }
 
-// Sweeps of every analysis and chain, one launch per phase and covariate
+// One sweep of every analysis and chain, one launch per phase and covariate
 // count P: latent draws and X'Z, colour 0, colour 1 (+ pair partials),
-// Wishart, in that order on the stream of P.
+// Wishart, in that order on the stream of P. The kernels read their Philox
+// sweep word as sw + *sweep_base of the problems in pbs.
+static int batch_enqueue(bsg_batch *b, const bsg::DevProblem *pbs, uint32_t sw, const bsg::Accum *accs,
+                         const bsg::BatchIndex &bidx)
+{
+  const bsg::WorkItem *it = b->d_items;
+  if (b->unified()) {  // four launches on s0
+    for (int ph = 0; ph < 4; ph++)
+      HIPCHK(bsg::launch_batch_all(ph, pbs, it + b->off_all[ph], b->cnt_all[ph], b->lds_all, b->bmf, sw, b->s0, accs,
+                                   bidx));
+    return BSG_OK;
+  }
+  for (int P = 1; P <= bsg::MAX_P; P++) {
+    hipStream_t st = b->sp[P];
+    if (!st) continue;
+    HIPCHK(bsg::launch_latent_multi(P, pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds[P], b->bmf, false, sw, st));
+    HIPCHK(bsg::launch_latent_multi(P, pbs, it + b->off[4][P], b->cnt[4][P], b->lat_lds[P], false, true, sw, st));
+    HIPCHK(bsg::launch_coef_multi(P, pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, st, accs, bidx));
+    HIPCHK(bsg::launch_coef_multi(P, pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, st, accs, bidx));
+    HIPCHK(bsg::launch_wishart_multi(P, pbs, it + b->off[3][P], b->cnt[3][P], sw, st));
+  }
+  return BSG_OK;
+}
+
+// Sweeps of every analysis and chain (batch_enqueue per sweep).
 static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
 {
   // the members' own queued work first; every member at the same sweep index,
@@ -1780,7 +1857,6 @@ static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
       return BSG_ERR_ARG;
     }
   }
-  const bsg::WorkItem *it = b->d_items;
   bsg::BatchIndex bidx{};
   for (int i = 0; i < nsweeps; i++) {
     const uint32_t sw = b->m[0]->sweep;
@@ -1794,21 +1870,103 @@ static int batch_run(bsg_batch *b, int nsweeps, int accum_every)
       }
     }
     const bsg::Accum *accs = save ? b->d_accs : nullptr;
-    for (int P = 1; P <= bsg::MAX_P; P++) {
-      hipStream_t st = b->sp[P];
-      if (!st) continue;
-      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds[P], b->bmf, false, sw,
-                                      st));
-      HIPCHK(bsg::launch_latent_multi(P, b->d_pbs, it + b->off[4][P], b->cnt[4][P], b->lat_lds[P], false, true, sw, st));
-      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, st, accs, bidx));
-      HIPCHK(bsg::launch_coef_multi(P, b->d_pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, st, accs, bidx));
-      HIPCHK(bsg::launch_wishart_multi(P, b->d_pbs, it + b->off[3][P], b->cnt[3][P], sw, st));
-    }
+    const int rc = batch_enqueue(b, b->d_pbs, sw, accs, bidx);
+    if (rc != BSG_OK) return rc;
     for (bsg_ctx *c : b->m) c->sweep++;
   }
   for (hipStream_t st : b->sp)
     if (st) HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipStreamSynchronize(b->s0));
   return BSG_OK;
+}
+
+int bsg_batch_set_launch_mode(bsg_batch *b, int mode)
+{
+  if (!b || mode < 0 || mode > 2) {
+    set_error("bsg_batch_set_launch_mode: mode must be 0, 1 or 2");
+    return BSG_ERR_ARG;
+  }
+  for (hipStream_t st : b->sp)
+    if (st) HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipStreamSynchronize(b->s0));
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  b->graph_k = 0;
+  b->launch_mode = mode;
+  return BSG_OK;
+}
+
+int bsg_batch_get_launch_mode(bsg_batch *b, int *unified)
+{
+  if (!b || !unified) {
+    set_error("bsg_batch_get_launch_mode: bad argument");
+    return BSG_ERR_ARG;
+  }
+  *unified = b->unified() ? 1 : 0;
+  return BSG_OK;
+}
+
+// A HIP graph of k batch sweeps: s0 forks to every P stream, each runs its
+// launches for sweeps 0..k-1 (words read as i + *gbase), s0 joins them and
+// advances gbase by k. Captured once per k.
+static int batch_build_graph(bsg_batch *b, int k)
+{
+  if (b->graph_exec && b->graph_k == k) return BSG_OK;
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal));
+  int rc = hipEventRecord(b->ev_fork, b->s0) == hipSuccess ? BSG_OK : BSG_ERR_HIP;
+  for (int P = 1; P <= bsg::MAX_P && rc == BSG_OK; P++)
+    if (b->sp[P] && hipStreamWaitEvent(b->sp[P], b->ev_fork, 0) != hipSuccess) rc = BSG_ERR_HIP;
+  const bsg::BatchIndex bidx{};
+  for (int i = 0; i < k && rc == BSG_OK; i++) rc = batch_enqueue(b, b->d_pbs_g, (uint32_t)i, nullptr, bidx);
+  for (int P = 1; P <= bsg::MAX_P && rc == BSG_OK; P++)
+    if (b->sp[P] && (hipEventRecord(b->ev_join[P], b->sp[P]) != hipSuccess ||
+                     hipStreamWaitEvent(b->s0, b->ev_join[P], 0) != hipSuccess))
+      rc = BSG_ERR_HIP;
+  if (rc == BSG_OK && bsg::launch_tick(b->gbase, (uint32_t)k, b->s0) != hipSuccess) rc = BSG_ERR_HIP;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->s0, &g);
+  if (rc != BSG_OK || ec != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    if (rc == BSG_OK) set_error(std::string("batch graph capture: ") + hipGetErrorString(ec));
+    return rc == BSG_OK ? BSG_ERR_HIP : rc;
+  }
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+  b->graph_k = k;
+  return BSG_OK;
+}
+
+int bsg_batch_sweep_graph(bsg_batch *b, int nsweeps, int sweeps_per_graph)
+{
+  if (!b || nsweeps < 0 || sweeps_per_graph < 1 || sweeps_per_graph > 256) {
+    set_error("bsg_batch_sweep_graph: need nsweeps >= 0 and 1 <= sweeps_per_graph <= 256");
+    return BSG_ERR_ARG;
+  }
+  const int k = sweeps_per_graph, q = nsweeps / k;
+  if (q > 0) {
+    for (bsg_ctx *c : b->m) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (c->sweep != b->m[0]->sweep) {
+        set_error("bsg_batch_sweep: members are at different sweep indices (a member was swept on its own)");
+        return BSG_ERR_ARG;
+      }
+    }
+    const int rc = batch_build_graph(b, k);
+    if (rc != BSG_OK) return rc;
+    b->gstage = b->m[0]->sweep;
+    HIPCHK(hipMemcpyAsync(b->gbase, &b->gstage, sizeof(uint32_t), hipMemcpyHostToDevice, b->s0));
+    for (int j = 0; j < q; j++) HIPCHK(hipGraphLaunch(b->graph_exec, b->s0));
+    for (bsg_ctx *c : b->m) c->sweep += (uint32_t)(q * k);
+    // the staging word is read by the copy above: keep it stable until then
+    HIPCHK(hipStreamSynchronize(b->s0));
+  }
+  return batch_run(b, nsweeps - q * k, 0);
 }
 
 int bsg_batch_sweep(bsg_batch *b, int nsweeps)

@@ -221,6 +221,11 @@ hipError_t launch_coef_multi(int P, const DevProblem *pbs, const WorkItem *items
                              uint32_t sweep, hipStream_t st, const Accum *accs, const BatchIndex &bidx);
 hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,
                                 hipStream_t st);
+// batches in one launch per phase over every covariate count (phase 0 latent,
+// 1 colour 0, 2 colour 1 with the pair partials, 3 Wishart); lds: the
+// largest latent staging of the members
+hipError_t launch_batch_all(int phase, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
+                            uint32_t sweep, hipStream_t st, const Accum *accs, const BatchIndex &bidx);
 // plane means: part[chain][blocks][P] FP64 block partials
 hipError_t launch_cmeans(const DevProblem &pb, double *part, int blocks, hipStream_t st);
 // out[c] = sum of part[b][c] over b < blocks in block order (one chain)

@@ -2098,6 +2098,99 @@ __global__ __launch_bounds__(256) void k_wishart_multi(const DevProblem *pbs, co
 }
 
 
+// ---------------------------------------------------------------------------
+// Batches in one launch per phase: every block reads its item and runs the
+// body of its member's covariate count (a switch on P), so that a batch
+// sweep is four launches on one stream whatever the covariate counts. Each
+// kernel's register allocation is that of its largest body (P = 8); batches
+// of small analyses, which cannot fill the chip and would otherwise wait on
+// one launch per phase and covariate count, gain from the fewer launches.
+// ---------------------------------------------------------------------------
+#define BSG_SWITCH_P(PV, BODY) \
+  switch (PV) {                \
+    case 1: BODY(1); break;    \
+    case 2: BODY(2); break;    \
+    case 3: BODY(3); break;    \
+    case 4: BODY(4); break;    \
+    case 5: BODY(5); break;    \
+    case 6: BODY(6); break;    \
+    case 7: BODY(7); break;    \
+    case 8: BODY(8); break;    \
+    default: break;            \
+  }
+
+template <bool BMF>
+__global__ __launch_bounds__(256) void k_latent_multi_all(const DevProblem *pbs, const WorkItem *items,
+                                                             uint32_t sweep)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  const uint32_t sw = sweep + *pb.sweep_base;
+  // members at four lanes per voxel drawn by inversion: the G = 4 body (colour tiles and pieces)
+  const bool g4 = !BMF && pb.lat_g == 4;
+#define B_(PP)                                                                                        \
+  do {                                                                                                \
+    if (g4) {                                                                                         \
+      if (vl_ctiles(pb)) vl_ctile_block<PP>(pb, sw, it.chain, it.blk, 3);                             \
+      else latent_vl_mfm_body<PP, false>(pb, sw, it.chain, 0, pb.V, it.blk, 1 << 30, 0);              \
+    } else {                                                                                          \
+      latent_vl_body<PP, false, BMF>(pb, pb.lat_g, sw, it.chain, it.blk, 1 << 30, 0);                 \
+    }                                                                                                 \
+  } while (0)
+  BSG_SWITCH_P(pb.P, B_)
+#undef B_
+}
+
+template <bool PREC>
+__global__ __launch_bounds__(256) void k_coef_multi_all(const DevProblem *pbs, const WorkItem *items, int colour,
+                                                        uint32_t sweep, const Accum *accs, BatchIndex bidx)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  const int ncb = ((colour ? pb.V - pb.n0 : pb.n0) + 255) / 256;
+  const Accum ac = accs ? accs[it.a] : Accum{};
+#define B_(PP)                                                                                             \
+  do {                                                                                                     \
+    if (it.blk >= ncb) tail_combine_block<PP>(pb, it.chain, it.blk - ncb);                                 \
+    else coef_body<PP, PREC>(pb, colour, sweep + *pb.sweep_base, it.chain, it.blk,                        \
+                             (colour == 0 && vl_ctiles(pb)) ? 1 : 0, ac, accs ? bidx.b[it.a] : 0);         \
+  } while (0)
+  BSG_SWITCH_P(pb.P, B_)
+#undef B_
+}
+
+__global__ __launch_bounds__(256) void k_wishart_multi_all(const DevProblem *pbs, const WorkItem *items,
+                                                           uint32_t sweep)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+#define B_(PP)                                                                                           \
+  wishart_block<PP>(pb, it.chain, (pb.V - pb.n0 + 255) / 256,                                            \
+                    pb.partials + (size_t)it.chain * pb.max_parts * ((PP) * ((PP) + 1) / 2),             \
+                    sweep + *pb.sweep_base)
+  BSG_SWITCH_P(pb.P, B_)
+#undef B_
+}
+
+hipError_t launch_batch_all(int phase, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
+                            uint32_t sweep, hipStream_t st, const Accum *accs, const BatchIndex &bidx)
+{
+  if (n <= 0) return hipSuccess;
+  if (lds > 65536) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)n);
+  if (phase == 0) {
+    if (bmf) hipLaunchKernelGGL((k_latent_multi_all<true>), grid, dim3(256), lds, st, pbs, items, sweep);
+    else hipLaunchKernelGGL((k_latent_multi_all<false>), grid, dim3(256), lds, st, pbs, items, sweep);
+  } else if (phase == 1) {
+    hipLaunchKernelGGL((k_coef_multi_all<false>), grid, dim3(256), 0, st, pbs, items, 0, sweep, accs, bidx);
+  } else if (phase == 2) {
+    hipLaunchKernelGGL((k_coef_multi_all<true>), grid, dim3(256), 0, st, pbs, items, 1, sweep, accs, bidx);
+  } else {
+    hipLaunchKernelGGL(k_wishart_multi_all, grid, dim3(256), 0, st, pbs, items, sweep);
+  }
+  return hipGetLastError();
+}
+
 // ===========================================================================
 // Wide problems: MAX_P < P <= PW_MAX (20) covariates. The reference's GPU path
 // accepts P <= 20 (per-thread var[20*20], covarGPU.cu:65, 300-304). These

@@ -244,9 +244,13 @@ int bsg_test_overlap_wait(bsg_ctx *ctx, int spin_limit, int tag);
 typedef struct bsg_batch bsg_batch;
 /* Groups up to 32 contexts (independent analyses: own masks, N, P and chain
  * counts) so that each phase of a sweep is one kernel launch over all of
- * them with the same covariate count (one launch per P present, so each has
- * its own register allocation, and each P on its own stream, so that the
- * launches of different P overlap). Members must use the Z-free form (xtz_mode 2, z_store 2), have no WM
+ * them: over every covariate count at once (four launches per batch sweep on
+ * one stream, each block running the body of its member's P), or one launch
+ * per phase and covariate count (each with its own register allocation, each
+ * P on its own stream so that the launches of different P overlap). The
+ * first is chosen for batches of fewer than 2^28 latent draws per sweep,
+ * which cannot keep the chip busy between launches; bsg_batch_set_launch_mode
+ * overrides the choice. Members must use the Z-free form (xtz_mode 2, z_store 2), have no WM
  * image and share the latent method and sweep index; each keeps its own
  * device state and lane split, and must outlive the batch. */
 int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out);
@@ -257,6 +261,17 @@ int bsg_batch_destroy(bsg_batch *batch);
 int bsg_batch_sweep(bsg_batch *batch, int nsweeps);
 /* as bsg_sweep_accumulate for every member (bsg_accum_reset each first) */
 int bsg_batch_sweep_accumulate(bsg_batch *batch, int nsweeps, int save_every);
+/* bsg_batch_sweep through a HIP graph of sweeps_per_graph batch sweeps
+ * (captured once, replayed for every full group; the remainder runs
+ * directly): the host issues one graph launch per group instead of one
+ * launch per phase and covariate count per sweep. Same results. */
+int bsg_batch_sweep_graph(bsg_batch *batch, int nsweeps, int sweeps_per_graph);
+/* mode 0: chosen from the draws per sweep; 1: one launch per phase over every
+ * covariate count; 2: one launch per phase and covariate count. Same results
+ * in every mode. bsg_batch_get_launch_mode tells whether the next sweep runs
+ * one launch per phase (1) or per phase and covariate count (0). */
+int bsg_batch_set_launch_mode(bsg_batch *batch, int mode);
+int bsg_batch_get_launch_mode(bsg_batch *batch, int *unified);
 
 /* ---- measurement utility (no counterpart in the reference) ------------- */
 /* Times reps launches of a float4 stream-copy kernel over `bytes`; gbps is

@@ -30,11 +30,18 @@ def make(ms):
             for k, (d, L, ch) in enumerate(ms)]
 
 
-def test_batch_equals_members_run_alone():
-    print("case batch of 4 analyses (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3, c5_1 N=30 P=3)", flush=True)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_equals_members_run_alone(mode):
+    """mode 1: one launch per phase over every covariate count (each block
+    switches on its member's P); mode 2: one launch per phase and P, each P on
+    its own stream."""
+    print(f"case batch of 4 analyses (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3, c5_1 N=30 P=3), mode {mode}",
+          flush=True)
     ms = members()
     solo, grouped = make(ms), make(ms)
     B = Batch(grouped)
+    B.set_launch_mode(mode)
+    assert B.one_launch_per_phase == (mode == 1)
     for S in solo:
         S.sweep(7)
     B.sweep(7)
@@ -118,3 +125,29 @@ def test_batch_refuses_members_swept_on_their_own():
     B.sweep(1)
     assert A.sweep_index() == Bm.sweep_index() == 4
     B.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_graph_replays_equal_direct_batch_sweeps(mode):
+    """bsg_batch_sweep_graph: groups of batch sweeps replayed from one HIP
+    graph (the stream of each covariate count forked from and joined to one
+    origin stream; the sweep word advanced by the graph's last node) equal
+    direct batch sweeps bit for bit, with direct sweeps and reads between
+    replays."""
+    print("case batch graph vs direct, 4 analyses, three covariate counts", flush=True)
+    ms = members()
+    direct, graphed = make(ms), make(ms)
+    BD, BG = Batch(direct), Batch(graphed)
+    BG.set_launch_mode(mode)
+    for step in (10, 3, 8):  # 10: two replays of 4 + 2 direct; 3: direct only; 8: two replays
+        BD.sweep(step)
+        BG.sweep_graph(step, 4)
+        for S, T in zip(direct, graphed):
+            S.synchronize()
+            T.synchronize()
+            assert S.sweep_index() == T.sweep_index()
+            for c in range(S.nchains):
+                assert np.array_equal(S.get_spat_coef(c), T.get_spat_coef(c))
+                assert np.array_equal(S.get_prec(c), T.get_prec(c))
+    BD.close()
+    BG.close()

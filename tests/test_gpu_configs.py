@@ -97,17 +97,21 @@ def test_c5_twelve_analysis_batch_equals_members_alone():
     Ps = sorted({S.P for S in grouped})
     assert Ps == [2, 3, 4, 5, 6, 7, 8]
     B = Batch(grouped)
-    B.sweep(2)
-    for S in solo:
-        S.sweep(2)
-    for k, (S, T) in enumerate(zip(solo, grouped)):
-        S.synchronize()
-        T.synchronize()
-        assert S.sweep_index() == T.sweep_index() == 2
-        for c in range(4):
-            assert np.array_equal(S.get_spat_coef(c), T.get_spat_coef(c)), (k, c)
-            assert np.array_equal(S.get_prec(c), T.get_prec(c)), (k, c)
-            assert np.all(np.isfinite(T.get_spat_coef(c)))
+    assert not B.one_launch_per_phase  # 3.1e9 draws per sweep: one launch per phase and P
+    # two sweeps one launch per phase and P, then two in one launch per phase
+    for mode, total in ((2, 2), (1, 4)):
+        B.set_launch_mode(mode)
+        B.sweep(2)
+        for S in solo:
+            S.sweep(2)
+        for k, (S, T) in enumerate(zip(solo, grouped)):
+            S.synchronize()
+            T.synchronize()
+            assert S.sweep_index() == T.sweep_index() == total
+            for c in range(4):
+                assert np.array_equal(S.get_spat_coef(c), T.get_spat_coef(c)), (mode, k, c)
+                assert np.array_equal(S.get_prec(c), T.get_prec(c)), (mode, k, c)
+                assert np.all(np.isfinite(T.get_spat_coef(c)))
     B.close()
     for S in solo + grouped:
         S.close()

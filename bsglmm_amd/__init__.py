@@ -106,6 +106,9 @@ def lib():
         "bsg_set_overlap": (C.c_int, [vp, C.c_int]),
         "bsg_test_overlap_wait": (C.c_int, [vp, C.c_int, C.c_int]),
         "bsg_get_overlap": (C.c_int, [vp, pi]),
+        "bsg_set_persistent": (C.c_int, [vp, C.c_int]),
+        "bsg_get_persistent": (C.c_int, [vp, pi]),
+        "bsg_set_persistent_lags": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "bsg_batch_create": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
         "bsg_batch_destroy": (C.c_int, [vp]),
         "bsg_batch_sweep": (C.c_int, [vp, C.c_int]),
@@ -323,6 +326,27 @@ class Sampler:
     @overlap.setter
     def overlap(self, on):
         _check(lib().bsg_set_overlap(self._h, 1 if on else 0))
+
+    @property
+    def persistent(self):
+        """True when the next sweep() runs as one persistent launch (an
+        overlapped one-chain context with P <= 6, asked for by assigning
+        True): resident blocks take the sweeps' latent tiles, colour updates
+        and CAR-precision draws from a queue ordered by what each reads. The
+        chains are the same either way."""
+        on = C.c_int()
+        _check(lib().bsg_get_persistent(self._h, C.byref(on)))
+        return bool(on.value)
+
+    @persistent.setter
+    def persistent(self, on):
+        _check(lib().bsg_set_persistent(self._h, 1 if on else 0))
+
+    def set_persistent_lags(self, latent, update, draw):
+        """Queue lags of the persistent sweep, as fractions of its resident
+        block count: how far behind its inputs each update and draw is
+        placed. They change the speed only."""
+        _check(lib().bsg_set_persistent_lags(self._h, float(latent), float(update), float(draw)))
 
     def coef_means(self, chain=0):
         """Plane means of the coefficients over the mask (alphaMean)."""

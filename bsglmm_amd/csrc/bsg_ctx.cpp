@@ -58,6 +58,17 @@ struct bsg_ctx {
   hipGraphExec_t graph_exec = nullptr;
   int graph_k = 0;
   bool graph_overlap = false;  // the form the graph was captured in
+  // the persistent sweep (k_persist): asked for, its tagged words (allocated
+  // on first use), the queue pattern and the launch epoch
+  bool persist = false;
+  bsg::PTags ptg{};
+  size_t ptg_words = 0;       // tagged words from ptg.bt on (one allocation)
+  uint32_t *ppat = nullptr;   // device copy of the pattern
+  std::vector<uint32_t> ppat_host;
+  int pwindows_extra = 0;     // windows past K that the pattern's offsets need
+  int pblocks = 0;            // resident blocks launched
+  double plag[3] = {1.0, 0.25, 0.1};  // latent, update, draw lags (fractions of pblocks)
+  bool pplan_ok = false;
 
   void free_accum()
   {
@@ -732,6 +743,330 @@ static int enqueue_overlap(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off
   return BSG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The persistent sweep's queue (k_persist). One window holds one relative
+// sweep's items: the latent items of each colour (whole tiles, then pieces),
+// the 256-voxel updates of each colour, the Bartlett factor and the Lambda
+// draw. Each item gets a time in queue positions: the latent items of colour
+// 0, then of colour 1, spread over the window; an update `lat` positions
+// after its own latent items and `upd` after the other colour's updates
+// around it; the draw `upd` after the last update; colour 0's next updates
+// `drw` after the draw; a latent item `upd` after the update of its voxels'
+// last coefficients. lat, upd and drw are fractions of the resident block
+// count (an item holds a block slot about that many positions), so that an
+// item reaches a block about when its inputs are ready. The times are found
+// as a fixed point over the periodic schedule (lags halved until one
+// exists), then the window is sorted by time; an item of time t sits dk =
+// floor(t / window) windows after its own sweep's. Correctness does not rest
+// on the lags: check_persist_pattern verifies that every item comes after
+// each item it waits for (the kernel's waits are the dependencies listed
+// there), which is all that the queue needs to drain.
+// ---------------------------------------------------------------------------
+struct PItemRef {
+  int type, idx;
+};
+
+struct PersistDeps {
+  int nl[2] = {0, 0}, nu[2] = {0, 0};
+  std::vector<int> lat_ublk[2];               // latent item -> update block of its voxels
+  std::vector<std::vector<int>> lat_of[2];    // update block -> its latent items
+  std::vector<std::pair<int, int>> nbr[2];    // update block -> range of the other colour's blocks around it
+};
+
+static void persist_deps(const bsg_ctx *ctx, const bsg::DevProblem &pb, PersistDeps &D)
+{
+  const int n0 = ctx->L.n[0], V = ctx->L.totvox;
+  for (int c = 0; c < 2; c++) {
+    const bsg::VlRange r = bsg::vl_range(pb, c);
+    D.nl[c] = r.nmain + r.left * r.K;
+    const int nv = c ? V - n0 : n0;
+    D.nu[c] = (nv + 255) / 256;
+    D.lat_ublk[c].resize(D.nl[c]);
+    D.lat_of[c].assign(D.nu[c], {});
+    for (int i = 0; i < D.nl[c]; i++) {
+      const int tile = i < r.nmain ? i : r.nmain + (i - r.nmain) / r.K;
+      const int u = std::min(tile / 4, D.nu[c] - 1);
+      D.lat_ublk[c][i] = u;
+      D.lat_of[c][u].push_back(i);
+    }
+    D.nbr[c].assign(D.nu[c], {INT32_MAX, -1});
+    const int base = c ? n0 : 0, obase = c ? 0 : n0;
+    for (int q = 0; q < nv; q++) {
+      const int s = base + q, u = q / 256;
+      for (int d = 0; d < 6; d++) {
+        const int t = ctx->L.nbr6[(size_t)s * 6 + d];
+        if (t < 0) continue;
+        const int v = (t - obase) / 256;
+        D.nbr[c][u].first = std::min(D.nbr[c][u].first, v);
+        D.nbr[c][u].second = std::max(D.nbr[c][u].second, v);
+      }
+    }
+  }
+}
+
+// every item after each item it waits for, over enough sweeps to see every
+// offset; entries: dk << 28 | type << 24 | index
+static bool check_persist_pattern(const PersistDeps &D, const std::vector<uint32_t> &pat, int dkmax)
+{
+  const int plen = (int)pat.size();
+  // rank of each item in the window, and its offset
+  std::vector<int> rl[2], ru[2], dl[2], du[2];
+  int rw = -1, rb = -1, dw = 0, db = 0;
+  for (int c = 0; c < 2; c++) {
+    rl[c].assign(D.nl[c], -1);
+    ru[c].assign(D.nu[c], -1);
+    dl[c].assign(D.nl[c], 0);
+    du[c].assign(D.nu[c], 0);
+  }
+  for (int j = 0; j < plen; j++) {
+    const uint32_t e = pat[j];
+    const int dk = (int)(e >> 28), type = (int)((e >> 24) & 15), idx = (int)(e & 0xFFFFFF);
+    switch (type) {
+    case bsg::PI_L0: case bsg::PI_L1: rl[type][idx] = j; dl[type][idx] = dk; break;
+    case bsg::PI_U0: ru[0][idx] = j; du[0][idx] = dk; break;
+    case bsg::PI_U1: ru[1][idx] = j; du[1][idx] = dk; break;
+    case bsg::PI_W: rw = j; dw = dk; break;
+    default: rb = j; db = dk; break;
+    }
+  }
+  for (int c = 0; c < 2; c++) {
+    for (int x : rl[c]) if (x < 0) return false;
+    for (int x : ru[c]) if (x < 0) return false;
+  }
+  if (rw < 0 || rb < 0) return false;
+  auto pos = [&](int k, int rank, int dk) { return (long long)(k + dk) * plen + rank; };
+  const int Kv = dkmax + 4;
+  for (int k = 0; k < Kv; k++) {
+    for (int c = 0; c < 2; c++) {
+      for (int i = 0; i < D.nl[c]; i++) {  // latent item: the update of k - 1 of its voxels
+        const long long p = pos(k, rl[c][i], dl[c][i]);
+        const int u = D.lat_ublk[c][i];
+        if (k >= 1 && pos(k - 1, ru[c][u], du[c][u]) >= p) return false;
+      }
+      for (int u = 0; u < D.nu[c]; u++) {  // update: its latent items, the other colour around it, Lambda
+        const long long p = pos(k, ru[c][u], du[c][u]);
+        for (int i : D.lat_of[c][u])
+          if (pos(k, rl[c][i], dl[c][i]) >= p) return false;
+        const int ko = c ? k : k - 1, o = 1 - c;
+        if (ko >= 0)
+          for (int v = D.nbr[c][u].first; v <= D.nbr[c][u].second; v++)
+            if (pos(ko, ru[o][v], du[o][v]) >= p) return false;
+        if (k >= 1 && pos(k - 1, rw, dw) >= p) return false;
+      }
+    }
+    const long long pw = pos(k, rw, dw);  // the draw: every update of k and the Bartlett factor
+    for (int c = 0; c < 2; c++)
+      for (int u = 0; u < D.nu[c]; u++)
+        if (pos(k, ru[c][u], du[c][u]) >= pw) return false;
+    if (pos(k, rb, db) >= pw) return false;
+    if (k >= 2 && pos(k - 2, rw, dw) >= pos(k, rb, db)) return false;  // Bartlett: the draw of k - 2
+  }
+  return true;
+}
+
+static int build_persist_pattern(bsg_ctx *ctx, const bsg::DevProblem &pb)
+{
+  PersistDeps D;
+  persist_deps(ctx, pb, D);
+  const int T = D.nl[0] + D.nl[1] + D.nu[0] + D.nu[1] + 2;
+  if (T >= (1 << 24) || D.nu[0] < 1 || D.nu[1] < 1) {
+    set_error("persistent sweep: problem too large for the queue pattern");
+    return BSG_ERR_ARG;
+  }
+  const double slots = std::max(1, ctx->pblocks);
+  std::vector<double> tl[2], tu[2];
+  double tw = 0.0, tb = 0.0;
+  bool found = false;
+  double scale = 1.0;
+  for (int attempt = 0; attempt < 40 && !found; attempt++, scale *= 0.5) {
+    const double DL = std::max(1.0, scale * ctx->plag[0] * slots), DU = std::max(1.0, scale * ctx->plag[1] * slots),
+                 DW = std::max(1.0, scale * ctx->plag[2] * slots);
+    std::vector<double> b[2];
+    for (int c = 0; c < 2; c++) {
+      b[c].resize(D.nl[c]);
+      for (int i = 0; i < D.nl[c]; i++) b[c][i] = c * 0.5 * T + (i + 0.5) * 0.5 * T / D.nl[c];
+      tl[c] = b[c];
+      tu[c].assign(D.nu[c], 0.0);
+    }
+    tw = 0.0;
+    tb = 0.0;
+    bool stable = false;
+    for (int it = 0; it < 200 && !stable; it++) {
+      double change = 0.0;
+      auto upd = [&](double &x, double v) {
+        change = std::max(change, std::fabs(v - x));
+        x = v;
+      };
+      for (int c = 0; c < 2; c++)
+        for (int u = 0; u < D.nu[c]; u++) {
+          double t = tw - T + DW;
+          for (int i : D.lat_of[c][u]) t = std::max(t, tl[c][i] + DL);
+          const int o = 1 - c;
+          const double shift = c ? 0.0 : -(double)T;  // colour 0 waits for colour 1 of the sweep before
+          for (int v = D.nbr[c][u].first; v <= D.nbr[c][u].second; v++) t = std::max(t, tu[o][v] + shift + DU);
+          upd(tu[c][u], t);
+        }
+      double w = tb + 1.0;
+      for (int c = 0; c < 2; c++)
+        for (double x : tu[c]) w = std::max(w, x + DU);
+      upd(tw, w);
+      upd(tb, tw - T + 1.0);
+      for (int c = 0; c < 2; c++)
+        for (int i = 0; i < D.nl[c]; i++) upd(tl[c][i], std::max(b[c][i], tu[c][D.lat_ublk[c][i]] - T + DU));
+      stable = change < 1e-9;
+      if (tw > 64.0 * T) break;  // not periodic at this lag
+    }
+    found = stable;
+  }
+  if (!found) {
+    set_error("persistent sweep: no periodic schedule");
+    return BSG_ERR_ARG;
+  }
+  // window offsets and the sorted window
+  struct E {
+    double t;
+    int dk;
+    uint32_t code;
+  };
+  std::vector<E> es;
+  es.reserve(T);
+  double tmin = tw;
+  for (int c = 0; c < 2; c++) {
+    for (double x : tl[c]) tmin = std::min(tmin, x);
+    for (double x : tu[c]) tmin = std::min(tmin, x);
+  }
+  tmin = std::min(tmin, tb);
+  const double t0 = std::floor(tmin / T) * T;
+  auto add = [&](double t, uint32_t type, int idx) {
+    const double r = t - t0;
+    const int dk = (int)std::floor(r / T);
+    es.push_back({r - (double)dk * T, dk, (type << 24) | (uint32_t)idx});
+  };
+  for (int c = 0; c < 2; c++) {
+    for (int i = 0; i < D.nl[c]; i++) add(tl[c][i], c ? bsg::PI_L1 : bsg::PI_L0, i);
+    for (int u = 0; u < D.nu[c]; u++) add(tu[c][u], c ? bsg::PI_U1 : bsg::PI_U0, u);
+  }
+  add(tw, bsg::PI_W, 0);
+  add(tb, bsg::PI_BT, 0);
+  int dkmax = 0;
+  for (const E &e : es) dkmax = std::max(dkmax, e.dk);
+  if (dkmax > 15) {
+    set_error("persistent sweep: schedule spans too many windows");
+    return BSG_ERR_ARG;
+  }
+  std::stable_sort(es.begin(), es.end(), [](const E &a, const E &b) { return a.t < b.t; });
+  std::vector<uint32_t> pat(es.size());
+  for (size_t j = 0; j < es.size(); j++) pat[j] = ((uint32_t)es[j].dk << 28) | es[j].code;
+  if (!check_persist_pattern(D, pat, dkmax)) {
+    set_error("persistent sweep: the queue pattern puts an item before one it waits for");
+    return BSG_ERR_ARG;
+  }
+  if (ctx->ppat && ctx->ppat_host.size() != pat.size()) {
+    hipFree(ctx->ppat);
+    auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), (void *)ctx->ppat);
+    if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+    ctx->ppat = nullptr;
+  }
+  if (!ctx->ppat) {
+    HIPCHK(hipMalloc((void **)&ctx->ppat, sizeof(uint32_t) * pat.size()));
+    ctx->allocs.push_back(ctx->ppat);
+  }
+  HIPCHK(hipMemcpy(ctx->ppat, pat.data(), sizeof(uint32_t) * pat.size(), hipMemcpyHostToDevice));
+  ctx->ppat_host = pat;
+  ctx->pwindows_extra = dkmax;
+  ctx->pplan_ok = true;
+  return BSG_OK;
+}
+
+// the tagged words, the queue head and the pattern, on first use
+static int persist_setup(bsg_ctx *ctx)
+{
+  const bsg::DevProblem &pb = ctx->pb;
+  if (!ctx->ptg.bt) {
+    const size_t V = ctx->L.totvox, P = ctx->P, NE = P * (P + 1) / 2;
+    const size_t nu0 = (ctx->L.n[0] + 255) / 256, nu1 = (ctx->L.n[1] + 255) / 256;
+    const size_t n_bt = 2 * V * P, n_mt = V * P, n_mpt = (size_t)std::max(pb.mp_stride, 1),
+                 n_pt = nu1 * 2 * NE, n_lt = 2 * P * P, n_wb = 4 * P * P, n_u0 = nu0;
+    if (!ctx->pblocks) ctx->pblocks = bsg::persist_max_blocks(pb);
+    const size_t n_dg = 4 + 2 * (size_t)std::max(ctx->pblocks, 1) + 4 * 256;  // the wait diagnosis
+    const size_t words = n_bt + n_mt + n_mpt + n_pt + n_lt + n_wb + n_u0 + n_dg;
+    unsigned long long *w = nullptr;
+    HIPCHK(hipMalloc((void **)&w, sizeof(unsigned long long) * words + 256));
+    ctx->allocs.push_back(w);
+    HIPCHK(hipMemset(w, 0, sizeof(unsigned long long) * words + 256));
+    bsg::PTags &t = ctx->ptg;
+    t.bt = w;
+    t.mt = t.bt + n_bt;
+    t.mpt = t.mt + n_mt;
+    t.pt = t.mpt + n_mpt;
+    t.lt = t.pt + n_pt;
+    t.wb = t.lt + n_lt;
+    t.u0d = t.wb + n_wb;
+    ctx->pb.pdiag = t.u0d + n_u0;
+    t.head = reinterpret_cast<uint32_t *>(t.u0d + n_u0 + n_dg);
+    t.epoch = 0;
+    ctx->ptg_words = words;
+  }
+  if (!ctx->pblocks) ctx->pblocks = bsg::persist_max_blocks(pb);
+  if (ctx->pblocks < 1) {
+    set_error("persistent sweep: the kernel cannot be resident on this device");
+    return BSG_ERR_HIP;
+  }
+  if (!ctx->pplan_ok) return build_persist_pattern(ctx, pb);
+  return BSG_OK;
+}
+
+static bool use_persist(const bsg_ctx *ctx)
+{
+  return ctx->persist && use_overlap(ctx) && bsg::persist_supported(ctx->pb);
+}
+
+// k sweeps of the persistent kernel (in launches of at most PERSIST_MAX_K);
+// ev (optional, 2 per sweep): the first pair brackets each launch, the others
+// are recorded back to back (so their sum is the launches' time)
+static int enqueue_persist(bsg_ctx *ctx, int k, hipEvent_t *ev)
+{
+  int rc = persist_setup(ctx);
+  if (rc != BSG_OK) return rc;
+  hipStream_t sa = ctx->stream;
+  int done = 0;
+  const int plen = (int)ctx->ppat_host.size();
+  const int kmax = std::min(bsg::PERSIST_MAX_K, (int)((1u << 31) / (uint32_t)plen) - ctx->pwindows_extra);
+  if (kmax < 1) {
+    set_error("persistent sweep: problem too large for one launch");
+    return BSG_ERR_ARG;
+  }
+  while (done < k) {
+    const int kk = std::min(k - done, kmax);
+    bsg::PTags t = ctx->ptg;
+    if (++ctx->ptg.epoch > (uint32_t)bsg::PERSIST_MAX_EPOCH) {
+      // every tag of the earlier launches cleared before the epochs repeat
+      HIPCHK(bsg::launch_zero64(ctx->ptg.bt, ctx->ptg_words, sa));
+      ctx->ptg.epoch = 1;
+    }
+    t.epoch = ctx->ptg.epoch;
+    t.pattern = ctx->ppat;
+    t.plen = plen;
+    t.K = kk;
+    t.windows = kk + ctx->pwindows_extra;
+    t.sweep0 = ctx->sweep + (uint32_t)done;
+    t.tail = (ctx->pb.tl_k[0] > 1 ? 1 : 0) | (ctx->pb.tl_k[1] > 1 ? 2 : 0);
+    if (ev) HIPCHK(hipEventRecord(ev[2 * done], sa));
+    HIPCHK(bsg::launch_persist(ctx->pb, t, ctx->pblocks, sa));
+    if (ev) {
+      HIPCHK(hipEventRecord(ev[2 * done + 1], sa));
+      for (int i = done + 1; i < done + kk; i++) {
+        HIPCHK(hipEventRecord(ev[2 * i], sa));
+        HIPCHK(hipEventRecord(ev[2 * i + 1], sa));
+      }
+    }
+    done += kk;
+  }
+  ctx->tail_pending = 0;
+  ctx->overlap_ran = true;
+  return BSG_OK;
+}
+
 static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_every = 0)
 {
   hipStream_t sa = ctx->stream;
@@ -739,6 +1074,11 @@ static int sweep_run(bsg_ctx *ctx, int nsweeps, hipEvent_t *lat_ev, int accum_ev
   if (accum_every > 0 && !ctx->acc.sum) {
     set_error("posterior accumulation: call bsg_accum_reset first");
     return BSG_ERR_ARG;
+  }
+  if (accum_every == 0 && use_persist(ctx)) {
+    const int rc = enqueue_persist(ctx, nsweeps, lat_ev);
+    if (rc == BSG_OK) ctx->sweep += (uint32_t)nsweeps;
+    return rc;
   }
   if (use_overlap(ctx)) {
     const int rc = enqueue_overlap(ctx, ctx->pb, ctx->sweep, nsweeps, lat_ev, accum_every);
@@ -997,10 +1337,52 @@ int bsg_synchronize(bsg_ctx *ctx)
     // first: a wait that gave up leaves colour 0 with an incomplete precision,
     // which can also show as the failure below; both are reported here, once,
     // and the next sweeps start with the flags clear
-    const uint32_t zero[2] = {0, 0};
+    const uint32_t zero[3] = {0, 0, 0};
     HIPCHK(hipMemcpy(ctx->pb.flags + 1, zero, sizeof(zero), hipMemcpyHostToDevice));
-    set_error("overlapped sweep: a colour-0 block waited past its bound for the CAR precision drawn in the "
-              "same launch");
+    std::string where;
+    if (flags[3] && ctx->pb.pdiag) {
+      const bsg::PTags &t = ctx->ptg;
+      const struct {
+        const char *name;
+        const void *p;
+      } bufs[] = {{"coefficients", t.bt}, {"X'Z", t.mt}, {"piece partials", t.mpt}, {"pair partials", t.pt},
+                  {"Lambda", t.lt}, {"Bartlett factor", t.wb}, {"colour-0 done words", t.u0d},
+                  {"diagnosis", ctx->pb.pdiag}};
+      auto name_of = [&](unsigned long long a, long long &off) {
+        for (int b = 6; b >= 0; b--) {
+          const unsigned long long base = (unsigned long long)(uintptr_t)bufs[b].p;
+          if (base && a >= base && a < (unsigned long long)(uintptr_t)bufs[b + 1].p) {
+            off = (long long)((a - base) / 8);
+            return bufs[b].name;
+          }
+        }
+        off = -1;
+        return "k_overlap Lambda";
+      };
+      static const char *const items[] = {"L0", "L1", "U0", "U1", "W", "BT"};
+      const size_t nb = ctx->pblocks, nrec = std::min<uint32_t>(flags[3], 16);  // the first 16 in the message
+      std::vector<unsigned long long> all(4 + 2 * nb + 4 * 256);
+      HIPCHK(hipMemcpy(all.data(), ctx->pb.pdiag, sizeof(unsigned long long) * all.size(), hipMemcpyDeviceToHost));
+      const unsigned long long *rec = all.data() + 4 + 2 * nb;
+      char buf[400];
+      snprintf(buf, sizeof(buf), " (%u waits past their bound", flags[3]);
+      where = buf;
+      for (size_t r = 0; r < nrec; r++) {
+        const unsigned long long *x = rec + 4 * r;
+        long long off = 0;
+        const char *nm = name_of(x[0], off);
+        const uint32_t e = (uint32_t)x[3];
+        snprintf(buf, sizeof(buf), "; %s word %lld wants %#llx has %#llx in %s(k=%u) #%u dk %u", nm, off,
+                 (unsigned long long)x[1], x[2] >> 32, items[std::min<uint32_t>((e >> 24) & 15, 5)],
+                 (uint32_t)(x[3] >> 32), e & 0xFFFFFFu, e >> 28);
+        where += buf;
+      }
+      where += ")";
+      HIPCHK(hipMemset(ctx->pb.pdiag + 4 + nb, 0, sizeof(unsigned long long) * (nb + 4 * 256)));
+      HIPCHK(hipMemset(ctx->pb.flags + 3, 0, sizeof(uint32_t)));
+    }
+    set_error("overlapped or persistent sweep: a block waited past its bound for a value drawn in the same "
+              "launch" + where);
     return BSG_ERR_HIP;
   }
   if (flags[1]) {
@@ -1547,6 +1929,38 @@ int bsg_test_overlap_wait(bsg_ctx *ctx, int spin_limit, int tag)
   drop_graph(ctx);
   ctx->pb.lam_spin = spin_limit;
   ctx->pb.lam_tag = tag;
+  return BSG_OK;
+}
+
+int bsg_set_persistent(bsg_ctx *ctx, int on)
+{
+  CTXCHK(ctx);
+  ctx->persist = on != 0;
+  return BSG_OK;
+}
+
+int bsg_get_persistent(bsg_ctx *ctx, int *on)
+{
+  CTXCHK(ctx);
+  if (!on) {
+    set_error("bsg_get_persistent: null argument");
+    return BSG_ERR_ARG;
+  }
+  *on = use_persist(ctx) ? 1 : 0;
+  return BSG_OK;
+}
+
+int bsg_set_persistent_lags(bsg_ctx *ctx, double latent, double update, double draw)
+{
+  CTXCHK(ctx);
+  if (!(latent > 0.0 && update > 0.0 && draw > 0.0) || latent > 64.0 || update > 64.0 || draw > 64.0) {
+    set_error("bsg_set_persistent_lags: lags must lie in (0, 64]");
+    return BSG_ERR_ARG;
+  }
+  ctx->plag[0] = latent;
+  ctx->plag[1] = update;
+  ctx->plag[2] = draw;
+  ctx->pplan_ok = false;
   return BSG_OK;
 }
 

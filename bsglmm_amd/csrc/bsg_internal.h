@@ -94,6 +94,38 @@ struct DevProblem {
   // the tag the draw writes (0: 1, the one the readers wait for). Set from the
   // host only to test the bound (bsg_test_overlap_wait).
   int lam_spin, lam_tag;
+  // diagnosis of the persistent sweep's waits (null outside it), nb blocks:
+  // [4 + b] the item block b runs (k << 32 | queue entry), [4 + nb + b] the
+  // item it last recorded, then from [4 + 2 nb] four words per wait past its
+  // bound (one per item of a block; flags[3] counts them): the word's
+  // address, the tag waited for, the word read last, the item.
+  unsigned long long *pdiag;
+};
+
+// The persistent sweep (k_persist, one chain): every item of K sweeps in one
+// launch, taken from a work queue in an order that puts each item after
+// everything it reads. Values passed between the blocks of the launch travel
+// as tagged words: the tag (high 32 bits) names the relative sweep that wrote
+// the payload (low 32 bits), so a reader waits for its own tag and needs no
+// fence; FP64 values take two words (high half, low half).
+// Item types of the queue pattern (entry = dk << 28 | type << 24 | index):
+enum PItem : uint32_t { PI_L0 = 0, PI_L1 = 1, PI_U0 = 2, PI_U1 = 3, PI_W = 4, PI_BT = 5 };
+constexpr int PERSIST_MAX_K = 65534;  // sweeps per launch (16-bit relative sweep in the tag)
+constexpr int PERSIST_MAX_EPOCH = 32767;  // launches before the tagged words are cleared
+struct PTags {
+  unsigned long long *bt;   // [2][V][P] coefficients; buffer k & 1 written by the updates of relative sweep k
+  unsigned long long *mt;   // [V][P] X'Z of whole tiles
+  unsigned long long *mpt;  // partial X'Z of the pieces, laid out as mpart (one chain)
+  unsigned long long *pt;   // [colour-1 blocks][2 NE] pair partials (FP64)
+  unsigned long long *lt;   // [2][P][P] Lambda drawn at relative sweep k, buffer k & 1
+  unsigned long long *wb;   // [2][P][P][2] Bartlett factor (FP64) of relative sweep k, buffer k & 1
+  unsigned long long *u0d;  // [colour-0 blocks] the colour-0 update of the block is done
+  uint32_t *head;           // next queue entry (zero at launch)
+  const uint32_t *pattern;  // one window of the queue: plen entries
+  int plen, windows, K;     // entries per window, windows, sweeps of the launch
+  uint32_t epoch;           // 1 .. PERSIST_MAX_EPOCH: tag = epoch << 16 | (k + 1)
+  uint32_t sweep0;          // Philox sweep word of relative sweep 0, before *sweep_base
+  int tail;                 // bit c: colour c has leftover tiles in pieces
 };
 
 // Lanes per voxel of the Z-free latent kernels. Inversion draws take G = 4 at
@@ -191,6 +223,14 @@ bool latent_carries_wishart(const DevProblem &pb);
 bool overlap_supported(const DevProblem &pb);
 hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int lat, int wish, uint32_t wsweep,
                           int tail, hipStream_t st, const Accum *ac = nullptr, int batch = 0);
+// The persistent sweep: the overlapped form's conditions with one chain (colour
+// tiles) and P <= 6. launch_persist clears the queue head, then runs nblocks
+// persistent blocks over tg.windows windows of the pattern.
+bool persist_supported(const DevProblem &pb);
+hipError_t launch_persist(const DevProblem &pb, const PTags &tg, int nblocks, hipStream_t st);
+int persist_max_blocks(const DevProblem &pb);  // resident blocks of the kernel on this device
+// zero n tagged words (a fill kernel: no memset node if captured)
+hipError_t launch_zero64(unsigned long long *p, size_t n, hipStream_t st);
 // WM coefficient draw. parts > 0: the latent kernel already wrote that many
 // partials; parts == 0: they are formed here from the stored Z first.
 hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st);

@@ -18,6 +18,9 @@
 // (index, sweep, tag | sub << 4, chain); no per-thread generator state.
 #include "bsg_internal.h"
 
+// scope of the tagged words' loads and stores
+#define TAG_SCOPE __HIP_MEMORY_SCOPE_AGENT
+
 #include <algorithm>
 #include <mutex>
 
@@ -85,6 +88,30 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NE], double *dst, 
   if (threadIdx.x < NE && (int)threadIdx.x < count) {
     const int e = threadIdx.x;
     dst[e] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+  }
+}
+
+// block_sum_store's sums, each written as two tagged words (high, low half)
+template <int NE>
+__device__ __forceinline__ void block_sum_store_tg(double (&acc)[NE], unsigned long long *dst, uint32_t tag)
+{
+  __shared__ double red[4][NE];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    double v = acc[e];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wv][e] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NE) {
+    const int e = threadIdx.x;
+    const double v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    __hip_atomic_store(dst + 2 * e, ((unsigned long long)tag << 32) | (b >> 32), __ATOMIC_RELAXED,
+                       TAG_SCOPE);
+    __hip_atomic_store(dst + 2 * e + 1, ((unsigned long long)tag << 32) | (b & 0xFFFFFFFFull), __ATOMIC_RELAXED,
+                       TAG_SCOPE);
   }
 }
 
@@ -700,14 +727,121 @@ __device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint
 // ---------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int P, bool WMU>
-__device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_t sweep, int chain, int vb, int ve,
-                                                   int t0, int tstep, int slot, int ra = 0, int rb = -1,
-                                                   float *part = nullptr)
+// ---------------------------------------------------------------------------
+// Tagged words of the persistent sweep (PTags). Writers store (tag << 32 |
+// payload) with one 64-bit agent-scope store; readers load with agent-scope
+// loads until the word shows the tag they need. Tag and payload arrive
+// together, so no fence orders anything. A wait is bounded: past pb.lam_spin
+// polls (0: PT_SPIN_MAX), or once another wait has given up (flags[2]), it
+// sets flags[2] (the host reports it) and goes on, so every block ends.
+// ---------------------------------------------------------------------------
+constexpr int PT_SPIN_MAX = 1 << 20;
+constexpr uint32_t PT_DIAG_RECS = 256;  // waits past their bound recorded per launch
+__device__ __forceinline__ uint32_t ptag(const PTags &tg, int k)
 {
-  static_assert(P <= 8, "means take at most two MFMAs");
-  constexpr bool MFM = P >= 3;
-  extern __shared__ float4 xs[];  // [4R][P]
+  return (tg.epoch << 16) | (uint32_t)(k + 1);
+}
+__device__ __forceinline__ unsigned long long tg_ld(const unsigned long long *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, TAG_SCOPE);
+}
+__device__ __forceinline__ void tg_st(unsigned long long *p, uint32_t tag, uint32_t bits)
+{
+  __hip_atomic_store(p, ((unsigned long long)tag << 32) | bits, __ATOMIC_RELAXED, TAG_SCOPE);
+}
+__device__ __forceinline__ void tg_std(unsigned long long *p, uint32_t tag, double v)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  tg_st(p, tag, (uint32_t)(b >> 32));
+  tg_st(p + 1, tag, (uint32_t)b);
+}
+__device__ __forceinline__ double tg_hilo(uint32_t hi, uint32_t lo)
+{
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ bool tg_abort(const DevProblem &pb)
+{
+  return __hip_atomic_load(pb.flags + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+// payloads of words p[0 .. N), every one showing `tag`. While some lane of
+// the wave still misses its tag, the wave polls one word only (the first
+// missing word of the first such lane: one cache line per poll, so that many
+// waiting waves do not crowd out the producers' memory traffic), then reads
+// every lane's words again.
+template <int N>
+__device__ __forceinline__ void tg_wait(const unsigned long long *p, uint32_t tag, const DevProblem &pb,
+                                        uint32_t (&out)[N])
+{
+  unsigned long long v[N];
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < N; c++) {
+    v[c] = tg_ld(p + c);
+    ok = ok && (uint32_t)(v[c] >> 32) == tag;
+  }
+  if (__ballot(!ok) != 0ull) {
+    const int lim = pb.lam_spin > 0 ? pb.lam_spin : PT_SPIN_MAX;
+    int it = 0;
+    bool gave_up = false;
+    while (__ballot(!ok) != 0ull) {
+      // the representative word: first missing word of the first missing lane
+      int cm = 0;
+      while (cm < N - 1 && (uint32_t)(v[cm] >> 32) == tag) cm++;
+      const unsigned long long mine = ok ? 0ull : (unsigned long long)(p + cm);
+      const int src = __ffsll((long long)__ballot(!ok)) - 1;
+      const unsigned long long *rep = (const unsigned long long *)__shfl(mine, src, 64);
+      for (;; it++) {
+        const bool past = it >= lim;
+        if (past || (pb.lam_spin == 0 && (it & 31) == 31 && tg_abort(pb))) {
+          gave_up = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(16);
+        if ((uint32_t)(tg_ld(rep) >> 32) == tag) break;
+      }
+      ok = true;
+#pragma unroll
+      for (int c = 0; c < N; c++) {
+        v[c] = tg_ld(p + c);
+        ok = ok && (uint32_t)(v[c] >> 32) == tag;
+      }
+      if (gave_up) break;
+    }
+    if (gave_up && !ok) {
+      const unsigned long long curi = pb.pdiag ? pb.pdiag[4 + blockIdx.x] : 0ull;
+      // one record per item of a block: the first of its waits past the bound
+      if (it >= lim && pb.pdiag && atomicExch(pb.pdiag + 4 + gridDim.x + blockIdx.x, curi) != curi) {
+        const uint32_t slot = atomicAdd(&pb.flags[3], 1u);
+        if (slot < PT_DIAG_RECS) {
+          int c = 0;  // the first word that does not show the tag
+          while (c < N - 1 && (uint32_t)(v[c] >> 32) == tag) c++;
+          unsigned long long *r = pb.pdiag + 4 + 2 * gridDim.x + 4 * slot;
+          r[0] = (unsigned long long)(p + c);
+          r[1] = tag;
+          r[2] = v[c];
+          r[3] = curi;  // the item the block runs
+        }
+      }
+      atomicOr(&pb.flags[2], 1u);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < N; c++) out[c] = (uint32_t)v[c];
+}
+template <int N>
+__device__ __forceinline__ void tg_waitf(const unsigned long long *p, uint32_t tag, const DevProblem &pb,
+                                         float (&out)[N])
+{
+  uint32_t w[N];
+  tg_wait<N>(p, tag, pb, w);
+#pragma unroll
+  for (int c = 0; c < N; c++) out[c] = __uint_as_float(w[c]);
+}
+
+// X of the G = 4 inversion path in LDS: [4R][P] quads, zero past NQ
+template <int P>
+__device__ __forceinline__ void stage_x_quads(const DevProblem &pb, float4 *xs)
+{
   const int R = (pb.NQ + 3) >> 2;
   for (int i = threadIdx.x; i < 4 * R * P; i += blockDim.x) {
     float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -717,7 +851,31 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
     }
     xs[i] = v;
   }
-  __syncthreads();
+}
+
+// TG (the persistent sweep, relative sweep pk): X is already in LDS; b of
+// relative sweep pk - 1 comes from the tagged words (pk > 0; the launch's
+// first sweep reads pb.beta, written before the launch); m_j goes to the
+// tagged words and, in the launch's last sweep (fin), to pb.M; a piece's
+// partial sums go to the tagged words only. (Plain stores of one address
+// from blocks of different XCDs in one launch reach memory in no fixed
+// order, since each XCD's L2 writes back its own lines: so the plain arrays
+// take one store per launch, the last sweep's.)
+template <int P, bool WMU, bool TG = false>
+__device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_t sweep, int chain, int vb, int ve,
+                                                   int t0, int tstep, int slot, int ra = 0, int rb = -1,
+                                                   float *part = nullptr, const PTags *tg = nullptr, int pk = 0,
+                                                   unsigned long long *tpart = nullptr, bool fin = true)
+{
+  static_assert(P <= 8, "means take at most two MFMAs");
+  static_assert(!(TG && WMU), "the persistent sweep keeps b fixed");
+  constexpr bool MFM = P >= 3;
+  extern __shared__ float4 xs[];  // [4R][P]
+  const int R = (pb.NQ + 3) >> 2;
+  if constexpr (!TG) {
+    stage_x_quads<P>(pb, xs);
+    __syncthreads();
+  }
   const float *xsf = reinterpret_cast<const float *>(xs);
   const int lane = threadIdx.x & 63, vl = lane & 15, g = lane >> 4;
   const float b = pb.bwm[chain];
@@ -741,8 +899,12 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
     const float bw = b * wmv;
     const float *bp = pb.beta + ((size_t)chain * pb.V + sv) * P;
     float bs[P];
+    if (TG && pk > 0) {
+      tg_waitf<P>(tg->bt + ((size_t)((pk - 1) & 1) * pb.V + sv) * P, ptag(*tg, pk - 1), pb, bs);
+    } else {
 #pragma unroll
-    for (int c = 0; c < P; c++) bs[c] = bp[c];
+      for (int c = 0; c < P; c++) bs[c] = bp[c];
+    }
     float bm0 = 0.0f, bm1 = 0.0f;  // B of the means: beta_g and beta_(g+4) of this voxel
 #pragma unroll
     for (int c = 0; c < P; c++) {
@@ -810,15 +972,27 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
       for (int c = 0; c < P; c++) acc[c] += __shfl_xor(acc[c], off, 64);
       if (WMU) zs += __shfl_xor(zs, off, 64);
     }
-    if (part) {  // a piece of a split tile: partial sums, tile-local voxel order
+    if (TG && tpart) {  // a piece of a split tile, persistent sweep: tagged partial sums
+      if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < P; c++)
+          tg_st(tpart + ((threadIdx.x >> 6) * 16 + vl) * P + c, ptag(*tg, pk), __float_as_uint(acc[c]));
+      }
+    } else if (part) {  // a piece of a split tile: partial sums, tile-local voxel order
       if (g == 0) {
 #pragma unroll
         for (int c = 0; c < P; c++) part[((threadIdx.x >> 6) * 16 + vl) * P + c] = acc[c];
       }
     } else if (valid && g == 0) {
       float *m = pb.M + ((size_t)chain * pb.V + s) * P;
+      if (!TG || fin) {
 #pragma unroll
-      for (int c = 0; c < P; c++) m[c] = acc[c];
+        for (int c = 0; c < P; c++) m[c] = acc[c];
+      }
+      if constexpr (TG) {
+#pragma unroll
+        for (int c = 0; c < P; c++) tg_st(tg->mt + (size_t)s * P + c, ptag(*tg, pk), __float_as_uint(acc[c]));
+      }
       if (WMU) {
         double r = (double)zs;
 #pragma unroll
@@ -896,6 +1070,46 @@ __device__ __forceinline__ bool vl_tail_sum(const DevProblem &pb, int chain, int
   for (int kk = 0; kk < r.K; kk++) {
 #pragma unroll
     for (int c = 0; c < P; c++) m[c] += tp[(size_t)kk * 64 * P + c];
+  }
+  return true;
+}
+
+// The persistent sweep's latent item b of colour c (the numbering of
+// vl_ctile_block over that colour alone: whole tiles, then pieces), relative
+// sweep pk, Philox sweep word `sweep`
+template <int P>
+__device__ __forceinline__ void vl_ctile_block_tg(const DevProblem &pb, const PTags &tg, int pk, uint32_t sweep, int b,
+                                                  int c)
+{
+  const VlRange r = vl_range(pb, c);
+  if (b < r.nmain) {
+    latent_vl_mfm_body<P, false, true>(pb, sweep, 0, r.base, r.end, b, 1 << 30, 0, 0, -1, nullptr, &tg, pk, nullptr,
+                                       pk == tg.K - 1);
+    return;
+  }
+  b -= r.nmain;
+  const int tl = b / r.K, k = b - tl * r.K;
+  const int R = (pb.NQ + 3) >> 2;
+  latent_vl_mfm_body<P, false, true>(pb, sweep, 0, r.base, r.end, r.nmain + tl, 1 << 30, 0, k * R / r.K,
+                                     (k + 1) * R / r.K, nullptr, &tg, pk,
+                                     tg.mpt + r.moff + ((size_t)tl * r.K + k) * 64 * P);
+}
+
+// vl_tail_sum from the tagged partials of relative sweep pk (same order)
+template <int P>
+__device__ __forceinline__ bool vl_tail_sum_tg(const DevProblem &pb, const PTags &tg, int pk, int s, float (&m)[P])
+{
+  const VlRange r = vl_range(pb, s >= pb.n0 ? 1 : 0);
+  const int loc = s - r.base, tl = (loc >> 6) - r.nmain;
+  if (tl < 0) return false;
+  const unsigned long long *tp = tg.mpt + r.moff + (size_t)tl * r.K * 64 * P + (size_t)(loc & 63) * P;
+#pragma unroll
+  for (int c = 0; c < P; c++) m[c] = 0.0f;
+  for (int kk = 0; kk < r.K; kk++) {
+    float v[P];
+    tg_waitf<P>(tp + (size_t)kk * 64 * P, ptag(tg, pk), pb, v);
+#pragma unroll
+    for (int c = 0; c < P; c++) m[c] += v[c];
   }
   return true;
 }
@@ -1259,24 +1473,28 @@ __device__ void wishart_algebra_wave(const DevProblem &pb, int chain, int P, con
       __hip_atomic_store(pb.lamtag + (size_t)chain * P * P + e,
                          ((unsigned long long)(pb.lam_tag ? pb.lam_tag : 1) << 32) |
                              (unsigned long long)__float_as_uint((float)v),
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                         __ATOMIC_RELAXED, TAG_SCOPE);
   }
 }
 
 // S = (T + I)^-1, its Cholesky factor L, B = L A and Lambda = B B' in FP64
 // (cholesky.cpp:245-286 steps), on one lane with the matrices in registers
 // (P <= 6); T packed lower triangle.
+// plain = false: Lambda goes to the tagged words only (no pb.lam, lam64, tsum)
 template <int P>
 __device__ __forceinline__ void wishart_algebra(const DevProblem &pb, int chain, const double *T,
                                                 const double (&A)[P][P], bool tagged, double (&S)[P][P],
-                                                double (&inv)[P][P], double (&B)[P][P], double (&y)[P])
+                                                double (&inv)[P][P], double (&B)[P][P], double (&y)[P],
+                                                bool plain = true)
 {
   for (int a = 0, e = 0; a < P; a++)
     for (int c = 0; c <= a; c++, e++) {
       S[a][c] = T[e];
       S[c][a] = T[e];
-      pb.tsum[(size_t)chain * P * P + a * P + c] = T[e];
-      pb.tsum[(size_t)chain * P * P + c * P + a] = T[e];
+      if (plain) {
+        pb.tsum[(size_t)chain * P * P + a * P + c] = T[e];
+        pb.tsum[(size_t)chain * P * P + c * P + a] = T[e];
+      }
     }
   for (int a = 0; a < P; a++) S[a][a] += 1.0;
   chol_lower<P>(S);
@@ -1307,13 +1525,15 @@ __device__ __forceinline__ void wishart_algebra(const DevProblem &pb, int chain,
       const int K = i < j ? i : j;
       double v = 0.0;
       for (int k = 0; k <= K; k++) v += B[i][k] * B[j][k];
-      lam64[i * P + j] = v;
-      lam[i * P + j] = (float)v;
+      if (plain) {
+        lam64[i * P + j] = v;
+        lam[i * P + j] = (float)v;
+      }
       if (tagged)
         __hip_atomic_store(pb.lamtag + (size_t)chain * P * P + i * P + j,
                            ((unsigned long long)(pb.lam_tag ? pb.lam_tag : 1) << 32) |
                                (unsigned long long)__float_as_uint((float)v),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                           __ATOMIC_RELAXED, TAG_SCOPE);
     }
 }
 
@@ -1635,16 +1855,28 @@ __device__ __forceinline__ void coef_fp64_redo(bool ok32, bool active, const flo
 // it) and the block goes on.
 constexpr int LAM_SPIN_MAX = 1 << 20;
 template <int P>
-__device__ __forceinline__ const float *lam_wait(const DevProblem &pb, const unsigned long long *lamw)
+__device__ __forceinline__ const float *lam_wait(const DevProblem &pb, const unsigned long long *lamw,
+                                                 uint32_t want = 1u)
 {
   __shared__ float lam_s[P * P];
   if (threadIdx.x < P * P) {
     const int spin = pb.lam_spin > 0 ? pb.lam_spin : LAM_SPIN_MAX;
     unsigned long long v = 0ull;
     for (int it = 0;; it++) {
-      v = __hip_atomic_load(lamw + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if ((v >> 32) == 1ull) break;
-      if (it >= spin) {
+      v = __hip_atomic_load(lamw + threadIdx.x, __ATOMIC_RELAXED, TAG_SCOPE);
+      if ((uint32_t)(v >> 32) == want) break;
+      if (it >= spin && pb.pdiag && want != 1u &&
+          atomicExch(pb.pdiag + 4 + gridDim.x + blockIdx.x, pb.pdiag[4 + blockIdx.x]) != pb.pdiag[4 + blockIdx.x]) {
+        const uint32_t slot = atomicAdd(&pb.flags[3], 1u);
+        if (slot < PT_DIAG_RECS) {
+          unsigned long long *r = pb.pdiag + 4 + 2 * gridDim.x + 4 * slot;
+          r[0] = (unsigned long long)(lamw + threadIdx.x);
+          r[1] = want;
+          r[2] = v;
+          r[3] = pb.pdiag[4 + blockIdx.x];
+        }
+      }
+      if (it >= spin || (pb.lam_spin == 0 && (it & 31) == 31 && want != 1u && tg_abort(pb))) {
         atomicOr(&pb.flags[2], 1u);
         break;
       }
@@ -1657,13 +1889,25 @@ __device__ __forceinline__ const float *lam_wait(const DevProblem &pb, const uns
 }
 
 // lamw != null: Lambda is read through lam_wait, after this block's
-// neighbour and X'Z loads are under way
-template <int P, bool PREC, bool KEEPNB = PREC>
+// neighbour and X'Z loads are under way.
+// TG (the persistent sweep, one chain, relative sweep pk): what other blocks
+// of the launch wrote comes from the tagged words: the other colour's b
+// (colour 0 reads relative sweep pk - 1, plain pb.beta at pk = 0; colour 1
+// reads pk), m_j and the pieces, Lambda of pk - 1 (plain pb.lam at pk = 0)
+// and, for a voxel that keeps its coefficients, its own b of pk - 1; the new
+// b goes to the tagged words and, in the launch's last sweep, to pb.beta (and
+// m_j of a leftover tile to pb.M): one plain store per address and launch;
+// the pair partials go to the tagged words only.
+template <int P, bool PREC, bool KEEPNB = PREC, bool TG = false>
 __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint32_t sweep, int chain, int blk,
                                           int tail, const Accum &ac, int batch,
-                                          const unsigned long long *lamw = nullptr)
+                                          const unsigned long long *lamw = nullptr, const PTags *tg = nullptr,
+                                          int pk = 0)
 {
+  const bool fin = !TG || pk == tg->K - 1;  // TG: plain outputs in the launch's last sweep only
+  static_assert(!TG || (P <= 6 && !KEEPNB), "the persistent sweep runs P <= 6 and reads the neighbours again");
   constexpr int NE = P * (P + 1) / 2;
+  const int nbk = colour ? pk : pk - 1;  // TG: relative sweep of the other colour's b
   const int q = blk * 256 + threadIdx.x;
   const int nc = colour ? pb.V - pb.n0 : pb.n0;
   const bool active = q < nc;
@@ -1690,11 +1934,17 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
     if (nb[d] >= 0) {
       nn++;
       const float *src = betc + (size_t)nb[d] * P;
+      float v[P];
+      if (TG && nbk >= 0) {
+        tg_waitf<P>(tg->bt + ((size_t)(nbk & 1) * pb.V + nb[d]) * P, ptag(*tg, nbk), pb, v);
+      } else {
+#pragma unroll
+        for (int c = 0; c < P; c++) v[c] = src[c];
+      }
 #pragma unroll
       for (int c = 0; c < P; c++) {
-        const float v = src[c];
-        sv[c] += v;
-        if (KB) bk[KB ? d : 0][c] = v;
+        sv[c] += v[c];
+        if (KB) bk[KB ? d : 0][c] = v[c];
       }
     } else if (KB) {
 #pragma unroll
@@ -1706,18 +1956,27 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
   float *mz = pb.M + ((size_t)chain * pb.V + s) * P;
   const float bw = pb.bwm[chain] * pb.wm[s];
   float m[P];
-  if (((tail >> colour) & 1) && vl_tail_sum<P>(pb, chain, s, m)) {
-    if (active) {
+  bool intail;
+  if constexpr (TG) intail = ((tail >> colour) & 1) && vl_tail_sum_tg<P>(pb, *tg, pk, s, m);
+  else intail = ((tail >> colour) & 1) && vl_tail_sum<P>(pb, chain, s, m);
+  if (intail) {
+    if (active && fin) {
 #pragma unroll
       for (int i = 0; i < P; i++) mz[i] = m[i];
     }
+  } else if constexpr (TG) {
+    tg_waitf<P>(tg->mt + (size_t)s * P, ptag(*tg, pk), pb, m);
   } else {
 #pragma unroll
     for (int i = 0; i < P; i++) m[i] = mz[i];
   }
 #pragma unroll
   for (int i = 0; i < P; i++) m[i] = fmaf(-bw, pb.xsum[i], m[i]);
-  if (lamw) lam = lam_wait<P>(pb, lamw);
+  if constexpr (TG) {
+    if (pk > 0) lam = lam_wait<P>(pb, tg->lt + (size_t)((pk - 1) & 1) * P * P, ptag(*tg, pk - 1));
+  } else if (lamw) {
+    lam = lam_wait<P>(pb, lamw);
+  }
   float Q[P][P], r[P];
 #pragma unroll
   for (int i = 0; i < P; i++) {
@@ -1767,16 +2026,27 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
       } else {
         // not positive definite: keep the previous coefficients
         if (active) atomicAdd(&pb.flags[1], 1u);
+        if (TG && pk > 0) {
+          tg_waitf<P>(tg->bt + ((size_t)((pk - 1) & 1) * pb.V + s) * P, ptag(*tg, pk - 1), pb, out);
+        } else {
 #pragma unroll
-        for (int c = 0; c < P; c++) out[c] = dst[c];
+          for (int c = 0; c < P; c++) out[c] = dst[c];
+        }
       }
     }
   } else {
     if (__ballot(!ok32)) coef_fp64_redo<P>(ok32, active, m, sv, z, (float)nn, lam, pb.xtx, dst, pb.flags, out);
   }
   if (active) {
+    if (fin) {
 #pragma unroll
-    for (int c = 0; c < P; c++) dst[c] = out[c];
+      for (int c = 0; c < P; c++) dst[c] = out[c];
+    }
+    if constexpr (TG) {
+#pragma unroll
+      for (int c = 0; c < P; c++)
+        tg_st(tg->bt + ((size_t)(pk & 1) * pb.V + s) * P + c, ptag(*tg, pk), __float_as_uint(out[c]));
+    }
     if (ac.sum) accum_voxel<P>(pb, ac, batch, chain, s, out);  // a save sweep: this voxel's final b
   }
   if (!PREC) return;
@@ -1793,11 +2063,19 @@ __device__ __forceinline__ void coef_body(const DevProblem &pb, int colour, uint
         } else {
           float bn[P];
           const float *src = betc + (size_t)nb[d] * P;
+          if (TG && nbk >= 0) {
+            tg_waitf<P>(tg->bt + ((size_t)(nbk & 1) * pb.V + nb[d]) * P, ptag(*tg, nbk), pb, bn);
+          } else {
 #pragma unroll
-          for (int c = 0; c < P; c++) bn[c] = src[c];
+            for (int c = 0; c < P; c++) bn[c] = src[c];
+          }
           add_pair<P>(out, bn, (d & 1) != 0, acc);
         }
       }
+  }
+  if constexpr (TG) {
+    block_sum_store_tg<NE>(acc, tg->pt + (size_t)blk * 2 * NE, ptag(*tg, pk));
+    return;
   }
   double *parts = pb.partials + (size_t)chain * pb.max_parts * NE;
   block_sum_store<NE>(acc, parts + (size_t)blk * NE);
@@ -1900,6 +2178,187 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, 
     pb.lamtag[(size_t)chain * P * P + threadIdx.x] = 0ull;
     pb.wbart[(size_t)chain * P * P + threadIdx.x] = bartlett_entry<P>(pb, chain, threadIdx.x, sweep + base);
   }
+}
+
+// ---------------------------------------------------------------------------
+// k_persist: K full sweeps of one chain in one launch (P <= 6, the overlapped
+// form's conditions). A fixed grid of resident blocks takes items from a
+// queue (tg.head) until it runs out. Per relative sweep k the items are
+//   L0(k), L1(k): a latent tile or piece of colour 0 / 1 (needs the b of its
+//                 voxels from the update of k - 1);
+//   U0(k), U1(k): the update of 256 voxels of colour 0 / 1 (needs its voxels'
+//                 m_j, the other colour's b around them, Lambda of k - 1);
+//                 U1 writes the pair partials, U0 a done word;
+//   BT(k):        the Bartlett factor of k's Lambda (needs the draw of k - 2
+//                 to have read that buffer);
+//   W(k):         Lambda of k from every U1(k)'s partials, once every U0(k)
+//                 is done;
+// each writing tagged words that the items needing them wait for. The host
+// lays the items out as a pattern of one window, repeated; entry = dk << 28 |
+// type << 24 | index, and window w holds the items of relative sweep w - dk
+// (items outside 0 .. K-1 are skipped). The host checks that every item
+// comes after each item it waits for, so the earliest unfinished item always
+// has its inputs and the queue drains whatever the residency. The same
+// variates and sums as the overlapped and the serial sweeps.
+// ---------------------------------------------------------------------------
+template <int P>
+__device__ void persist_bartlett(const DevProblem &pb, const PTags &tg, int k, uint32_t sweep)
+{
+  if (threadIdx.x >= P * P) return;
+  // buffer k & 1 was last read by W(k - 2): its Lambda shows that
+  if (k >= 2) {
+    uint32_t w[1];
+    tg_wait<1>(tg.lt + (size_t)(k & 1) * P * P, ptag(tg, k - 2), pb, w);
+  }
+  tg_std(tg.wb + ((size_t)(k & 1) * P * P + threadIdx.x) * 2, ptag(tg, k),
+         bartlett_entry<P>(pb, 0, threadIdx.x, sweep));
+}
+
+template <int P>
+__device__ void persist_wishart(const DevProblem &pb, const PTags &tg, int k)
+{
+  constexpr int NE = P * (P + 1) / 2;
+  const uint32_t tag = ptag(tg, k);
+  const int ncb0 = (pb.n0 + 255) / 256, ncb1 = (pb.V - pb.n0 + 255) / 256;
+  // every colour-0 update of k has read Lambda of k - 1 (their done words)
+  for (int b = threadIdx.x; b < ncb0; b += 256) {
+    uint32_t w[1];
+    tg_wait<1>(tg.u0d + b, tag, pb, w);
+  }
+  // the colour-1 partials, summed in sum_partials' order
+  __shared__ double T[NE];
+  __shared__ double A[P][P];
+  double acc[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) acc[e] = 0.0;
+  for (int b = threadIdx.x; b < ncb1; b += 256) {
+    const unsigned long long *src = tg.pt + (size_t)b * 2 * NE;
+    constexpr int CH = 4;  // entries per round of loads
+#pragma unroll
+    for (int e0 = 0; e0 < NE; e0 += CH) {
+      constexpr int W2 = 2 * CH;
+      uint32_t w[W2];
+      if (e0 + CH <= NE) {
+        tg_wait<W2>(src + 2 * e0, tag, pb, w);
+      } else {
+#pragma unroll
+        for (int i = 0; i < W2; i += 2) {
+          if (e0 + i / 2 < NE) {
+            uint32_t v[2];
+            tg_wait<2>(src + 2 * e0 + i, tag, pb, v);
+            w[i] = v[0];
+            w[i + 1] = v[1];
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < CH; i++)
+        if (e0 + i < NE) acc[e0 + i] += tg_hilo(w[2 * i], w[2 * i + 1]);
+    }
+  }
+  block_sum_store<NE>(acc, T, NE);
+  __syncthreads();
+  const int lane = threadIdx.x;
+  if (lane >= 64) return;
+  for (int idx = lane; idx < P * P; idx += 64) {
+    uint32_t w[2];
+    tg_wait<2>(tg.wb + ((size_t)(k & 1) * P * P + idx) * 2, tag, pb, w);
+    A[idx / P][idx % P] = tg_hilo(w[0], w[1]);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  if (lane != 0) return;
+  DevProblem q = pb;  // Lambda of k to buffer k & 1 with k's tag, and to pb.lam
+  q.lamtag = tg.lt + (size_t)(k & 1) * P * P;
+  // pb.lam_tag != 0 (set from the host only to test the bound): a tag that
+  // no reader accepts
+  q.lam_tag = (int)(pb.lam_tag ? (tag ^ 0x4000u) : tag);
+  double S[P][P], inv[P][P], B[P][P], y[P];
+  wishart_algebra<P>(q, 0, T, A, true, S, inv, B, y, k == tg.K - 1);
+}
+
+// the persistent sweep's one kernel argument
+struct PArgs {
+  DevProblem pb;
+  PTags tg;
+};
+
+template <int P>
+__global__ __launch_bounds__(256, 8) void k_persist(PArgs args)
+{
+  static_assert(P <= 6, "the persistent sweep runs P <= 6");
+  // The argument is read in place, through the kernel-argument segment (its
+  // only argument, at offset 0), by a pointer that the compiler must take as
+  // new in every item: each item loads the fields it uses, and none of them
+  // stays in scalar registers through another item's loops (they would crowd
+  // the latent loop's registers). `args` itself is never named below, so no
+  // copy of it is made.
+  typedef const __attribute__((address_space(4))) PArgs KArgs;
+  KArgs *ka = (KArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  const DevProblem &pb = *(const DevProblem *)&ka->pb;
+  const PTags &tg = *(const PTags *)&ka->tg;
+  const uint32_t base = *pb.sweep_base + tg.sweep0;
+  extern __shared__ float4 xs[];
+  stage_x_quads<P>(pb, xs);  // once for every latent item of the launch
+  // the next queue entry is taken when an item ends, not ahead: an entry taken
+  // early would wait behind this block's item while others need its outputs
+  __shared__ uint32_t s_q;
+  if (threadIdx.x == 0) s_q = atomicAdd(tg.head, 1u);
+  __syncthreads();
+  // wave-uniform (one value for the block): keeps the item's indexing scalar
+  uint32_t g = __builtin_amdgcn_readfirstlane(s_q);
+  const uint32_t total = (uint32_t)tg.windows * (uint32_t)tg.plen;
+  while (g < total) {
+    KArgs *kq = ka;
+    asm volatile("" : "+s"(kq));
+    const DevProblem &ip = *(const DevProblem *)&kq->pb;
+    const PTags &it = *(const PTags *)&kq->tg;
+    const uint32_t w = g / (uint32_t)it.plen;
+    const uint32_t e = it.pattern[g - w * (uint32_t)it.plen];
+    const int k = (int)w - (int)(e >> 28);
+    const int idx = (int)(e & 0xFFFFFFu);
+    if (k >= 0 && k < it.K) {
+      const uint32_t sw = base + (uint32_t)k;
+      if (threadIdx.x == 0) ip.pdiag[4 + blockIdx.x] = ((unsigned long long)(uint32_t)k << 32) | e;
+      switch ((e >> 24) & 15u) {
+      case PI_L0:
+        vl_ctile_block_tg<P>(ip, it, k, sw, idx, 0);
+        break;
+      case PI_L1:
+        vl_ctile_block_tg<P>(ip, it, k, sw, idx, 1);
+        break;
+      case PI_U0:
+        coef_body<P, false, false, true>(ip, 0, sw, 0, idx, it.tail, Accum{}, 0, nullptr, &it, k);
+        __syncthreads();
+        if (threadIdx.x == 0) tg_st(it.u0d + idx, ptag(it, k), 1u);
+        break;
+      case PI_U1:
+        coef_body<P, true, false, true>(ip, 1, sw, 0, idx, it.tail, Accum{}, 0, nullptr, &it, k);
+        break;
+      case PI_W:
+        persist_wishart<P>(ip, it, k);
+        break;
+      default:
+        persist_bartlett<P>(ip, it, k, sw);
+        break;
+      }
+    }
+    __syncthreads();  // the item's LDS is free again
+    if (threadIdx.x == 0) s_q = atomicAdd(it.head, 1u);
+    __syncthreads();
+    g = __builtin_amdgcn_readfirstlane(s_q);
+  }
+}
+
+__global__ void k_persist_init(uint32_t *head)
+{
+  if (threadIdx.x == 0) *head = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_zero64(unsigned long long *p, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -3003,6 +3462,59 @@ hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int l
   } while (0)
   BSG_DISPATCH_P(pb.P, L_)
 #undef L_
+  return hipGetLastError();
+}
+
+bool persist_supported(const DevProblem &pb)
+{
+  return overlap_supported(pb) && pb.P <= 6 && pb.nchains == 1 && vl_ctiles(pb);
+}
+
+int persist_max_blocks(const DevProblem &pb)
+{
+  int per_cu = 0;
+  const size_t lds = latent_vl_lds(pb);
+#define L_(PP) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist<PP>, 256, lds)
+  switch (pb.P) {
+  case 1: L_(1); break;
+  case 2: L_(2); break;
+  case 3: L_(3); break;
+  case 4: L_(4); break;
+  case 5: L_(5); break;
+  case 6: L_(6); break;
+  default: return 0;
+  }
+#undef L_
+  return per_cu * pb.ncu;
+}
+
+hipError_t launch_persist(const DevProblem &pb, const PTags &tg, int nblocks, hipStream_t st)
+{
+  if (!persist_supported(pb) || nblocks < 1 || tg.K < 1 || tg.K > PERSIST_MAX_K || tg.plen < 1 || tg.windows < tg.K ||
+      tg.epoch < 1 || tg.epoch > (uint32_t)PERSIST_MAX_EPOCH)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_persist_init, dim3(1), dim3(64), 0, st, tg.head);
+  const size_t lds = latent_vl_lds(pb);
+  const PArgs args{pb, tg};
+#define L_(PP) hipLaunchKernelGGL(k_persist<PP>, dim3((unsigned)nblocks), dim3(256), lds, st, args)
+  switch (pb.P) {
+  case 1: L_(1); break;
+  case 2: L_(2); break;
+  case 3: L_(3); break;
+  case 4: L_(4); break;
+  case 5: L_(5); break;
+  case 6: L_(6); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_zero64(unsigned long long *p, size_t n, hipStream_t st)
+{
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero64, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
   return hipGetLastError();
 }
 

@@ -235,6 +235,20 @@ int bsg_get_lane_split(bsg_ctx *ctx, int *g); /* the split the next sweep uses *
  * bsg_get_overlap tells whether the next bsg_sweep* call runs overlapped. */
 int bsg_set_overlap(bsg_ctx *ctx, int on);
 int bsg_get_overlap(bsg_ctx *ctx, int *on);
+/* Persistent sweeps (no counterpart in the reference, mcmc.cpp:311-394 runs
+ * a sweep's kernels one after another): bsg_sweep's sweeps of an overlapped
+ * one-chain context with P <= 6 in one launch. A fixed grid of resident
+ * blocks takes the sweeps' latent tiles, colour updates and CAR-precision
+ * draws from a queue in an order that puts every item after what it reads,
+ * so no launch boundary drains the chip between phases. Same variates, same
+ * sums, same chains as the overlapped and the serial forms. Off by default;
+ * bsg_sweep_accumulate and captured graphs keep the overlapped form.
+ * bsg_get_persistent tells whether the next bsg_sweep runs persistent. The
+ * lags (fractions of the resident block count, in (0, 64]) place each update
+ * after its inputs in the queue; they change the speed, never the result. */
+int bsg_set_persistent(bsg_ctx *ctx, int on);
+int bsg_get_persistent(bsg_ctx *ctx, int *on);
+int bsg_set_persistent_lags(bsg_ctx *ctx, double latent, double update, double draw);
 /* Test hook of the overlapped sweep's bounded wait (no counterpart in the
  * reference): the colour-0 blocks give up after spin_limit polls (0: the
  * default 2^20) and the CAR-precision draw writes `tag` beside its entries

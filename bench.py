@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--cpu-c1-sweeps", type=int, default=2000, help="reference CPU sweeps at C1 (the full run)")
     ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="time the headline sweeps as replays of a HIP graph prepared before the warm-up "
+                         "(at 20 sweeps no faster than direct launches on this machine)")
     ap.add_argument("--config-lines", type=int, default=1, help="also time the C1 and C3 configurations")
     ap.add_argument("--c1-sweeps", type=int, default=2000, help="timed sweeps of the C1 line (its full run)")
     ap.add_argument("--c3-sweeps", type=int, default=200, help="timed sweeps of each C3 form")
@@ -184,11 +187,15 @@ def finish_cpu_legs(procs, timeout=900):
     return res
 
 
-def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, profile_sweeps=50):
-    """Timed region: `steps` direct sweeps (bsg_sweep) after `warmup` untimed
-    ones, bracketed by stream synchronisations. `settle` (optional) runs just
-    before the warm-up, after the context is built, so that the GPU is busy
-    and at its working clock when the few warm-up sweeps start (bench.py's
+def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, profile_sweeps=50, graph=False):
+    """Timed region: `steps` sweeps after `warmup` untimed direct ones,
+    bracketed by stream synchronisations: direct launches (bsg_sweep), or
+    with graph=True replays of a HIP graph of min(steps, 256) sweeps
+    (bsg_sweep_graph; captured and uploaded before any GPU work here, so the
+    timed call only replays it), and then `steps` direct sweeps timed the
+    same way for comparison. `settle` (optional) runs just before the
+    warm-up, after the context is built, so that the GPU is busy and at its
+    working clock when the few warm-up sweeps start (bench.py's
     copy-bandwidth measurement). Then, in the same run, `steps` more sweeps
     with HIP events around every latent launch on the stream it runs on
     (kernel duration and sweep span; the event packets cost a few
@@ -200,14 +207,26 @@ def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, prof
     S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15, xtz_mode=mode, z_store=zstore)
     S.overlap = overlap
     ran_overlapped = S.overlap
+    per_graph = min(steps, 256)
+    if graph:
+        S.prepare_graph(per_graph)
     S.synchronize()
     extra = settle() if settle else None
     S.sweep(warmup)
     S.synchronize()
     t0 = time.perf_counter()
-    S.sweep(steps)
+    if graph:
+        S.sweep_graph(steps, per_graph)
+    else:
+        S.sweep(steps)
     S.synchronize()
     wall = time.perf_counter() - t0
+    direct_wall = None
+    if graph:
+        t0 = time.perf_counter()
+        S.sweep(steps)
+        S.synchronize()
+        direct_wall = time.perf_counter() - t0
     n_ev = max(steps, 200)
     lat_ms, span_ms = S.sweep_timed(n_ev)
     S.synchronize()
@@ -215,7 +234,8 @@ def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, prof
     kms = S.sweep_profiled(n_prof) if n_prof > 0 else {}
     S.close()
     return {"wall": wall, "lat_ms": lat_ms / n_ev, "span_ms": span_ms / n_ev,
-            "kms": {k: v / n_prof for k, v in kms.items()}, "overlapped": ran_overlapped, "settle": extra}
+            "kms": {k: v / n_prof for k, v in kms.items()}, "overlapped": ran_overlapped, "settle": extra,
+            "graph_group": per_graph if graph else 0, "direct_wall": direct_wall}
 
 
 def config_line(name, forms, steps, warmup):
@@ -465,7 +485,8 @@ def main():
 
     # fused X'Z, Z not kept, overlapped; the copy-bandwidth measurement runs
     # between building the context and the warm-up sweeps
-    head = time_mode(L, d, 2, 2, a.steps, a.warmup, settle=lambda: copy_bandwidth(1 << 30, 100))
+    head = time_mode(L, d, 2, 2, a.steps, a.warmup, settle=lambda: copy_bandwidth(1 << 30, 100),
+                     graph=a.graph)
     if not head["overlapped"]:
         raise RuntimeError("the headline workload must run overlapped")
     copy_gbs, copy_ms = head["settle"]
@@ -513,6 +534,10 @@ def main():
                                f"Z not kept, overlapped sweep (two launches per sweep)",
                    "V": V, "N": N, "P": P, "chains": 1},
         "latent_draws_per_s": value * N * V,
+        "timed_path": (f"bsg_sweep_graph: the {a.steps} timed sweeps replayed from a HIP graph of "
+                       f"{head['graph_group']} sweeps, captured before the warm-up (no capture in the timed region)"
+                       if head["graph_group"] else "bsg_sweep: direct launches"),
+        "direct_launch_sweeps_per_s": a.steps / head["direct_wall"] if head["direct_wall"] else value,
         "sweep_kernel_ms": head["lat_ms"],
         "k_overlap_avg_launch_ms": head["lat_ms"] / 2,
         "kernel_ms_serial_breakdown": head["kms"],

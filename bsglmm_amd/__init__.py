@@ -62,6 +62,7 @@ def lib():
         "bsg_destroy": (C.c_int, [vp]),
         "bsg_sweep": (C.c_int, [vp, C.c_int]),
         "bsg_sweep_graph": (C.c_int, [vp, C.c_int, C.c_int]),
+        "bsg_sweep_graph_prepare": (C.c_int, [vp, C.c_int]),
         "bsg_update_latent": (C.c_int, [vp]),
         "bsg_compute_xtz": (C.c_int, [vp]),
         "bsg_update_wm_coef": (C.c_int, [vp]),
@@ -211,6 +212,11 @@ class Sampler:
     def sweep_graph(self, n, per_graph=8):
         """n sweeps replayed from a HIP graph of per_graph sweeps (same results as sweep)."""
         _check(lib().bsg_sweep_graph(self._h, int(n), int(per_graph)))
+
+    def prepare_graph(self, per_graph):
+        """Capture and instantiate the graph of per_graph sweeps that
+        sweep_graph uses, without running a sweep."""
+        _check(lib().bsg_sweep_graph_prepare(self._h, int(per_graph)))
 
     def sweep_profiled(self, n):
         ms = np.zeros(6, dtype=np.float64)

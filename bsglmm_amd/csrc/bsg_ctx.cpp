@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 
 #include "../../include/bsglmm.h"
@@ -1159,6 +1160,20 @@ int bsg_sweep_graph(bsg_ctx *ctx, int nsweeps, int sweeps_per_graph)
   return sweep_run(ctx, nsweeps - q * k, nullptr);
 }
 
+int bsg_sweep_graph_prepare(bsg_ctx *ctx, int sweeps_per_graph)
+{
+  CTXCHK(ctx);
+  if (sweeps_per_graph < 1 || sweeps_per_graph > 256) {
+    set_error("bsg_sweep_graph_prepare: need 1 <= sweeps_per_graph <= 256");
+    return BSG_ERR_ARG;
+  }
+  const int rc = build_graph(ctx, sweeps_per_graph);
+  if (rc != BSG_OK) return rc;
+  HIPCHK(hipGraphUpload(ctx->graph_exec, ctx->stream));  // its device-side setup, ahead of the first replay
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BSG_OK;
+}
+
 int bsg_sweep(bsg_ctx *ctx, int nsweeps)
 {
   CTXCHK(ctx);
@@ -1826,6 +1841,13 @@ int bsg_get_fallback_count(bsg_ctx *ctx, uint32_t *count)
   return BSG_OK;
 }
 
+// The probe's buffers stay allocated for the process (per device, grown on
+// demand): freeing them leaves the GPU idle for milliseconds, and work timed
+// right after a probe would start below the GPU's working clock.
+static std::mutex g_probe_mu;
+static void *g_probe_buf[2][bsg::MAX_DEVICES];
+static size_t g_probe_bytes[bsg::MAX_DEVICES];
+
 int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double *ms_per_launch)
 {
   if (!gbps || bytes < 16 || reps < 1 || kind < 0 || kind > 2) {
@@ -1833,13 +1855,28 @@ int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double
     return BSG_ERR_ARG;
   }
   bytes &= ~(size_t)15;
-  void *src = nullptr, *dst = nullptr;
+  std::lock_guard<std::mutex> lock(g_probe_mu);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= bsg::MAX_DEVICES) {
+    set_error("bsg_measure_bandwidth: no device");
+    return BSG_ERR_HIP;
+  }
+  if (g_probe_bytes[dev] < bytes) {
+    for (int b = 0; b < 2; b++) {
+      if (g_probe_buf[b][dev]) hipFree(g_probe_buf[b][dev]);
+      g_probe_buf[b][dev] = nullptr;
+    }
+    g_probe_bytes[dev] = 0;
+  }
+  void *src = g_probe_buf[0][dev], *dst = g_probe_buf[1][dev];
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = BSG_OK;
   float ms = 0.0f;
-  if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) {
+  if (!src && (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess)) {
     set_error("bsg_measure_bandwidth: hipMalloc failed");
+    if (src) hipFree(src);
+    src = dst = nullptr;
     rc = BSG_ERR_NOMEM;
   } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
@@ -1865,8 +1902,11 @@ int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   if (st) hipStreamDestroy(st);
-  if (src) hipFree(src);
-  if (dst) hipFree(dst);
+  if (src && !g_probe_buf[0][dev]) {
+    g_probe_buf[0][dev] = src;
+    g_probe_buf[1][dev] = dst;
+    g_probe_bytes[dev] = bytes;
+  }
   return rc;
 }
 

@@ -99,6 +99,10 @@ int bsg_sweep(bsg_ctx *ctx, int nsweeps);
  * Same updates and random counters as bsg_sweep, so the same results.
  * Returns after the replays drained. */
 int bsg_sweep_graph(bsg_ctx *ctx, int nsweeps, int sweeps_per_graph);
+/* Captures and instantiates the graph of sweeps_per_graph sweeps that
+ * bsg_sweep_graph would use, without running it (no sweep, the sweep index
+ * unchanged), so that the first bsg_sweep_graph call only replays. */
+int bsg_sweep_graph_prepare(bsg_ctx *ctx, int sweeps_per_graph);
 /* updateZGPU (covarGPU.cu:1625-1669) */
 int bsg_update_latent(bsg_ctx *ctx);
 /* X'Z, the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352),
@@ -289,7 +293,9 @@ int bsg_batch_get_launch_mode(bsg_batch *batch, int *unified);
 
 /* ---- measurement utility (no counterpart in the reference) ------------- */
 /* Times reps launches of a float4 stream-copy kernel over `bytes`; gbps is
- * (read + written bytes) / time. Calibrates bandwidth and counter figures. */
+ * (read + written bytes) / time. Calibrates bandwidth and counter figures.
+ * The two buffers stay allocated (per device, for the process) so that no
+ * free leaves the GPU idle between the probe and work timed after it. */
 int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_per_copy);
 /* kind 0: copy (read + write counted), 1: write-only stream, 2: read-only stream */
 int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double *ms_per_launch);

@@ -151,3 +151,19 @@ def test_overlap_off_where_unsupported():
     assert not A.overlap
     A.lane_split = 4
     assert A.overlap
+
+
+def test_prepared_graph_runs_no_sweep_and_replays_the_chain():
+    """bsg_sweep_graph_prepare captures and uploads the graph without running
+    it: the sweep index and state are unchanged; the replays that follow give
+    the chain of direct serial sweeps."""
+    print("case prepared graph c1 N=300 P=4", flush=True)
+    L, A, B = pair("c1", 300, 4)
+    before = A.get_spat_coef()
+    A.prepare_graph(4)
+    A.synchronize()
+    assert A.sweep_index() == B.sweep_index()
+    assert np.array_equal(A.get_spat_coef(), before)
+    A.sweep_graph(8, per_graph=4)
+    B.sweep(8)
+    assert_same_state(A, B)

@@ -24,7 +24,8 @@ and the twelve uneven analyses (C5).
 Prints one JSON line with the dominant kernel's roofline (its algorithmic
 bytes against 8 TB/s, its counted HBM bytes, and the limiter it actually
 meets: VALU issue, from the VALU count of its rocprofv3 profile over the
-launch time measured here), the effective sweep-level HBM fraction (the
+launch time measured here, against the VALU issue rate a probe kernel
+measures in the same run), the effective sweep-level HBM fraction (the
 unfused sweep's bytes N V (8 + 1/8) + V (24 P + 5), which credits Z traffic
 this form never moves), a measured copy-kernel bandwidth, and the reference
 CPU path: C2 for 3 warm-up + 20 timed sweeps and C1 for its full 2,000
@@ -196,7 +197,7 @@ def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, prof
     same way for comparison. `settle` (optional) runs just before the
     warm-up, after the context is built, so that the GPU is busy and at its
     working clock when the few warm-up sweeps start (bench.py's
-    copy-bandwidth measurement). Then, in the same run, `steps` more sweeps
+    copy-bandwidth and VALU-rate measurements). Then, in the same run, `steps` more sweeps
     with HIP events around every latent launch on the stream it runs on
     (kernel duration and sweep span; the event packets cost a few
     microseconds per sweep, so they stay out of the timed region), and a
@@ -475,7 +476,7 @@ def main():
     # the reference CPU path on two pinned cores of their own, beside the GPU work
     legs = [] if a.no_cpu else [("c2", a.cpu_warmup, a.cpu_sweeps), ("c1", 0, a.cpu_c1_sweeps)]
     procs = start_cpu_legs(legs) if legs else []
-    from bsglmm_amd import Layout, copy_bandwidth, synth
+    from bsglmm_amd import Layout, copy_bandwidth, synth, valu_rate
 
     d = synth.generate(a.config)
     nx, ny, nz = d["grid"]
@@ -483,13 +484,20 @@ def main():
     P, N = d["covar"].shape
     V = L.totvox
 
-    # fused X'Z, Z not kept, overlapped; the copy-bandwidth measurement runs
-    # between building the context and the warm-up sweeps
-    head = time_mode(L, d, 2, 2, a.steps, a.warmup, settle=lambda: copy_bandwidth(1 << 30, 100),
-                     graph=a.graph)
+    # fused X'Z, Z not kept, overlapped; the copy-bandwidth and VALU-rate
+    # measurements run between building the context and the warm-up sweeps,
+    # the FP32 FMA probe (~0.1 s of VALU-bound work) last, so that the sweeps
+    # start at the clock the chip holds under VALU-bound work, as in a long run
+    def settle():
+        cb = copy_bandwidth(1 << 30, 100)
+        tr = valu_rate("trans", 20)
+        fm = valu_rate("fma", 100)
+        return cb, fm, tr
+
+    head = time_mode(L, d, 2, 2, a.steps, a.warmup, settle=settle, graph=a.graph)
     if not head["overlapped"]:
         raise RuntimeError("the headline workload must run overlapped")
-    copy_gbs, copy_ms = head["settle"]
+    (copy_gbs, copy_ms), (fma_rate, _), (trans_rate, _) = head["settle"]
     forms = {"z_free_serial": (2, 2, False), "fused_xtz_z_stored": (2, 0, True), "separate_xtz_fp32": (0, 0, True),
              "separate_xtz_mfma_bf16x2": (1, 0, True)}
     others = {k: time_mode(L, d, m, z, a.steps, a.warmup, o) for k, (m, z, o) in forms.items()}
@@ -510,10 +518,19 @@ def main():
     limiter = None
     if prof.get("valu_wave_instructions_per_launch"):
         w = prof["valu_wave_instructions_per_launch"]
-        # wave64 VALU issue takes 4 cycles on a 16-lane SIMD; 1024 SIMDs at 2.4 GHz
-        limiter = {"kind": "valu_issue", "frac": w * 4 / (1024 * 2.4e9 * launch_s),
+        # peak: 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction; the
+        # VALU probe of this run measures what one instruction kind reaches
+        nominal = 1024 * 2.4e9 / 4
+        limiter = {"kind": "valu_issue", "frac": w / (nominal * launch_s),
                    "wave_instructions_per_launch": w, "valu_per_draw": w * 2 * 64 / (N * V),
-                   "peak": "1024 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction", "source": src}
+                   "peak": "1024 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction",
+                   "peak_winst_per_s": nominal,
+                   "probe_fma_winst_per_s": fma_rate, "frac_of_probe_fma_rate": w / (fma_rate * launch_s),
+                   "probe_transcendental_winst_per_s": trans_rate,
+                   "note": "the probe's all-FMA loop runs below the nominal rate (the chip's clock under it); this "
+                           "kernel's integer/FP mix issues faster than that. SQ_INSTS_VALU counts a transcendental "
+                           "(half rate: see the probe) as one instruction, so frac understates the issue slots used",
+                   "source": src}
 
     def form_line(r, z_written, fused):
         lat_b = vl_bytes if not z_written else N * V * (4 + 1 / 8) + 4 * P * V + (4 * P * V if fused else 0)
@@ -549,6 +566,7 @@ def main():
                            "achieved": sweep_bytes * value / 1e9, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": sweep_bytes * value / 1e9 / HBM_PEAK_GBS},
         "copy_kernel_gbs": copy_gbs,
+        "valu_probe_winst_per_s": {"fma": fma_rate, "transcendental": trans_rate},
         "roofline": {"kernel": "k_overlap (one colour's latent draws + X'Z beside the other colour's update)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -93,6 +93,7 @@ def lib():
         "bsg_checkpoint_read": (C.c_int, [vp, C.c_char_p]),
         "bsg_measure_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, pd, pd]),
         "bsg_measure_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, pd, pd]),
+        "bsg_measure_valu_rate": (C.c_int, [C.c_int, C.c_int, pd, pd]),
         "bsg_coef_means": (C.c_int, [vp, C.c_int, pd]),
         "bsg_coef_means_noalloc": (C.c_int, [vp, C.c_int, pd]),
         "bsg_loglik": (C.c_int, [vp, C.c_int, C.c_int, pd]),
@@ -427,6 +428,15 @@ def copy_bandwidth(nbytes=1 << 30, reps=20):
     g, ms = C.c_double(), C.c_double()
     _check(lib().bsg_measure_copy_bandwidth(C.c_size_t(nbytes), int(reps), C.byref(g), C.byref(ms)))
     return g.value, ms.value
+
+
+def valu_rate(kind="fma", reps=100):
+    """(wave64 VALU instructions per second over the chip, ms per launch) of a
+    VALU-bound probe: kind 'fma' (v_fma_f32) or 'trans' (v_exp_f32)."""
+    k = {"fma": 0, "trans": 1}[kind]
+    w, ms = C.c_double(), C.c_double()
+    _check(lib().bsg_measure_valu_rate(k, int(reps), C.byref(w), C.byref(ms)))
+    return w.value, ms.value
 
 
 def stream_bandwidth(kind, nbytes=1 << 30, reps=20):

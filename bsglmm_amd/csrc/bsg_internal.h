@@ -245,6 +245,8 @@ hipError_t launch_pairs(const DevProblem &pb, hipStream_t st);
 hipError_t launch_wishart(const DevProblem &pb, bool side, uint32_t sweep, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_probe(int kind, const void *src, void *dst, size_t bytes, hipStream_t st);
+constexpr int VALU_CHAINS_PER_ROUND = 8;  // vector instructions per wave and round of k_valu
+hipError_t launch_valu(int kind, float *out, int blocks, int iters, hipStream_t st);
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st);
 hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st);
 // Z-free latent form: lanes per voxel chosen from N, its tile count and LDS bytes

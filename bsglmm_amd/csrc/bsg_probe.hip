@@ -1,5 +1,6 @@
 // bsg_probe.hip -- stream kernels that calibrate bandwidth and counter figures
-// (a copy, a write-only and a read-only stream over one buffer).
+// (a copy, a write-only and a read-only stream over one buffer), and VALU
+// kernels that measure the chip's wave64 instruction issue rate.
 #include "bsg_internal.h"
 
 namespace bsg {
@@ -57,6 +58,40 @@ hipError_t launch_probe(int kind, const void *src, void *dst, size_t bytes, hipS
   else
     hipLaunchKernelGGL(k_sum, dim3(8192), dim3(256), 0, st, static_cast<const float4 *>(src), n,
                        static_cast<float *>(dst));
+  return hipGetLastError();
+}
+
+// VALU issue probe: every thread runs `iters` rounds of VALU_CHAINS independent
+// chains of one instruction kind, 0: v_fma_f32, 1: v_exp_f32 (a
+// transcendental). The loop counter lives in scalar registers, so each round
+// issues exactly VALU_CHAINS vector instructions per wave; a data-dependent
+// store keeps the chains alive. Eight waves per SIMD hide each chain's latency.
+constexpr int VALU_CHAINS = VALU_CHAINS_PER_ROUND;
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu(float *out, int iters, float s)
+{
+  float a[VALU_CHAINS];
+#pragma unroll
+  for (int c = 0; c < VALU_CHAINS; c++) a[c] = 1.0f + 1e-3f * (float)threadIdx.x + (float)c;
+  for (int i = 0; i < iters; i++) {
+#pragma unroll
+    for (int c = 0; c < VALU_CHAINS; c++) a[c] = KIND == 0 ? fmaf(a[c], s, 0.5f) : __builtin_amdgcn_exp2f(a[c]);
+  }
+  float r = 0.0f;
+#pragma unroll
+  for (int c = 0; c < VALU_CHAINS; c++) r += a[c];
+  if (r == 1234.5f) out[blockIdx.x] = r;
+}
+
+// blocks of 256 threads; out holds at least `blocks` floats. Wave64 VALU
+// instructions in the timed loop: blocks * 4 * iters * VALU_CHAINS.
+hipError_t launch_valu(int kind, float *out, int blocks, int iters, hipStream_t st)
+{
+  if (kind == 0)
+    hipLaunchKernelGGL(k_valu<0>, dim3((unsigned)blocks), dim3(256), 0, st, out, iters, 0.99999f);
+  else
+    hipLaunchKernelGGL(k_valu<1>, dim3((unsigned)blocks), dim3(256), 0, st, out, iters, 0.5f);
   return hipGetLastError();
 }
 

@@ -299,6 +299,13 @@ int bsg_batch_get_launch_mode(bsg_batch *batch, int *unified);
 int bsg_measure_copy_bandwidth(size_t bytes, int reps, double *gbps, double *ms_per_copy);
 /* kind 0: copy (read + write counted), 1: write-only stream, 2: read-only stream */
 int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double *ms_per_launch);
+/* Times reps launches of a VALU-bound kernel (two rounds of eight resident
+ * 256-thread blocks per compute unit, 8 independent chains per thread) and
+ * returns the chip's wave64 vector-instruction rate: kind 0 FP32 FMA (full
+ * rate), 1 the v_exp_f32 transcendental. The denominator of an issue-bound
+ * kernel's roofline; run before timed work, it also leaves the GPU at the
+ * clock it holds under such a kernel. */
+int bsg_measure_valu_rate(int kind, int reps, double *winst_per_s, double *ms_per_launch);
 
 /* ---- host KISS, for seed.dat (randgen.cpp:9-23, 36-45; main.cu:54, 413) */
 double bsg_kiss(unsigned long seed[3]);

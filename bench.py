@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="time the headline sweeps as replays of a HIP graph prepared before the warm-up "
                          "(at 20 sweeps no faster than direct launches on this machine)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="time the headline sweeps in the persistent form (one k_persist launch per call; "
+                         "the same chain, slower on this machine than the overlapped default)")
     ap.add_argument("--config-lines", type=int, default=1, help="also time the C1 and C3 configurations")
     ap.add_argument("--c1-sweeps", type=int, default=2000, help="timed sweeps of the C1 line (its full run)")
     ap.add_argument("--c3-sweeps", type=int, default=200, help="timed sweeps of each C3 form")
@@ -188,7 +191,8 @@ def finish_cpu_legs(procs, timeout=900):
     return res
 
 
-def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, profile_sweeps=50, graph=False):
+def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, profile_sweeps=50, graph=False,
+              persistent=False):
     """Timed region: `steps` sweeps after `warmup` untimed direct ones,
     bracketed by stream synchronisations: direct launches (bsg_sweep), or
     with graph=True replays of a HIP graph of min(steps, 256) sweeps
@@ -208,6 +212,10 @@ def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, prof
     S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15, xtz_mode=mode, z_store=zstore)
     S.overlap = overlap
     ran_overlapped = S.overlap
+    if persistent:
+        S.persistent = True  # bsg_sweep: one k_persist launch per call (same chain)
+        if not S.persistent:
+            raise RuntimeError("the persistent sweep does not support this workload")
     per_graph = min(steps, 256)
     if graph:
         S.prepare_graph(per_graph)
@@ -494,7 +502,7 @@ def main():
         fm = valu_rate("fma", 100)
         return cb, fm, tr
 
-    head = time_mode(L, d, 2, 2, a.steps, a.warmup, settle=settle, graph=a.graph)
+    head = time_mode(L, d, 2, 2, a.steps, a.warmup, settle=settle, graph=a.graph, persistent=a.persistent)
     if not head["overlapped"]:
         raise RuntimeError("the headline workload must run overlapped")
     (copy_gbs, copy_ms), (fma_rate, _), (trans_rate, _) = head["settle"]
@@ -548,7 +556,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{a.config}: grid {nx}x{ny}x{nz} ellipsoidal-shell mask, plain single chain, "
                                f"FP32 state, X'Z on FP32 FMA fused into the latent draws (means x'b on FP32 MFMA), "
-                               f"Z not kept, overlapped sweep (two launches per sweep)",
+                               f"Z not kept, " + ("persistent sweep (one launch per call)" if a.persistent else
+                                                 "overlapped sweep (two launches per sweep)"),
                    "V": V, "N": N, "P": P, "chains": 1},
         "latent_draws_per_s": value * N * V,
         "timed_path": (f"bsg_sweep_graph: the {a.steps} timed sweeps replayed from a HIP graph of "

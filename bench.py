@@ -25,7 +25,8 @@ Prints one JSON line with the dominant kernel's roofline (its algorithmic
 bytes against 8 TB/s, its counted HBM bytes, and the limiter it actually
 meets: VALU issue, from the VALU count of its rocprofv3 profile over the
 launch time measured here, against the VALU issue rate a probe kernel
-measures in the same run), the effective sweep-level HBM fraction (the
+measures in the same run; and against the draw rate of the latent sampler
+alone, a probe without memory traffic), the effective sweep-level HBM fraction (the
 unfused sweep's bytes N V (8 + 1/8) + V (24 P + 5), which credits Z traffic
 this form never moves), a measured copy-kernel bandwidth, and the reference
 CPU path: C2 for 3 warm-up + 20 timed sweeps and C1 for its full 2,000
@@ -201,7 +202,7 @@ def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, prof
     same way for comparison. `settle` (optional) runs just before the
     warm-up, after the context is built, so that the GPU is busy and at its
     working clock when the few warm-up sweeps start (bench.py's
-    copy-bandwidth and VALU-rate measurements). Then, in the same run, `steps` more sweeps
+    copy-bandwidth, VALU-rate and sampler-rate probes). Then, in the same run, `steps` more sweeps
     with HIP events around every latent launch on the stream it runs on
     (kernel duration and sweep span; the event packets cost a few
     microseconds per sweep, so they stay out of the timed region), and a
@@ -484,7 +485,7 @@ def main():
     # the reference CPU path on two pinned cores of their own, beside the GPU work
     legs = [] if a.no_cpu else [("c2", a.cpu_warmup, a.cpu_sweeps), ("c1", 0, a.cpu_c1_sweeps)]
     procs = start_cpu_legs(legs) if legs else []
-    from bsglmm_amd import Layout, copy_bandwidth, synth, valu_rate
+    from bsglmm_amd import Layout, copy_bandwidth, sampler_rate, synth, valu_rate
 
     d = synth.generate(a.config)
     nx, ny, nz = d["grid"]
@@ -492,20 +493,22 @@ def main():
     P, N = d["covar"].shape
     V = L.totvox
 
-    # fused X'Z, Z not kept, overlapped; the copy-bandwidth and VALU-rate
-    # measurements run between building the context and the warm-up sweeps,
-    # the FP32 FMA probe (~0.1 s of VALU-bound work) last, so that the sweeps
-    # start at the clock the chip holds under VALU-bound work, as in a long run
+    # fused X'Z, Z not kept, overlapped; the probes (copy bandwidth, VALU
+    # rates, the latent sampler's draw rate) run between building the context
+    # and the warm-up sweeps, the sampler probe (~60 ms of the latent kernels'
+    # own instruction mix) last, so that the sweeps start at the clock the
+    # chip holds under that mix, as in a long run
     def settle():
         cb = copy_bandwidth(1 << 30, 100)
         tr = valu_rate("trans", 20)
         fm = valu_rate("fma", 100)
-        return cb, fm, tr
+        sr = sampler_rate(100)
+        return cb, fm, tr, sr
 
     head = time_mode(L, d, 2, 2, a.steps, a.warmup, settle=settle, graph=a.graph, persistent=a.persistent)
     if not head["overlapped"]:
         raise RuntimeError("the headline workload must run overlapped")
-    (copy_gbs, copy_ms), (fma_rate, _), (trans_rate, _) = head["settle"]
+    (copy_gbs, copy_ms), (fma_rate, _), (trans_rate, _), (sampler_dps, _) = head["settle"]
     forms = {"z_free_serial": (2, 2, False), "fused_xtz_z_stored": (2, 0, True), "separate_xtz_fp32": (0, 0, True),
              "separate_xtz_mfma_bf16x2": (1, 0, True)}
     others = {k: time_mode(L, d, m, z, a.steps, a.warmup, o) for k, (m, z, o) in forms.items()}
@@ -576,6 +579,11 @@ def main():
                            "unit": "GB/s", "frac": sweep_bytes * value / 1e9 / HBM_PEAK_GBS},
         "copy_kernel_gbs": copy_gbs,
         "valu_probe_winst_per_s": {"fma": fma_rate, "transcendental": trans_rate},
+        "sampler_ceiling": {"draws_per_s": sampler_dps,
+                            "what": "the latent sampler alone (Philox4x32-10 + inversion draw, no memory traffic), "
+                                    "k_sampler_probe",
+                            "sweep_latent_draws_frac": value * N * V / sampler_dps,
+                            "latent_kernel_draws_frac": (N * V / 2) / (launch_s * sampler_dps)},
         "roofline": {"kernel": "k_overlap (one colour's latent draws + X'Z beside the other colour's update)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -94,6 +94,7 @@ def lib():
         "bsg_measure_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, pd, pd]),
         "bsg_measure_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, pd, pd]),
         "bsg_measure_valu_rate": (C.c_int, [C.c_int, C.c_int, pd, pd]),
+        "bsg_measure_sampler_rate": (C.c_int, [C.c_int, pd, pd]),
         "bsg_coef_means": (C.c_int, [vp, C.c_int, pd]),
         "bsg_coef_means_noalloc": (C.c_int, [vp, C.c_int, pd]),
         "bsg_loglik": (C.c_int, [vp, C.c_int, C.c_int, pd]),
@@ -437,6 +438,14 @@ def valu_rate(kind="fma", reps=100):
     w, ms = C.c_double(), C.c_double()
     _check(lib().bsg_measure_valu_rate(k, int(reps), C.byref(w), C.byref(ms)))
     return w.value, ms.value
+
+
+def sampler_rate(reps=100):
+    """(draws per second over the chip, ms per launch) of the latent sampler
+    alone: Philox4x32-10 and the inversion draw, no memory traffic."""
+    r, ms = C.c_double(), C.c_double()
+    _check(lib().bsg_measure_sampler_rate(int(reps), C.byref(r), C.byref(ms)))
+    return r.value, ms.value
 
 
 def stream_bandwidth(kind, nbytes=1 << 30, reps=20):

@@ -1969,6 +1969,55 @@ int bsg_measure_valu_rate(int kind, int reps, double *winst_per_s, double *ms_pe
   return rc;
 }
 
+int bsg_measure_sampler_rate(int reps, double *draws_per_s, double *ms_per_launch)
+{
+  if (!draws_per_s || reps < 1) {
+    set_error("bsg_measure_sampler_rate: bad argument");
+    return BSG_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_probe_mu);
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= bsg::MAX_DEVICES ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+    set_error("bsg_measure_sampler_rate: no device");
+    return BSG_ERR_HIP;
+  }
+  // two rounds of eight resident 256-thread blocks per compute unit; about
+  // 1 ms per launch on 256 compute units
+  const int blocks = std::min(16 * cus, VALU_BLOCKS_MAX), rounds = 128;
+  if (!g_valu_out[dev] && hipMalloc((void **)&g_valu_out[dev], sizeof(float) * VALU_BLOCKS_MAX) != hipSuccess) {
+    g_valu_out[dev] = nullptr;
+    set_error("bsg_measure_sampler_rate: hipMalloc failed");
+    return BSG_ERR_NOMEM;
+  }
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = BSG_OK;
+  float ms = 0.0f;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess || bsg::launch_sampler_probe(g_valu_out[dev], blocks, rounds, st) != hipSuccess ||
+      hipEventRecord(e0, st) != hipSuccess) {
+    rc = BSG_ERR_HIP;
+  } else {
+    for (int i = 0; i < reps && rc == BSG_OK; i++)
+      if (bsg::launch_sampler_probe(g_valu_out[dev], blocks, rounds, st) != hipSuccess) rc = BSG_ERR_HIP;
+    if (rc == BSG_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f))
+      rc = BSG_ERR_HIP;
+  }
+  if (rc != BSG_OK) set_error("bsg_measure_sampler_rate: HIP failure");
+  if (rc == BSG_OK) {
+    const double draws = (double)blocks * 256.0 * rounds * 4.0;
+    *draws_per_s = draws * reps / (ms * 1e-3);
+    if (ms_per_launch) *ms_per_launch = ms / reps;
+  }
+  if (st) hipStreamSynchronize(st);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  return rc;
+}
+
 // ---- host KISS (randgen.cpp:9-23, 36-45; main.cu:54) -------------------
 double bsg_kiss(unsigned long seed[3])
 {

@@ -18,6 +18,7 @@ constexpr uint32_t TAG_GAMMA = 3;   // index = Bartlett row, sub = trial
 constexpr uint32_t TAG_WNORM = 4;   // index = row * P + column
 constexpr uint32_t TAG_BWM = 5;     // index = 0: normal of the WM coefficient draw
 constexpr uint32_t TAG_FALLBACK = 6; // index = voxel * NQ + quad, sub = subject in quad: exact redraw
+constexpr uint32_t TAG_PROBE = 7;    // the sampler probe's own stream (no chain state)
 
 constexpr int PAIR_BLOCKS = 512;    // fixed partial count per chain: deterministic sums
 constexpr int BSTAT_SLOTS = 4096;   // per-chain partial slots of the WM-coefficient statistic
@@ -247,6 +248,10 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 hipError_t launch_probe(int kind, const void *src, void *dst, size_t bytes, hipStream_t st);
 constexpr int VALU_CHAINS_PER_ROUND = 8;  // vector instructions per wave and round of k_valu
 hipError_t launch_valu(int kind, float *out, int blocks, int iters, hipStream_t st);
+// The latent sampler alone (Philox4x32-10 and the inversion draw of a quad,
+// no loads or stores in the loop): blocks of 256 threads, `rounds` quads per
+// thread; out holds at least `blocks` floats.
+hipError_t launch_sampler_probe(float *out, int blocks, int rounds, hipStream_t st);
 hipError_t launch_accum(const DevProblem &pb, const Accum &ac, int batch, hipStream_t st);
 hipError_t launch_tick(uint32_t *base, uint32_t k, hipStream_t st);
 // Z-free latent form: lanes per voxel chosen from N, its tile count and LDS bytes

@@ -3510,6 +3510,42 @@ hipError_t launch_persist(const DevProblem &pb, const PTags &tg, int nblocks, hi
   return hipGetLastError();
 }
 
+// Sampler probe: the draws of the latent kernels without their data. Each
+// thread has four means spread over [-4, 2], moved by a step that changes
+// every round (so that nothing of the draw is the same from round to round),
+// and a response per subject (about one in eight a lesion); it draws `rounds`
+// quads from its own Philox counters, summing the draws so that none is
+// dropped. Its draw rate is the ceiling of the latent kernels, which add the
+// means (MFMA), X'Z, the loads of b_j and y and the stores of m_j.
+__global__ __launch_bounds__(256) void k_sampler_probe(float *out, int rounds)
+{
+  const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+  uint32_t h[4];
+  philox10(gid, 0u, TAG_PROBE, 1u, 0x5A3F1C27u, 0x0BADCAFEu, h);
+  float mu[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) mu[l] = fmaf(6.0f, u01f(h[l]), -4.0f);
+  const uint32_t yq = ((h[0] & 7u) == 0u ? 1u : 0u) | ((h[1] & 7u) == 0u ? 2u : 0u) |
+                      ((h[2] & 7u) == 0u ? 4u : 0u) | ((h[3] & 7u) == 0u ? 8u : 0u);
+  float acc = 0.0f;
+  for (int r = 0; r < rounds; r++) {
+    uint32_t rr[4];
+    philox10(gid, (uint32_t)r, TAG_PROBE, 0u, 0x5A3F1C27u, 0x0BADCAFEu, rr);
+    const float dr = (float)(r & 31) * 0.0625f - 1.0f;
+    const float m[4] = {mu[0] + dr, mu[1] + dr, mu[2] + dr, mu[3] + dr};
+    float z[4];
+    latent_quad(m, yq, rr, 1e-6f, z);
+    acc += (z[0] + z[1]) + (z[2] + z[3]);
+  }
+  if (acc == 1234.5f) out[blockIdx.x] = acc;
+}
+
+hipError_t launch_sampler_probe(float *out, int blocks, int rounds, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_sampler_probe, dim3((unsigned)blocks), dim3(256), 0, st, out, rounds);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero64(unsigned long long *p, size_t n, hipStream_t st)
 {
   if (n == 0) return hipSuccess;

@@ -306,6 +306,11 @@ int bsg_measure_bandwidth(int kind, size_t bytes, int reps, double *gbps, double
  * kernel's roofline; run before timed work, it also leaves the GPU at the
  * clock it holds under such a kernel. */
 int bsg_measure_valu_rate(int kind, int reps, double *winst_per_s, double *ms_per_launch);
+/* Times reps launches of the latent sampler alone (Philox4x32-10 and the
+ * truncated-normal inversion draw, no memory traffic; two rounds of eight
+ * resident 256-thread blocks per compute unit) and returns its draws per
+ * second: the ceiling of the latent kernels' draw rate on this device. */
+int bsg_measure_sampler_rate(int reps, double *draws_per_s, double *ms_per_launch);
 
 /* ---- host KISS, for seed.dat (randgen.cpp:9-23, 36-45; main.cu:54, 413) */
 double bsg_kiss(unsigned long seed[3]);

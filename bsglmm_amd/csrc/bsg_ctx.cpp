@@ -394,6 +394,9 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   ALLOC(wmd, (size_t)V);
   float4 *xqd = nullptr;
   if (P > bsg::MAX_P) ALLOC(xqd, (size_t)ctx->NQ * P);
+  uint16_t *xbfd = nullptr;  // BF16 hi/lo rows of X for the MFMA X'Z
+  const int NSB = (NS + 7) & ~7;
+  if (P <= bsg::MAX_P) ALLOC(xbfd, (size_t)2 * P * NSB);
   ALLOC(pb.Z, o.z_store == 2 ? (size_t)1 : (size_t)C * V * NS);
   ALLOC(pb.beta, (size_t)C * V * P);
   ALLOC(pb.M, (size_t)C * V * P);
@@ -445,6 +448,8 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.ybits = yb;
   pb.X = Xd;
   pb.Xq = xqd;
+  pb.xbf = xbfd;
+  pb.NSB = NSB;
   pb.xtx = xtxd;
   pb.xsum = xsumd;
   pb.wm = wmd;
@@ -453,6 +458,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   HIPCHK(hipMemcpyAsync(nbr6, L.nbr6.data(), sizeof(int) * L.nbr6.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(yb, ybits.data(), sizeof(uint32_t) * ybits.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(Xd, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice, st));
+  if (xbfd) HIPCHK(bsg::launch_xbf(pb, xbfd, st));
   std::vector<float> xq;
   if (xqd) {  // [NQ][P] quads of X for the wide kernels, zero past N
     xq.assign((size_t)ctx->NQ * P * 4, 0.0f);

@@ -52,6 +52,8 @@ struct DevProblem {
   const uint32_t *ybits;   // [V][NW], bit i of row s = (y_is == 1)
   const float *X;          // [P][NS], zero past N
   const float4 *Xq;        // [NQ][P] quads of X, zero past N (wide problems, P > MAX_P)
+  const uint16_t *xbf;     // [2P][NSB] BF16 bits of X_hi (rows 0..P-1) and X_lo (P..2P-1), zero past N (P <= MAX_P)
+  int NSB;                 // NS rounded up to a multiple of 8
   const float *xtx;        // [P][P]
   const float *xsum;       // [P]  column sums of X
   const float *wm;         // [V]
@@ -232,6 +234,8 @@ hipError_t launch_persist(const DevProblem &pb, const PTags &tg, int nblocks, hi
 int persist_max_blocks(const DevProblem &pb);  // resident blocks of the kernel on this device
 // zero n tagged words (a fill kernel: no memset node if captured)
 hipError_t launch_zero64(unsigned long long *p, size_t n, hipStream_t st);
+// the BF16 hi/lo split of X for the MFMA X'Z (pb.xbf), from pb.X, once at creation
+hipError_t launch_xbf(const DevProblem &pb, uint16_t *xbf, hipStream_t st);
 // WM coefficient draw. parts > 0: the latent kernel already wrote that many
 // partials; parts == 0: they are formed here from the stored Z first.
 hipError_t launch_bwm(const DevProblem &pb, int parts, uint32_t sweep, hipStream_t st);

@@ -1251,7 +1251,10 @@ __global__ __launch_bounds__(256) void k_xtz_f32(DevProblem pb, int lshift, int 
 // columns, so D = (Z_hi + Z_lo)[X_hi | X_lo] from two MFMAs per block and
 // m_j,c = D[j][c] + D[j][P + c]. A and B use the same (lane, slot) -> k map,
 // so the contraction is right whatever order the hardware sums k in; the
-// output layout is lane l -> D[4 (l / 16) + i][l % 16].
+// output layout is lane l -> D[4 (l / 16) + i][l % 16]. X is constant, so its
+// split is formed once (k_xbf, pb.xbf) and each lane loads its eight B values
+// as 16 bytes; Z is the stream, and each lane loads the next K step's eight
+// values before it splits and multiplies the current ones.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -1276,6 +1279,42 @@ __device__ __forceinline__ void load8(const float *row, int k, int NS, float (&v
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 
+// BF16 hi/lo rows of X: xbf[c][k] = hi(X[c][k]) for c < P, lo(X[c - P][k])
+// for P <= c < 2P, with the same conversions k_xtz_mfma made in its loop
+__global__ __launch_bounds__(256) void k_xbf(const float *X, int P, int NS, int NSB, uint16_t *xbf)
+{
+  const int n = 2 * P * NSB;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int c = i / NSB, k = i - c * NSB;
+    const float v = k < NS ? X[(size_t)(c < P ? c : c - P) * NS + k] : 0.0f;
+    const __bf16 h = (__bf16)v;
+    const __bf16 o = c < P ? h : (__bf16)(v - (float)h);
+    xbf[i] = __builtin_bit_cast(uint16_t, o);
+  }
+}
+
+hipError_t launch_xbf(const DevProblem &pb, uint16_t *xbf, hipStream_t st)
+{
+  const int n = 2 * pb.P * pb.NSB;
+  hipLaunchKernelGGL(k_xbf, dim3((unsigned)std::max(1, std::min((n + 255) / 256, 1024))), dim3(256), 0, st, pb.X,
+                     pb.P, pb.NS, pb.NSB, xbf);
+  return hipGetLastError();
+}
+
+// load8 without branches: both float4 loads always issue, from addresses
+// clamped into the row (NS >= 4, a multiple of 4), and values past the row
+// end are zeroed by selects
+__device__ __forceinline__ void load8c(const float *row, int k, int NS, float (&v)[8])
+{
+  float4 a = *reinterpret_cast<const float4 *>(row + min(k, NS - 4));
+  float4 b = *reinterpret_cast<const float4 *>(row + min(k + 4, NS - 4));
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (k >= NS) a = zero;
+  if (k + 4 >= NS) b = zero;
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
 {
@@ -1289,30 +1328,28 @@ __global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
   const int srow = min(s0 + r, pb.ve - 1);
   const float *zrow = pb.Z + ((size_t)chain * pb.V + srow) * pb.NS;
   const int col = r;  // B column of this lane
-  const float *xrow = pb.X + (size_t)(col < P ? col : (col < 2 * P ? col - P : 0)) * pb.NS;
-  const bool xlo = col >= P && col < 2 * P;
   const bool xon = col < 2 * P;
+  const uint16_t *xrow = pb.xbf + (size_t)(xon ? col : 0) * pb.NSB;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  float zv[8];
+  load8c(zrow, 8 * g, pb.NS, zv);
   for (int k0 = 0; k0 < pb.NS; k0 += 32) {
     const int k = k0 + 8 * g;
-    float zv[8], xv[8];
-    load8(zrow, k, pb.NS, zv);
+    // B of this step first, then the next step's Z: loads retire in order,
+    // so waiting for B leaves the next step's loads in flight. The loads are
+    // unconditional (addresses clamped into the row, values past it zeroed
+    // by selects), so no branch hides them from the wait counts.
+    uint4 xw = *reinterpret_cast<const uint4 *>(xrow + min(k, pb.NSB - 8));
+    if (!(xon && k < pb.NSB)) xw = make_uint4(0u, 0u, 0u, 0u);
+    float zn[8];
+    load8c(zrow, k + 32, pb.NS, zn);  // the next step (zero past the row)
     bf16x8 zh, zl;
     split8(zv, zh, zl);
-    bf16x8 xb;
-    if (xon) {
-      load8(xrow, k, pb.NS, xv);
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const __bf16 h = (__bf16)xv[j];
-        xb[j] = xlo ? (__bf16)(xv[j] - (float)h) : h;
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; j++) xb[j] = (__bf16)0.0f;
-    }
+    const bf16x8 xb = __builtin_bit_cast(bf16x8, xw);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zh, xb, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zl, xb, acc, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 8; j++) zv[j] = zn[j];
   }
 #pragma unroll
   for (int i = 0; i < 4; i++) {

@@ -387,7 +387,8 @@ def uneven_batch(steps):
     for S in samplers:
         S.close()
     small = small_batch(max(steps, 100))
-    return {"small_analyses_batch": small, "analyses": rows, "chains_per_analysis": 4,
+    prec = precision_batch(max(steps // 2, 10))
+    return {"small_analyses_batch": small, "precision_form_batch": prec, "analyses": rows, "chains_per_analysis": 4,
             "latent_draws_per_batch_sweep": draws,
             "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
             "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat,
@@ -397,6 +398,41 @@ def uneven_batch(steps):
             "batch_sweep_ms": 1e3 * bat / steps, "graph_batch_sweep_ms": 1e3 * gra / steps,
             "form": "FP32, Z-free fused X'Z (the BF16-split MFMA X'Z needs Z in HBM and is slower, "
                     "see forms.separate_xtz_mfma_bf16x2)"}
+
+
+def precision_batch(steps):
+    """The same twelve analyses (4 chains each) in the configuration's stated
+    form: Z stored (k_latent body), X'Z on the matrix cores with Z split into
+    BF16 hi + lo (k_xtz_mfma body). Batched (one launch per phase and
+    covariate count, each count on its own stream) against one at a time and
+    against concurrent streams."""
+    from bsglmm_amd import Batch, Layout, Sampler, synth
+    samplers, draws = [], 0
+    for k in range(12):
+        d = synth.generate(f"c5_{k}")
+        L = Layout(d["mask"], *d["grid"])
+        samplers.append(Sampler(L, d["covar"], d["data"], seed=0xC5C5 + k, nchains=4, xtz_mode=1, z_store=0))
+        P, N = d["covar"].shape
+        draws += 4 * N * L.totvox
+        del d
+    for S in samplers:
+        S.sweep(2)
+        S.synchronize()
+    seq, conc = baselines(samplers, steps, interleave=False)
+    B = Batch(samplers)
+    B.sweep(2)
+    t0 = time.perf_counter()
+    B.sweep(steps)
+    bat = time.perf_counter() - t0
+    B.close()
+    for S in samplers:
+        S.close()
+    return {"form": "precision: Z stored, X'Z on MFMA with Z split into BF16 hi + lo, FP32 accumulation",
+            "analyses": 12, "chains_per_analysis": 4, "timed_batch_sweeps": steps,
+            "batch_sweeps_per_s": steps / bat, "batch_sweep_ms": 1e3 * bat / steps,
+            "batched_draws_per_s": draws * steps / bat, "one_by_one_draws_per_s": draws * steps / seq,
+            "batched_over_one_by_one": seq / bat, "concurrent_streams_draws_per_s": draws * steps / conc,
+            "batched_over_concurrent_streams": conc / bat}
 
 
 def baselines(samplers, steps, interleave):

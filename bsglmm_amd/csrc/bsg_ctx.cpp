@@ -2231,9 +2231,11 @@ struct bsg_batch {
   bsg::WorkItem *d_items = nullptr;
   bsg::Accum *d_accs = nullptr;
   // item ranges per phase (latent, coef 0, coef 1, wishart, latent of the
-  // members drawn by inversion at G = 4) and P; save sweeps accumulate in
-  // the coefficient phases
-  int off[5][bsg::MAX_P + 1] = {}, cnt[5][bsg::MAX_P + 1] = {};
+  // members drawn by inversion at G = 4, latent with Z stored and X'Z on
+  // MFMA of the members in the precision form) and P; save sweeps accumulate
+  // in the coefficient phases
+  int off[7][bsg::MAX_P + 1] = {}, cnt[7][bsg::MAX_P + 1] = {};
+  bool zform = false;  // a member in the precision form (one launch per phase and P then)
   size_t lat_lds[bsg::MAX_P + 1] = {};  // per P: the largest X staging of its members
   // one launch per phase over every covariate count, on s0 (phases latent,
   // colour 0, colour 1, Wishart): chosen for batches whose sweep is too small
@@ -2242,7 +2244,7 @@ struct bsg_batch {
   size_t lds_all = 0;
   long long draws = 0;  // latent draws per batch sweep
   int launch_mode = 0;  // 0 chosen from the draws, 1 one launch per phase, 2 one per phase and P
-  bool unified() const { return launch_mode == 1 || (launch_mode == 0 && draws < (1LL << 28)); }
+  bool unified() const { return !zform && (launch_mode == 1 || (launch_mode == 0 && draws < (1LL << 28))); }
   bool bmf = false;
   std::vector<bsg::Accum> h_accs;
   // graph-captured groups of batch sweeps (bsg_batch_sweep_graph): the
@@ -2291,7 +2293,7 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     return BSG_ERR_NOMEM;
   }
   std::vector<bsg::DevProblem> pbs;
-  std::vector<bsg::WorkItem> lists[5];
+  std::vector<bsg::WorkItem> lists[7];
   for (int a = 0; a < n; a++) {
     bsg_ctx *c = members[a];
     if (!c) {
@@ -2302,9 +2304,12 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
       set_error("bsg_batch_create: batches take members of at most " + std::to_string(bsg::MAX_P) + " covariates");
       return BSG_ERR_ARG;
     }
-    if (c->opts.z_store != 2 || c->opts.xtz_mode != 2 || c->pb.wm_update || c->L.n[1] == 0) {
-      set_error("bsg_batch_create: members must use the Z-free form (xtz_mode 2, z_store 2), have no WM "
-                "image and two colours");
+    // the Z-free form, or the precision form (Z stored, X'Z on MFMA, inversion draws)
+    const bool zfree = c->opts.z_store == 2 && c->opts.xtz_mode == 2;
+    const bool zprec = c->opts.z_store == 0 && c->opts.xtz_mode == 1 && c->opts.latent_method == 0;
+    if (!(zfree || zprec) || c->pb.wm_update || c->L.n[1] == 0) {
+      set_error("bsg_batch_create: members must use the Z-free form (xtz_mode 2, z_store 2) or the precision form "
+                "(xtz_mode 1, z_store 0, latent_method 0), have no WM image and two colours");
       return BSG_ERR_ARG;
     }
     if (c->opts.latent_method != members[0]->opts.latent_method || c->sweep != members[0]->sweep) {
@@ -2313,15 +2318,19 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     bsg::DevProblem pb = c->pb;
-    if (pb.lat_g <= 0) pb.lat_g = bsg::latent_vl_groups(pb);
+    if (zfree && pb.lat_g <= 0) pb.lat_g = bsg::latent_vl_groups(pb);
     pbs.push_back(pb);
-    b->lat_lds[pb.P] = std::max(b->lat_lds[pb.P], bsg::latent_vl_lds_bytes(pb));
-    const int lat_list = (pb.lat_g == 4 && c->opts.latent_method == 0) ? 4 : 0;
+    if (zfree) b->lat_lds[pb.P] = std::max(b->lat_lds[pb.P], bsg::latent_vl_lds_bytes(pb));
+    b->zform = b->zform || zprec;
+    const int lat_list = zprec ? 5 : ((pb.lat_g == 4 && c->opts.latent_method == 0) ? 4 : 0);
     // colour-0 items past its blocks write colour 1's m_j from its pieces
-    const int combine = bsg::vl_ctiles(pb) ? bsg::vl_range(pb, 1).left : 0;
-    const int tiles = lat_list == 4 ? bsg::latent_vl_blocks_g4(pb) : bsg::latent_vl_tiles(pb, pb.lat_g);
+    const int combine = (zfree && bsg::vl_ctiles(pb)) ? bsg::vl_range(pb, 1).left : 0;
+    const int tiles = zprec ? bsg::latent_blocks(pb)
+                            : (lat_list == 4 ? bsg::latent_vl_blocks_g4(pb) : bsg::latent_vl_tiles(pb, pb.lat_g));
     for (int ch = 0; ch < c->C; ch++) {
       for (int t = 0; t < tiles; t++) lists[lat_list].push_back({a, ch, t});
+      if (zprec)
+        for (int t = 0; t < bsg::xtz_mfma_blocks(pb); t++) lists[6].push_back({a, ch, t});
       for (int k = 0; k < (c->L.n[0] + 255) / 256 + combine; k++) lists[1].push_back({a, ch, k});
       for (int k = 0; k < (c->L.n[1] + 255) / 256; k++) lists[2].push_back({a, ch, k});
       lists[3].push_back({a, ch, 0});
@@ -2330,7 +2339,7 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
   }
   b->bmf = members[0]->opts.latent_method == 1;
   std::vector<bsg::WorkItem> all;
-  for (int k = 0; k < 5; k++)
+  for (int k = 0; k < 7; k++)
     for (int P = 1; P <= bsg::MAX_P; P++) {
       b->off[k][P] = (int)all.size();
       for (const bsg::WorkItem &w : lists[k])
@@ -2357,7 +2366,7 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
     return BSG_ERR_NOMEM;
   }
   for (int P = 1; P <= bsg::MAX_P; P++)
-    if (b->cnt[0][P] + b->cnt[4][P] > 0) {
+    if (b->cnt[0][P] + b->cnt[4][P] + b->cnt[5][P] > 0) {
       HIPCHK(hipStreamCreateWithFlags(&b->sp[P], hipStreamNonBlocking));
       HIPCHK(hipEventCreateWithFlags(&b->ev_join[P], hipEventDisableTiming));
     }
@@ -2383,8 +2392,9 @@ int bsg_batch_destroy(bsg_batch *b)
 }
 
 // One sweep of every analysis and chain, one launch per phase and covariate
-// count P: latent draws and X'Z, colour 0, colour 1 (+ pair partials),
-// Wishart, in that order on the stream of P. The kernels read their Philox
+// count P: latent draws and X'Z (the precision form: latent draws with Z
+// stored, then X'Z on MFMA), colour 0, colour 1 (+ pair partials), Wishart,
+// in that order on the stream of P. The kernels read their Philox
 // sweep word as sw + *sweep_base of the problems in pbs.
 static int batch_enqueue(bsg_batch *b, const bsg::DevProblem *pbs, uint32_t sw, const bsg::Accum *accs,
                          const bsg::BatchIndex &bidx)
@@ -2401,6 +2411,8 @@ static int batch_enqueue(bsg_batch *b, const bsg::DevProblem *pbs, uint32_t sw, 
     if (!st) continue;
     HIPCHK(bsg::launch_latent_multi(P, pbs, it + b->off[0][P], b->cnt[0][P], b->lat_lds[P], b->bmf, false, sw, st));
     HIPCHK(bsg::launch_latent_multi(P, pbs, it + b->off[4][P], b->cnt[4][P], b->lat_lds[P], false, true, sw, st));
+    HIPCHK(bsg::launch_latent_multi_z(P, pbs, it + b->off[5][P], b->cnt[5][P], sw, st));
+    HIPCHK(bsg::launch_xtz_multi(P, pbs, it + b->off[6][P], b->cnt[6][P], st));
     HIPCHK(bsg::launch_coef_multi(P, pbs, it + b->off[1][P], b->cnt[1][P], 0, false, sw, st, accs, bidx));
     HIPCHK(bsg::launch_coef_multi(P, pbs, it + b->off[2][P], b->cnt[2][P], 1, true, sw, st, accs, bidx));
     HIPCHK(bsg::launch_wishart_multi(P, pbs, it + b->off[3][P], b->cnt[3][P], sw, st));

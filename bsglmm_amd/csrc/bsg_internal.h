@@ -197,6 +197,35 @@ struct Accum {
 struct WorkItem {
   int a, chain, blk;
 };
+// k_latent (Z stored): lanes per voxel 2^lshift, the fewest covering the
+// subject quads up to 64, and reps quads per lane; each wave walks
+// LATENT_VITER voxel groups
+constexpr int LATENT_VITER = 8;
+__host__ __device__ inline void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
+{
+  int L = 1;
+  lshift = 0;
+  while (L < pb.NQ && L < 64) {
+    L <<= 1;
+    lshift++;
+  }
+  reps = (pb.NQ + L - 1) / L;
+}
+// blocks of a whole-range k_latent launch (one chain) and of k_xtz_mfma
+inline int latent_blocks(const DevProblem &pb)
+{
+  int lshift, reps;
+  lane_grouping(pb, lshift, reps);
+  const int vpw = 64 >> lshift;
+  const long long groups = ((long long)pb.V + vpw - 1) / vpw;
+  const long long waves = (groups + LATENT_VITER - 1) / LATENT_VITER;
+  return (int)((waves + 3) / 4);
+}
+inline int xtz_mfma_blocks(const DevProblem &pb)
+{
+  const long long tiles = ((long long)pb.V + 15) / 16;
+  return (int)((tiles + 3) / 4);
+}
 constexpr int MAX_BATCH = 32;  // analyses per batch
 struct BatchIndex {
   int b[MAX_BATCH];  // posterior-accumulator batch of each analysis
@@ -268,6 +297,10 @@ size_t latent_vl_lds_bytes(const DevProblem &pb);
 hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, size_t lds, bool bmf,
                                bool g4, uint32_t sweep, hipStream_t st);
 // accs != null: a save sweep, each member's updated voxels are accumulated (batch bidx)
+// precision-form members: Z stored (k_latent body) and X'Z on MFMA (k_xtz_mfma body)
+hipError_t launch_latent_multi_z(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,
+                                 hipStream_t st);
+hipError_t launch_xtz_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, hipStream_t st);
 hipError_t launch_coef_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, int colour, bool prec,
                              uint32_t sweep, hipStream_t st, const Accum *accs, const BatchIndex &bidx);
 hipError_t launch_wishart_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,

@@ -346,14 +346,14 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 // ZS: 0 = store Z, 1 = store Z with nontemporal (streaming) stores,
 // 2 = do not keep Z (only with FUSED: X'Z is formed from the draws in registers)
+// the body of block blk of chain `chain` (sweep: the Philox sweep word)
 template <int P, bool XREG, bool FUSED, int ZS>
-__global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps, int viter, uint32_t sweep)
+__device__ __forceinline__ void latent_body(const DevProblem &pb, int lshift, int reps, int viter, uint32_t sweep,
+                                            int chain, int blk)
 {
-  sweep += *pb.sweep_base;
   static_assert(ZS != 2 || FUSED, "Z can only be dropped when X'Z is fused");
-  const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = blk * 4 + (threadIdx.x >> 6);
   const int vpw = 64 >> lshift;
   const int sub = lane >> lshift;
   const int ql = lane & ((1 << lshift) - 1);
@@ -458,6 +458,12 @@ __global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int r
       }
     }
   }
+}
+
+template <int P, bool XREG, bool FUSED, int ZS>
+__global__ __launch_bounds__(256) void k_latent(DevProblem pb, int lshift, int reps, int viter, uint32_t sweep)
+{
+  latent_body<P, XREG, FUSED, ZS>(pb, lshift, reps, viter, sweep + *pb.sweep_base, blockIdx.y, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1315,13 +1321,13 @@ __device__ __forceinline__ void load8c(const float *row, int k, int NS, float (&
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 
+// block blk of chain `chain`: four waves of 16 voxels each
 template <int P>
-__global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
+__device__ __forceinline__ void xtz_mfma_body(const DevProblem &pb, int chain, int blk)
 {
   static_assert(2 * P <= 16, "hi/lo columns must fit 16");
-  const int chain = blockIdx.y;
   const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int tile = blk * 4 + (threadIdx.x >> 6);
   const int s0 = pb.vb + tile * 16;
   if (s0 >= pb.ve) return;  // wave-uniform
   const int r = lane & 15, g = lane >> 4;
@@ -1357,6 +1363,12 @@ __global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
     const int s = s0 + 4 * g + i;
     if (col < P && s < pb.ve) pb.M[((size_t)chain * pb.V + s) * P + col] = acc[i] + other;
   }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_xtz_mfma(DevProblem pb)
+{
+  xtz_mfma_body<P>(pb, blockIdx.y, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2564,6 +2576,28 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_multi_mfm(const 
   else latent_vl_mfm_body<P, false>(pb, sweep + *pb.sweep_base, it.chain, 0, pb.V, it.blk, 1 << 30, 0);
 }
 
+// members in the precision form (Z stored, X'Z on the matrix cores): item
+// blk is block blk of the member's own whole-range k_latent launch (Z stored,
+// no X'Z), then of its k_xtz_mfma launch
+template <int P>
+__global__ __launch_bounds__(256) void k_latent_multi_z(const DevProblem *pbs, const WorkItem *items, uint32_t sweep)
+{
+  const WorkItem it = items[blockIdx.x];
+  const DevProblem pb = pbs[it.a];
+  int lshift, reps;
+  lane_grouping(pb, lshift, reps);
+  const uint32_t sw = sweep + *pb.sweep_base;
+  if (reps == 1) latent_body<P, true, false, 0>(pb, lshift, reps, LATENT_VITER, sw, it.chain, it.blk);
+  else latent_body<P, false, false, 0>(pb, lshift, reps, LATENT_VITER, sw, it.chain, it.blk);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_xtz_mfma_multi(const DevProblem *pbs, const WorkItem *items)
+{
+  const WorkItem it = items[blockIdx.x];
+  xtz_mfma_body<P>(pbs[it.a], it.chain, it.blk);
+}
+
 template <int P, bool PREC>
 __global__ __launch_bounds__(256) void k_coef_multi(const DevProblem *pbs, const WorkItem *items, int colour,
                                                     uint32_t sweep, const Accum *accs, BatchIndex bidx)
@@ -3177,16 +3211,6 @@ __global__ __launch_bounds__(256) void k_loglik_w(DevProblem pb, int T, double *
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static void lane_grouping(const DevProblem &pb, int &lshift, int &reps)
-{
-  int L = 1;
-  lshift = 0;
-  while (L < pb.NQ && L < 64) {
-    L <<= 1;
-    lshift++;
-  }
-  reps = (pb.NQ + L - 1) / L;
-}
 
 
 #define BSG_DISPATCH_P(PVAL, CALL) \
@@ -3361,7 +3385,7 @@ hipError_t launch_latent(const DevProblem &pb0, bool fused, int zstore, uint32_t
   lane_grouping(pb, lshift, reps);
   const int vpw = 64 >> lshift;
   const long long groups = ((long long)(v1 - v0) + vpw - 1) / vpw;
-  const int viter = 8;
+  const int viter = LATENT_VITER;
   const long long waves = (groups + viter - 1) / viter;
   dim3 grid((unsigned)((waves + 3) / 4), (unsigned)pb.nchains);
   const bool xreg = reps == 1;
@@ -3732,6 +3756,27 @@ hipError_t launch_latent_multi(int P, const DevProblem *pbs, const WorkItem *ite
     else if (bmf) hipLaunchKernelGGL((k_latent_multi<PP, true>), grid, dim3(256), lds, st, pbs, items, sweep); \
     else hipLaunchKernelGGL((k_latent_multi<PP, false>), grid, dim3(256), lds, st, pbs, items, sweep);       \
   } while (0)
+  BSG_DISPATCH_P(P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_latent_multi_z(int P, const DevProblem *pbs, const WorkItem *items, int n, uint32_t sweep,
+                                 hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  if (P < 1 || P > MAX_P) return hipErrorInvalidValue;
+#define L_(PP) hipLaunchKernelGGL(k_latent_multi_z<PP>, dim3((unsigned)n), dim3(256), 0, st, pbs, items, sweep)
+  BSG_DISPATCH_P(P, L_)
+#undef L_
+  return hipGetLastError();
+}
+
+hipError_t launch_xtz_multi(int P, const DevProblem *pbs, const WorkItem *items, int n, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  if (P < 1 || P > MAX_P) return hipErrorInvalidValue;
+#define L_(PP) hipLaunchKernelGGL(k_xtz_mfma_multi<PP>, dim3((unsigned)n), dim3(256), 0, st, pbs, items)
   BSG_DISPATCH_P(P, L_)
 #undef L_
   return hipGetLastError();

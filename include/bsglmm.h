@@ -268,9 +268,12 @@ typedef struct bsg_batch bsg_batch;
  * P on its own stream so that the launches of different P overlap). The
  * first is chosen for batches of fewer than 2^28 latent draws per sweep,
  * which cannot keep the chip busy between launches; bsg_batch_set_launch_mode
- * overrides the choice. Members must use the Z-free form (xtz_mode 2, z_store 2), have no WM
- * image and share the latent method and sweep index; each keeps its own
- * device state and lane split, and must outlive the batch. */
+ * overrides the choice. Members must use the Z-free form (xtz_mode 2, z_store 2)
+ * or the precision form (xtz_mode 1, z_store 0, latent_method 0: Z stored,
+ * X'Z on MFMA; a batch with such a member runs one launch per phase and
+ * covariate count), have no WM image and share the latent method and sweep
+ * index; each keeps its own device state and lane split, and must outlive the
+ * batch. */
 int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out);
 int bsg_batch_destroy(bsg_batch *batch);
 /* nsweeps sweeps of every member; returns when they are done. Waits for work

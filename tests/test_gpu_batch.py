@@ -65,6 +65,55 @@ def test_batch_equals_members_run_alone(mode):
     B.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_precision_form_equals_members_run_alone(mixed):
+    """Members in the precision form (Z stored by the k_latent body, X'Z on
+    the matrix cores with BF16 hi+lo by the k_xtz_mfma body): the batch runs
+    each member's own blocks, so every member's coefficients, X'Z, CAR
+    precision and accumulators equal its sweeps alone bit for bit; mixed with
+    Z-free members in one batch as well (one launch per phase and P)."""
+    print(f"case precision-form batch (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3, c5_1 N=30 P=8), "
+          f"mixed={mixed}", flush=True)
+    ms = []
+    for (name, n, p, chains) in (("c1", 50, 3, 2), ("c1", 7, 2, 1), ("c5_0", 120, 5, 3), ("c5_1", 30, 8, 1)):
+        d = synth.generate(name, nsubs=n, ncovar=p)
+        ms.append((d, Layout(d["mask"], *d["grid"]), chains))
+
+    def mk():
+        out = []
+        for k, (d, L, ch) in enumerate(ms):
+            zfree = mixed and k % 2 == 1
+            out.append(Sampler(L, d["covar"], d["data"], seed=SEED + k, nchains=ch, xtz_mode=2 if zfree else 1,
+                               z_store=2 if zfree else 0))
+        return out
+
+    solo, grouped = mk(), mk()
+    B = Batch(grouped)
+    assert not B.one_launch_per_phase
+    for S in solo:
+        S.sweep(5)
+    B.sweep(5)
+    for S in solo:
+        S.accum_reset(ntypes=1, nbatches=3, batch_len=2)
+        S.sweep_accumulate(6, 1)
+    for S in grouped:
+        S.accum_reset(ntypes=1, nbatches=3, batch_len=2)
+    B.sweep_accumulate(6, 1)
+    for S, T in zip(solo, grouped):
+        S.synchronize()
+        T.synchronize()
+        assert S.sweep_index() == T.sweep_index() == 11
+        assert T.fallback_count() == 0
+        for c in range(S.nchains):
+            assert np.array_equal(S.get_spat_coef(c), T.get_spat_coef(c))
+            assert np.array_equal(S.get_xtz(c), T.get_xtz(c))
+            assert np.array_equal(S.get_prec(c), T.get_prec(c))
+            a, b = S.accum_get(c), T.accum_get(c)
+            for k in ("mean", "var", "prb", "mcse"):
+                assert np.array_equal(a[k], b[k]), k
+    B.close()
+
+
 @pytest.mark.parametrize("name,nchains", [("c5_3", 2), ("c1", 1)])
 def test_autotune_keeps_the_chain_state(name, nchains):
     """The autotuner times whole sweeps at every lane split from one saved

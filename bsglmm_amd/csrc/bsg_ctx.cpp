@@ -253,12 +253,17 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     set_error("bsg_create: nchains must be in 1..4096");
     return BSG_ERR_ARG;
   }
-  if (o.xtz_mode < 0 || o.xtz_mode > 2) {
-    set_error("bsg_create: xtz_mode must be 0, 1 or 2");
+  if (o.xtz_mode < 0 || o.xtz_mode > 3) {
+    set_error("bsg_create: xtz_mode must be 0, 1, 2 or 3");
     return BSG_ERR_ARG;
   }
-  if (o.z_store < 0 || o.z_store > 2 || (o.z_store == 2 && o.xtz_mode != 2)) {
-    set_error("bsg_create: z_store must be 0, 1, or 2 with xtz_mode 2");
+  if (o.z_store < 0 || o.z_store > 2 || (o.z_store == 2) != (o.xtz_mode >= 2)) {
+    set_error("bsg_create: z_store must be 0 or 1 with xtz_mode 0 or 1, and 2 with xtz_mode 2 or 3");
+    return BSG_ERR_ARG;
+  }
+  if (o.xtz_mode == 3 && (o.latent_method != 0 || ncovar > bsg::MAX_P || wm)) {
+    set_error("bsg_create: xtz_mode 3 (BF16 hi+lo X'Z on the matrix cores, fused into the draws) needs "
+              "latent_method 0, at most " + std::to_string(bsg::MAX_P) + " covariates and no WM image");
     return BSG_ERR_ARG;
   }
   if (o.latent_method < 0 || o.latent_method > 1 || (o.latent_method == 1 && o.z_store != 2)) {
@@ -352,6 +357,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   pb.vb = 0;
   pb.ve = V;
   pb.zfree = o.z_store == 2 ? 1 : 0;
+  pb.xbx = o.xtz_mode == 3 ? 1 : 0;
   // b is drawn each sweep only when a WM image is given (mcmc.cpp:344-345, UPDATE_WM)
   pb.wm_update = wm ? 1 : 0;
   pb.latent_method = o.latent_method;
@@ -510,7 +516,7 @@ int bsg_destroy(bsg_ctx *ctx)
 int bsg_update_latent(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode == 2, ctx->opts.z_store, ctx->sweep, ctx->stream, 0,
+  HIPCHK(bsg::launch_latent(ctx->pb, ctx->opts.xtz_mode >= 2, ctx->opts.z_store, ctx->sweep, ctx->stream, 0,
                             ctx->L.totvox, &ctx->bstat_parts, 0, 0, &ctx->tail_pending));
   return BSG_OK;
 }
@@ -677,7 +683,7 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
                          bool &wish_pending, bool &w_inline, const bsg::Accum *ac = nullptr, int batch = 0)
 {
   hipStream_t sa = ctx->stream, sb = ctx->stream2;
-  const bool fused = ctx->opts.xtz_mode == 2;
+  const bool fused = ctx->opts.xtz_mode >= 2;
   const int V = ctx->L.totvox;
   if (lat_ev2) HIPCHK(hipEventRecord(lat_ev2[0], sa));
   int parts = 0, pieces = 0;
@@ -716,7 +722,7 @@ static int enqueue_sweep(bsg_ctx *ctx, const bsg::DevProblem &pb, uint32_t off, 
 // the Wishart draw small enough to carry (any number of chains).
 static bool use_overlap(const bsg_ctx *ctx)
 {
-  return ctx->overlap && ctx->opts.z_store == 2 && ctx->opts.xtz_mode == 2 && bsg::overlap_supported(ctx->pb);
+  return ctx->overlap && ctx->opts.z_store == 2 && bsg::overlap_supported(ctx->pb);
 }
 
 // k full sweeps of problem pb, Philox words *pb.sweep_base + off0 + i, in the
@@ -1427,7 +1433,7 @@ int bsg_sweep_profiled(bsg_ctx *ctx, int nsweeps, double kernel_ms[6])
   for (int k = 0; k < 6; k++) kernel_ms[k] = 0.0;
   hipStream_t st = ctx->stream;
   for (int i = 0; i < nsweeps; i++) {
-    const bool fused = ctx->opts.xtz_mode == 2;
+    const bool fused = ctx->opts.xtz_mode >= 2;
     const uint32_t sw = ctx->sweep;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     int parts = 0, pieces = 0;
@@ -2052,6 +2058,10 @@ int bsg_set_lane_split(bsg_ctx *ctx, int g)
     set_error("bsg_set_lane_split: G must be 0, 1, 2, 4 or 8");
     return BSG_ERR_ARG;
   }
+  if (ctx->pb.xbx && g != 0 && g != 4) {
+    set_error("bsg_set_lane_split: xtz_mode 3 forms X'Z on the matrix cores at G = 4 only");
+    return BSG_ERR_ARG;
+  }
   if (ctx->pb.lat_g != g) drop_graph(ctx);
   ctx->pb.lat_g = g;
   return BSG_OK;
@@ -2187,6 +2197,10 @@ int bsg_autotune(bsg_ctx *ctx, int reps, int *best_g, double *ms_per_g)
   for (int k = 0; k < 4 && rc == BSG_OK; k++) {
     const int G = cand[k];
     double ms = -1.0;
+    if (pb.xbx && G != 4) {  // xtz_mode 3 runs at G = 4 only
+      if (ms_per_g) ms_per_g[k] = ms;
+      continue;
+    }
     {
       pb.lat_g = G;
       float f = 0.0f;

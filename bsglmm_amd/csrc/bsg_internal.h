@@ -87,6 +87,7 @@ struct DevProblem {
   float *mpart;            // [chain][colour 0 pieces | colour 1 pieces][64][P] partial X'Z of the pieces
   int mp_stride;           // floats of mpart per chain
   int zfree;               // Z not kept, X'Z fused into the latent kernel (z_store 2)
+  int xbx;                 // the fused X'Z as BF16 hi + lo products on the matrix cores (xtz_mode 3, G = 4)
   // Lambda of the overlapped sweep: (1 << 32 | float bits) per entry, written
   // by the block that draws it, read by the colour-0 blocks of the same
   // launch once their entries show the tag (no fence needed: each entry

@@ -730,8 +730,39 @@ __device__ __forceinline__ void latent_vl_body(const DevProblem &pb, int G, uint
 // and would cost more MFMA cycles than the FMAs it replaces). X is staged for
 // 4R quads, zero past NQ, so every lane reads in bounds in every round; the
 // means differ from the FMA chain of latent_vl_body only in FP32 rounding.
+// BX (X'Z as BF16 hi + lo products on the matrix cores): per round one
+// v_mfma_f32_16x16x16_bf16 pair forms D += (Z_hi + Z_lo) [X_hi | X_lo], with
+// A = Z (rows: the wave's 16 voxels; K: the round's 16 subjects, lane (j, g)
+// holding subjects 4g..4g+3 of its quad as it drew them) and B = the BF16 rows
+// of X (pb.xbf: column c < P is X_hi of covariate c, P + c its X_lo; lane
+// (c, g) loads the same four subjects). A and B use the same (lane, slot) -> k
+// map, so the contraction holds whatever order the hardware sums k in. D
+// lands as lane (c, g) -> D[4g + i][c], and m_j,c = D[j][c] + D[j][P + c].
+// Z never leaves registers; the four cross products of the hi and lo parts
+// are all formed, so the error is that of the splits (|v - hi - lo| <=
+// 2^-18 |v| per operand) plus FP32 accumulation.
 // ---------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// two floats to packed BF16 (round to nearest even), low half first
+__device__ __forceinline__ uint32_t bf16_pk(float a, float b)
+{
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// hi = bf16(v), lo = bf16(v - hi) of four floats, packed as MFMA operands
+__device__ __forceinline__ void split4_bf16(const float (&v)[4], s16x4 &hi, s16x4 &lo)
+{
+  const uint32_t h0 = bf16_pk(v[0], v[1]), h1 = bf16_pk(v[2], v[3]);
+  const uint32_t l0 = bf16_pk(v[0] - __uint_as_float(h0 << 16), v[1] - __uint_as_float(h0 & 0xffff0000u));
+  const uint32_t l1 = bf16_pk(v[2] - __uint_as_float(h1 << 16), v[3] - __uint_as_float(h1 & 0xffff0000u));
+  hi = __builtin_bit_cast(s16x4, make_uint2(h0, h1));
+  lo = __builtin_bit_cast(s16x4, make_uint2(l0, l1));
+}
 
 // ---------------------------------------------------------------------------
 // Tagged words of the persistent sweep (PTags). Writers store (tag << 32 |
@@ -867,7 +898,7 @@ __device__ __forceinline__ void stage_x_quads(const DevProblem &pb, float4 *xs)
 // from blocks of different XCDs in one launch reach memory in no fixed
 // order, since each XCD's L2 writes back its own lines: so the plain arrays
 // take one store per launch, the last sweep's.)
-template <int P, bool WMU, bool TG = false>
+template <int P, bool WMU, bool TG = false, bool BX = false>
 __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_t sweep, int chain, int vb, int ve,
                                                    int t0, int tstep, int slot, int ra = 0, int rb = -1,
                                                    float *part = nullptr, const PTags *tg = nullptr, int pk = 0,
@@ -875,6 +906,7 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
 {
   static_assert(P <= 8, "means take at most two MFMAs");
   static_assert(!(TG && WMU), "the persistent sweep keeps b fixed");
+  static_assert(!(BX && (TG || WMU)), "the BF16 X'Z form keeps b fixed and runs outside the persistent sweep");
   constexpr bool MFM = P >= 3;
   extern __shared__ float4 xs[];  // [4R][P]
   const int R = (pb.NQ + 3) >> 2;
@@ -897,6 +929,10 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
   const int q0 = g * R;
   const int tiles = (ve - vb + 63) / 64;
   double bpart = 0.0;
+  // BX: this lane's B column of the X'Z product (X_hi of covariate vl, or
+  // X_lo of covariate vl - P; none past 2P)
+  const bool xon = vl < 2 * P;
+  const uint16_t *xrow = BX ? pb.xbf + (size_t)(xon ? vl : 0) * pb.NSB : nullptr;
   for (int t = t0; t < tiles; t += tstep) {
     const int s = vb + (t * 4 + (threadIdx.x >> 6)) * 16 + vl;
     const bool valid = s < ve;
@@ -923,6 +959,7 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
     float acc[P], zs = 0.0f;
 #pragma unroll
     for (int c = 0; c < P; c++) acc[c] = 0.0f;
+    f32x4 dx = {0.0f, 0.0f, 0.0f, 0.0f};  // BX: D[4g + i][vl] of this tile
     const int rend = rb < 0 ? R : rb;
     int am = ra * 4 * P;  // step of the means' A offsets
     for (int r = ra; r < rend; r++, am += 4 * P) {
@@ -951,6 +988,13 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
       }
       const int qy = min(q, pb.NQ - 1);
       const uint32_t yw = yrow[qy >> 3] >> ((qy & 7) * 4);
+      // BX: B of this round (X_hi or X_lo of four subjects; zero past NQ and
+      // past column 2P), loaded from an address clamped into the row
+      uint2 xw = make_uint2(0u, 0u);
+      if constexpr (BX) {
+        xw = *reinterpret_cast<const uint2 *>(xrow + 4 * qy);
+        if (!xon || q >= pb.NQ) xw = make_uint2(0u, 0u);
+      }
       uint32_t rr[4];
       philox10(cbase + (uint32_t)q, sweep, TAG_LATENT, (uint32_t)chain, pb.key0, pb.key1, rr);
       float z[4];
@@ -959,18 +1003,41 @@ __device__ __forceinline__ void latent_vl_mfm_body(const DevProblem &pb, uint32_
 #pragma unroll
         for (int l = 0; l < 4; l++) z[l] = (4 * q + l < pb.N) ? z[l] : 0.0f;
       }
-      if (MFM) {
+      if constexpr (BX) {
+        s16x4 zh, zl;
+        split4_bf16(z, zh, zl);
+        const s16x4 xb = __builtin_bit_cast(s16x4, xw);
+        dx = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(zh, xb, dx, 0, 0, 0);
+        dx = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(zl, xb, dx, 0, 0, 0);
+      } else {
+        if (MFM) {
 #pragma unroll
-        for (int c = 0; c < P; c++) x[c] = xs[q * P + c];
-      }
+          for (int c = 0; c < P; c++) x[c] = xs[q * P + c];
+        }
 #pragma unroll
-      for (int c = 0; c < P; c++) {
-        acc[c] = fmaf(x[c].x, z[0], acc[c]);
-        acc[c] = fmaf(x[c].y, z[1], acc[c]);
-        acc[c] = fmaf(x[c].z, z[2], acc[c]);
-        acc[c] = fmaf(x[c].w, z[3], acc[c]);
+        for (int c = 0; c < P; c++) {
+          acc[c] = fmaf(x[c].x, z[0], acc[c]);
+          acc[c] = fmaf(x[c].y, z[1], acc[c]);
+          acc[c] = fmaf(x[c].z, z[2], acc[c]);
+          acc[c] = fmaf(x[c].w, z[3], acc[c]);
+        }
       }
       if (WMU) zs += (z[0] + z[1]) + (z[2] + z[3]);
+    }
+    if constexpr (BX) {
+      // m of voxel 4g + i, covariate vl: D[.][vl] + D[.][vl + P] (2P <= 16:
+      // the partner lane is in the same row of 16)
+      const int wv = threadIdx.x >> 6;
+      const int sw = vb + (t * 4 + wv) * 16 + 4 * g;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float mv = dx[i] + __shfl_down(dx[i], P, 16);
+        if (vl < P) {
+          if (part) part[(wv * 16 + 4 * g + i) * P + vl] = mv;
+          else if (sw + i < ve) pb.M[((size_t)chain * pb.V + sw + i) * P + vl] = mv;
+        }
+      }
+      continue;
     }
 #pragma unroll
     for (int off = 16; off < 64; off <<= 1) {
@@ -1040,15 +1107,15 @@ __device__ void wishart_block(const DevProblem &pb, int chain, int nparts, const
 // counters); only the order of the X'Z sum differs. Tiles and pieces depend
 // on the problem alone, so the whole-range launch, the one-colour launches of
 // the overlapped sweep and a batch all form the same sums.
-template <int P>
+template <int P, bool BX = false>
 __device__ __forceinline__ void vl_ctile_block(const DevProblem &pb, uint32_t sweep, int chain, int b, int cs)
 {
   const VlRange r0 = vl_range(pb, 0), r1 = vl_range(pb, 1);
   const int m0 = (cs & 1) ? r0.nmain : 0, m1 = (cs & 2) ? r1.nmain : 0;
   if (b < m0 + m1) {
     const bool c0 = b < m0;
-    latent_vl_mfm_body<P, false>(pb, sweep, chain, c0 ? r0.base : r1.base, c0 ? r0.end : r1.end, c0 ? b : b - m0,
-                                 1 << 30, 0);
+    latent_vl_mfm_body<P, false, false, BX>(pb, sweep, chain, c0 ? r0.base : r1.base, c0 ? r0.end : r1.end,
+                                            c0 ? b : b - m0, 1 << 30, 0);
     return;
   }
   b -= m0 + m1;
@@ -1057,9 +1124,9 @@ __device__ __forceinline__ void vl_ctile_block(const DevProblem &pb, uint32_t sw
   if (b >= p0) b -= p0;
   const int tl = b / r.K, k = b - tl * r.K;
   const int R = (pb.NQ + 3) >> 2;
-  latent_vl_mfm_body<P, false>(pb, sweep, chain, r.base, r.end, r.nmain + tl, 1 << 30, 0, k * R / r.K,
-                               (k + 1) * R / r.K,
-                               pb.mpart + (size_t)chain * pb.mp_stride + r.moff + ((size_t)tl * r.K + k) * 64 * P);
+  latent_vl_mfm_body<P, false, false, BX>(pb, sweep, chain, r.base, r.end, r.nmain + tl, 1 << 30, 0, k * R / r.K,
+                                          (k + 1) * R / r.K,
+                                          pb.mpart + (size_t)chain * pb.mp_stride + r.moff + ((size_t)tl * r.K + k) * 64 * P);
 }
 
 // m_j of sorted voxel s when it lies in a leftover tile of its colour: its K
@@ -1126,7 +1193,7 @@ __device__ __forceinline__ bool vl_tail_sum_tg(const DevProblem &pb, const PTags
 // precision of sweep wsweep (the sweep before: its colour-1 partials are
 // complete, and nothing here reads Lambda) beside the latent blocks, instead
 // of a launch of its own.
-template <int P, bool WMU>
+template <int P, bool WMU, bool BX = false>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_vl_mfm(DevProblem pb, uint32_t sweep, int wish,
                                                                        uint32_t wsweep, int cs)
 {
@@ -1140,8 +1207,8 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_vl_mfm(DevProble
     return;
   }
   const int b = blockIdx.x - wish;
-  if (!WMU && cs) vl_ctile_block<P>(pb, sweep + base, chain, b, cs);
-  else latent_vl_mfm_body<P, WMU>(pb, sweep + base, chain, pb.vb, pb.ve, b, gridDim.x - wish, b);
+  if (!WMU && cs) vl_ctile_block<P, BX>(pb, sweep + base, chain, b, cs);
+  else latent_vl_mfm_body<P, WMU, false, BX>(pb, sweep + base, chain, pb.vb, pb.ve, b, gridDim.x - wish, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -2183,7 +2250,7 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
 // ---------------------------------------------------------------------------
 // MC: several chains (whole tiles); else one chain (colour tiles and pieces),
 // compiled apart so that each keeps its own registers
-template <int P, bool MC>
+template <int P, bool MC, bool BX = false>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, int phase, uint32_t sweep, int nlat,
                                                                int wish, uint32_t wsweep, int tail, Accum ac,
                                                                int batch)
@@ -2204,10 +2271,11 @@ __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, 
     // one chain: colour tiles and pieces; several: one whole tile per block
     const int c = phase ? 0 : 1;
     if (!MC) {
-      vl_ctile_block<P>(pb, sweep + base + (uint32_t)phase, chain, b, c ? 2 : 1);
+      vl_ctile_block<P, BX>(pb, sweep + base + (uint32_t)phase, chain, b, c ? 2 : 1);
     } else {
       const VlRange r = vl_range(pb, c);
-      latent_vl_mfm_body<P, false>(pb, sweep + base + (uint32_t)phase, chain, r.base, r.end, b, 1 << 30, 0);
+      latent_vl_mfm_body<P, false, false, BX>(pb, sweep + base + (uint32_t)phase, chain, r.base, r.end, b, 1 << 30,
+                                              0);
     }
     return;
   }
@@ -3319,6 +3387,7 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
   const bool wmu = pb.wm_update != 0, bmf = pb.latent_method == 1;
   if (G != 1 && G != 2 && G != 4 && G != 8) return hipErrorInvalidValue;
   if (wish && (G != 4 || bmf)) return hipErrorInvalidValue;
+  if (pb.xbx && (G != 4 || bmf || wmu)) return hipErrorInvalidValue;  // the BF16 X'Z form is the G = 4 body's
   // colour tiles (and pieces) of the inversion path at G = 4
   const int cs = ctile_set(pb, G);
   if (cs) nlb = vl_blocks(pb, cs);
@@ -3331,6 +3400,8 @@ static hipError_t launch_latent_vl(const DevProblem &pb, int G, uint32_t sweep, 
       else hipLaunchKernelGGL((k_latent_vl<PP, false, true>), grid, dim3(256), lds, st, pb, G, sweep);      \
     } else if (G == 4) {                                                                                   \
       if (wmu) hipLaunchKernelGGL((k_latent_vl_mfm<PP, true>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, cs); \
+      else if (pb.xbx) hipLaunchKernelGGL((k_latent_vl_mfm<PP, false, true>), grid, dim3(256), lds, st, pb, sweep,    \
+                                          wish, wsweep, cs);                                                           \
       else hipLaunchKernelGGL((k_latent_vl_mfm<PP, false>), grid, dim3(256), lds, st, pb, sweep, wish, wsweep, cs);    \
     } else {                                                                                               \
       if (wmu) hipLaunchKernelGGL((k_latent_vl<PP, true, false>), grid, dim3(256), lds, st, pb, G, sweep);  \
@@ -3514,9 +3585,15 @@ hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int l
   dim3 grid((unsigned)(wish + nlat + ncb + (phase ? 1 : 0)), (unsigned)pb.nchains);
 #define L_(PP)                                                                                                     \
   do {                                                                                                             \
-    if (vl_ctiles(pb))                                                                                             \
+    if (vl_ctiles(pb) && pb.xbx)                                                                                   \
+      hipLaunchKernelGGL((k_overlap<PP, false, true>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish,     \
+                         wsweep, tail, acc, batch);                                                                \
+    else if (vl_ctiles(pb))                                                                                        \
       hipLaunchKernelGGL((k_overlap<PP, false>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep,   \
                          tail, acc, batch);                                                                        \
+    else if (pb.xbx)                                                                                               \
+      hipLaunchKernelGGL((k_overlap<PP, true, true>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish,      \
+                         wsweep, tail, acc, batch);                                                                \
     else                                                                                                           \
       hipLaunchKernelGGL((k_overlap<PP, true>), grid, dim3(256), lds, st, pb, phase, sweep, nlat, wish, wsweep,    \
                          tail, acc, batch);                                                                        \
@@ -3528,7 +3605,7 @@ hipError_t launch_overlap(const DevProblem &pb, int phase, uint32_t sweep, int l
 
 bool persist_supported(const DevProblem &pb)
 {
-  return overlap_supported(pb) && pb.P <= 6 && pb.nchains == 1 && vl_ctiles(pb);
+  return overlap_supported(pb) && pb.P <= 6 && pb.nchains == 1 && vl_ctiles(pb) && !pb.xbx;
 }
 
 int persist_max_blocks(const DevProblem &pb)

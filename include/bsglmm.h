@@ -68,12 +68,15 @@ typedef struct {
   uint64_t seed;        /* Philox key; the CLI derives it from seed.dat as main.cu:413 does */
   int nchains;          /* independent chains over the same data (>= 1) */
   int xtz_mode;         /* X'Z: 0 FP32 FMA kernel reading Z; 1 BF16 hi+lo split on MFMA,
-                           FP32 accumulate; 2 FP32 FMA fused into the latent kernel */
+                           FP32 accumulate; 2 FP32 FMA fused into the latent kernel;
+                           3 BF16 hi+lo split on MFMA fused into the latent kernel (the
+                           precision form without Z in HBM: z_store 2, latent_method 0,
+                           P <= 8, no WM image, four lanes per voxel) */
   int df_offset;        /* Wishart df = TOTVOX + df_offset: 19 as covarGPU.cu:1104, P as covar.cpp:291 */
   float trunc_eps;      /* latent truncation: 1e-6 as covarGPU.cu:1414-1418, 0 as covar.cpp:317,320 */
   float pivot_min;      /* FP32 Cholesky pivot below this -> FP64 refactor of that voxel */
   int z_store;          /* Z in HBM: 0 stored, 1 stored with streaming (nontemporal) stores,
-                           2 not kept (xtz_mode 2 only; the probit sweep needs Z only for X'Z) */
+                           2 not kept (xtz_mode 2 and 3; the probit sweep needs Z only for X'Z) */
   int latent_method;    /* z_store 2 only: 0 every draw by inversion (the draws of z_store 0);
                            1 Box-Muller candidate kept inside the interval of y, exact
                            inversion redraw otherwise (an exact sampler with other draws) */
@@ -106,7 +109,7 @@ int bsg_sweep_graph_prepare(bsg_ctx *ctx, int sweeps_per_graph);
 /* updateZGPU (covarGPU.cu:1625-1669) */
 int bsg_update_latent(bsg_ctx *ctx);
 /* X'Z, the contraction fused in Spat_Coef_Probit (covarGPU.cu:327-352),
- * recomputed from Z (xtz_mode 2 already forms it inside bsg_update_latent).
+ * recomputed from Z (xtz_mode 2 and 3 already form it inside bsg_update_latent).
  * The b WM_j X'1 term of X'(Z - b WM) is applied by bsg_update_coef with the
  * b of the sweep. */
 int bsg_compute_xtz(bsg_ctx *ctx);

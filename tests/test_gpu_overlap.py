@@ -25,7 +25,7 @@ def spat_coef_from_ranks(beta, lay):
     return sc
 
 
-def pair(name, nsubs, ncovar, g=0, nchains=1, wm=False):
+def pair(name, nsubs, ncovar, g=0, nchains=1, wm=False, xtz_mode=2):
     kw = {} if name in ("c2", "c3") else dict(nsubs=nsubs, ncovar=ncovar)
     d = synth.generate(name, **kw)
     nx, ny, nz = d["grid"]
@@ -36,7 +36,7 @@ def pair(name, nsubs, ncovar, g=0, nchains=1, wm=False):
         w = (0.5 + np.random.default_rng(3).random(d["data"].shape[1])).astype(np.float32)
     out = []
     for on in (True, False):
-        S = Sampler(L, d["covar"], d["data"], wm=w, seed=SEED, nchains=nchains, xtz_mode=2, z_store=2)
+        S = Sampler(L, d["covar"], d["data"], wm=w, seed=SEED, nchains=nchains, xtz_mode=xtz_mode, z_store=2)
         if g:
             S.lane_split = g
         S.overlap = on
@@ -79,6 +79,23 @@ def test_overlapped_sweep_equals_serial(name, nsubs, ncovar, g):
     B.sweep(5)
     assert_same_state(A, B)
     assert A.fallback_count() == B.fallback_count()
+
+
+# the X'Z of the draws as BF16 hi + lo products on the matrix cores (xtz_mode 3):
+# one chain (colour tiles, pieces or none) and several (whole tiles), C3 at
+# full size
+@pytest.mark.parametrize("name,nsubs,ncovar,nchains", [("c1", 300, 4, 1), ("c1", 57, 5, 1), ("c1", 13, 3, 1),
+                                                       ("c1", 90, 2, 1), ("c1", 200, 8, 1), ("c1", 300, 4, 2),
+                                                       ("c1", 150, 7, 3), ("c3", None, None, 1)])
+def test_bf16_fused_overlapped_sweep_equals_serial(name, nsubs, ncovar, nchains):
+    print(f"case xtz_mode 3 overlap vs serial {name} N={nsubs} P={ncovar} chains={nchains}", flush=True)
+    L, A, B = pair(name, nsubs, ncovar, nchains=nchains, xtz_mode=3)
+    A.overlap = True
+    assert A.overlap and not B.overlap
+    A.sweep(2)
+    A.sweep(3)
+    B.sweep(5)
+    assert_same_state(A, B, nchains)
 
 
 def test_overlapped_accumulation_and_graph_equal_serial():

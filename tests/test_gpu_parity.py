@@ -170,6 +170,34 @@ def test_z_free_every_lane_split(nsubs, ncovar, g, nchains):
     check_z_free(nsubs, ncovar, nchains, 0, g)
 
 
+# X'Z as BF16 hi + lo products on the matrix cores inside the latent draws
+# (xtz_mode 3, G = 4): P = 1..8, leftover tiles in pieces (N >= 57) or none
+# (N = 13), several chains (whole tiles), one lane column per covariate
+@pytest.mark.parametrize("nsubs,ncovar,nchains", [(300, 4, 1), (200, 8, 1), (200, 8, 2), (57, 5, 1), (13, 3, 2),
+                                                  (90, 2, 1), (150, 1, 1), (150, 7, 1), (37, 6, 1)])
+def test_bf16_mfma_xtz_fused_into_the_draws(nsubs, ncovar, nchains):
+    check_z_free(nsubs, ncovar, nchains, 0, 0, xtz_mode=3)
+
+
+def test_bf16_mfma_fused_form_limits():
+    """xtz_mode 3 needs the Z-free form at four lanes per voxel with
+    inversion draws and no WM image: other settings are refused."""
+    d = synth.generate("c1", nsubs=60, ncovar=4)
+    L = Layout(d["mask"], *d["grid"])
+    for kw in (dict(z_store=0), dict(z_store=1), dict(z_store=2, latent_method=1)):
+        with pytest.raises(Exception):
+            Sampler(L, d["covar"], d["data"], seed=1, xtz_mode=3, **kw)
+    with pytest.raises(Exception):
+        Sampler(L, d["covar"], d["data"], wm=np.ones(L.totvox, np.float32), seed=1, xtz_mode=3, z_store=2)
+    S = Sampler(L, d["covar"], d["data"], seed=1, xtz_mode=3, z_store=2)
+    assert S.lane_split == 4
+    with pytest.raises(Exception):
+        S.lane_split = 8
+    S.lane_split = 4
+    g, ms = S.autotune(2)
+    assert g == 4 and [m for m in ms if m >= 0] == [ms[2]]
+
+
 def test_split_tail_single_steps_and_full_size():
     """Four lanes per voxel split a one-chain launch's leftover 64-voxel tiles
     into pieces whose partial X'Z sums the next coefficient kernel (or
@@ -230,13 +258,14 @@ def test_split_tail_single_steps_and_full_size():
     assert np.all(np.isfinite(A.get_spat_coef())) and np.all(np.isfinite(A.get_prec()))
 
 
-def check_z_free(nsubs, ncovar, nchains, method, g=0):
-    print(f"case c1 z_store=2 method={method} N={nsubs} P={ncovar} chains={nchains} G={g}", flush=True)
+def check_z_free(nsubs, ncovar, nchains, method, g=0, xtz_mode=2):
+    print(f"case c1 z_store=2 xtz_mode={xtz_mode} method={method} N={nsubs} P={ncovar} chains={nchains} G={g}",
+          flush=True)
     d = synth.generate("c1", nsubs=nsubs, ncovar=ncovar)
     nx, ny, nz = d["grid"]
     lay = oracle.layout(d["mask"], nx, ny, nz)
     L = Layout(d["mask"], nx, ny, nz)
-    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2,
+    S = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=xtz_mode, z_store=2,
                 latent_method=method)
     if g:
         S.lane_split = g
@@ -260,7 +289,8 @@ def check_z_free(nsubs, ncovar, nchains, method, g=0):
         z_orc = O.update_z(scs[c], sweep, method=method)
         m_gpu = S.get_xtz(c)
         scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_orc)
-        # the GPU draws are FP32 (1e-5 of the oracle's) and summed in FP32
+        # the GPU draws are FP32 (1e-5 of the oracle's) and summed in FP32 (xtz_mode 3: as BF16 hi + lo
+        # products, each operand within 2^-18 of its value)
         assert np.max(np.abs(m_gpu - O.xtz(z_orc.astype(np.float32))) / scale) < 2e-5
         ref = O.update_coef(0, m_gpu.astype(np.float64), prec0, scs[c].astype(np.float64), sweep)
         ref = O.update_coef(1, m_gpu.astype(np.float64), prec0, ref, sweep)

@@ -66,6 +66,17 @@ def test_precision_form_errors_against_fp64():
     for mode, name in ((1, "xtz_mfma_bf16x2"), (0, "xtz_fp32")):
         e = np.abs(S[mode].get_xtz() - m64) / scale
         report[name] = {"max_err_over_sum_abs": float(e.max()), "rms": float(np.sqrt(np.mean(e * e)))}
+    # the BF16 hi + lo MFMA product fused into the draws (xtz_mode 3, Z never
+    # stored): its own draws are those of the stored form up to the FP32
+    # rounding of the means (matrix-core means there, FMA here)
+    F = Sampler(L, d["covar"], d["data"], seed=SEED, xtz_mode=3, z_store=2)
+    F.set_spat_coef(sc0)
+    F.set_prec(prec0)
+    F.set_sweep_index(sweep)
+    F.update_latent()
+    e = np.abs(F.get_xtz() - m64) / scale
+    report["xtz_mfma_bf16x2_fused"] = {"max_err_over_sum_abs": float(e.max()), "rms": float(np.sqrt(np.mean(e * e)))}
+    F.close()
     del z, m64, scale
 
     # coefficients from the MFMA X'Z, against the FP64 oracle on the same M
@@ -100,6 +111,7 @@ def test_precision_form_errors_against_fp64():
     print("PRECISION " + json.dumps(report), flush=True)
 
     assert report["xtz_mfma_bf16x2"]["max_err_over_sum_abs"] < 2e-5
+    assert report["xtz_mfma_bf16x2_fused"]["max_err_over_sum_abs"] < 2e-5
     assert report["xtz_fp32"]["max_err_over_sum_abs"] < 1e-5
     assert report["beta"]["max_rel"] < 1e-5
     # FP64 sums of 675,036 pair terms in a different order: n u = 7.5e-11

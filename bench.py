@@ -251,7 +251,8 @@ def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, prof
 def config_line(name, forms, steps, warmup):
     """Sweeps/s of configuration `name` in each (label, xtz_mode, z_store,
     overlap, graph group) form: `steps` sweeps after `warmup`, timed by the
-    host clock around the stream (graph group > 0: bsg_sweep_graph)."""
+    host clock around the stream (graph group > 0: bsg_sweep_graph; overlap
+    None: the context's default)."""
     from bsglmm_amd import Layout, Sampler, synth
     d = synth.generate(name)
     L = Layout(d["mask"], *d["grid"])
@@ -260,7 +261,8 @@ def config_line(name, forms, steps, warmup):
     out = {"config": name, "V": V, "N": N, "P": P, "chains": 1, "timed_sweeps": steps, "forms": {}}
     for label, mode, zs, ov, grp in forms:
         S = Sampler(L, d["covar"], d["data"], seed=0x5EED0C0F, xtz_mode=mode, z_store=zs)
-        S.overlap = ov
+        if ov is not None:
+            S.overlap = ov
         S.sweep(warmup)
         S.synchronize()
         t0 = time.perf_counter()
@@ -275,7 +277,7 @@ def config_line(name, forms, steps, warmup):
              "sweeps_per_s": steps / wall, "us_per_sweep": 1e6 * wall / steps,
              "latent_draws_per_s": N * V * steps / wall,
              "latent_kernel_us": 1e3 * lat_ms / min(steps, 200)}
-        if mode != 2:
+        if mode < 2:
             r["kernel_us_serial_breakdown"] = {k: 1e3 * v / 20 for k, v in S.sweep_profiled(20).items()}
         out["forms"][label] = r
         S.close()
@@ -353,18 +355,21 @@ def batched_vs_plain(L, d, chains, iters, burnin):
     return out
 
 
-def uneven_batch(steps):
+def uneven_batch(steps, xtz_mode=3, extras=True):
     """Twelve independent analyses (ellipsoid masks of 20,031 to 398,905
-    voxels, N from 40 to 1,000, P from 2 to 8), 4 chains each. Each analysis
-    gets its lane split from the autotuner; then `steps` sweeps of the whole
-    batch (one launch per phase over every analysis and chain) are timed
-    against the same sweeps run analysis by analysis."""
+    voxels, N from 40 to 1,000, P from 2 to 8), 4 chains each, Z not kept and
+    X'Z formed inside the draws: xtz_mode 3, the configuration's BF16 hi + lo
+    split on the matrix cores (FP32 accumulation), or 2, FP32 FMA. Each
+    analysis gets its lane split from the autotuner (xtz_mode 3 runs at four
+    lanes per voxel); then `steps` sweeps of the whole batch are timed against
+    the same sweeps run analysis by analysis. With `extras`, the same twelve
+    analyses with FP32 X'Z and with Z stored, and a batch of small analyses."""
     from bsglmm_amd import Batch, Layout, Sampler, synth
     rows, samplers = [], []
     for k in range(12):
         d = synth.generate(f"c5_{k}")
         L = Layout(d["mask"], *d["grid"])
-        S = Sampler(L, d["covar"], d["data"], seed=0xC5C5 + k, nchains=4, xtz_mode=2, z_store=2)
+        S = Sampler(L, d["covar"], d["data"], seed=0xC5C5 + k, nchains=4, xtz_mode=xtz_mode, z_store=2)
         S.sweep(3)
         g, ms = S.autotune(reps=10)
         P, N = d["covar"].shape
@@ -386,18 +391,21 @@ def uneven_batch(steps):
     B.close()
     for S in samplers:
         S.close()
-    small = small_batch(max(steps, 100))
-    prec = precision_batch(max(steps // 2, 10))
-    return {"small_analyses_batch": small, "precision_form_batch": prec, "analyses": rows, "chains_per_analysis": 4,
+    out = {}
+    if extras:
+        out["fp32_xtz_batch"] = uneven_batch(steps, 2, False)
+        out["small_analyses_batch"] = small_batch(max(steps, 100))
+        out["precision_form_z_stored_batch"] = precision_batch(max(steps // 2, 10))
+    form = ("FP32 state, Z not kept; X'Z as BF16 hi + lo products on MFMA (v_mfma_f32_16x16x16_bf16) inside the "
+            "draws, FP32 accumulation" if xtz_mode == 3 else "FP32 state, Z not kept; X'Z on FP32 FMA inside the draws")
+    return {**out, "form": form, "xtz_mode": xtz_mode, "analyses": rows, "chains_per_analysis": 4,
             "latent_draws_per_batch_sweep": draws,
             "batch_sweeps_per_s": steps / bat, "batched_draws_per_s": draws * steps / bat,
             "one_by_one_draws_per_s": draws * steps / seq, "batched_over_one_by_one": seq / bat,
             "concurrent_streams_draws_per_s": draws * steps / conc, "batched_over_concurrent_streams": conc / bat,
             "sum_of_tuned_sweep_ms": sum(min(v for v in r["sweep_ms_by_G"].values() if v is not None)
                                          for r in rows),
-            "batch_sweep_ms": 1e3 * bat / steps, "graph_batch_sweep_ms": 1e3 * gra / steps,
-            "form": "FP32, Z-free fused X'Z (the BF16-split MFMA X'Z needs Z in HBM and is slower, "
-                    "see forms.separate_xtz_mfma_bf16x2)"}
+            "batch_sweep_ms": 1e3 * bat / steps, "graph_batch_sweep_ms": 1e3 * gra / steps}
 
 
 def precision_batch(steps):
@@ -635,8 +643,12 @@ def main():
         out["configs"] = {
             "c1": config_line("c1", [("z_free_overlapped", 2, 2, True, 0), ("z_free_overlapped_graph50", 2, 2, True, 50)],
                               a.c1_sweeps, 50),
-            "c3": config_line("c3", [("precision_mfma_bf16x2_xtz", 1, 0, False, 0),
-                                     ("z_free_fp32_fused_xtz", 2, 2, True, 0)], a.c3_sweeps, 10),
+            # C3's precision form with X'Z on the matrix cores: fused into the draws
+            # (Z never in HBM), then with Z stored and read back; the FP32 fused form
+            # beside them. Overlap is left at its default (off at P = 8)
+            "c3": config_line("c3", [("precision_mfma_bf16x2_fused_xtz", 3, 2, None, 0),
+                                     ("precision_mfma_bf16x2_xtz_z_stored", 1, 0, False, 0),
+                                     ("z_free_fp32_fused_xtz", 2, 2, None, 0)], a.c3_sweeps, 10),
         }
     if a.batched_chains > 0:
         out["batched"] = batched_vs_plain(L, d, a.batched_chains, a.batched_iters, a.batched_burnin)

@@ -257,13 +257,13 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
     set_error("bsg_create: xtz_mode must be 0, 1, 2 or 3");
     return BSG_ERR_ARG;
   }
-  if (o.z_store < 0 || o.z_store > 2 || (o.z_store == 2) != (o.xtz_mode >= 2)) {
-    set_error("bsg_create: z_store must be 0 or 1 with xtz_mode 0 or 1, and 2 with xtz_mode 2 or 3");
+  if (o.z_store < 0 || o.z_store > 2 || (o.z_store == 2 && o.xtz_mode < 2)) {
+    set_error("bsg_create: z_store must be 0, 1, or 2 with xtz_mode 2 or 3");
     return BSG_ERR_ARG;
   }
-  if (o.xtz_mode == 3 && (o.latent_method != 0 || ncovar > bsg::MAX_P || wm)) {
+  if (o.xtz_mode == 3 && (o.z_store != 2 || o.latent_method != 0 || ncovar > bsg::MAX_P || wm)) {
     set_error("bsg_create: xtz_mode 3 (BF16 hi+lo X'Z on the matrix cores, fused into the draws) needs "
-              "latent_method 0, at most " + std::to_string(bsg::MAX_P) + " covariates and no WM image");
+              "z_store 2, latent_method 0, at most " + std::to_string(bsg::MAX_P) + " covariates and no WM image");
     return BSG_ERR_ARG;
   }
   if (o.latent_method < 0 || o.latent_method > 1 || (o.latent_method == 1 && o.z_store != 2)) {
@@ -2319,11 +2319,11 @@ int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out)
       return BSG_ERR_ARG;
     }
     // the Z-free form, or the precision form (Z stored, X'Z on MFMA, inversion draws)
-    const bool zfree = c->opts.z_store == 2 && c->opts.xtz_mode == 2;
+    const bool zfree = c->opts.z_store == 2 && c->opts.xtz_mode >= 2;
     const bool zprec = c->opts.z_store == 0 && c->opts.xtz_mode == 1 && c->opts.latent_method == 0;
     if (!(zfree || zprec) || c->pb.wm_update || c->L.n[1] == 0) {
-      set_error("bsg_batch_create: members must use the Z-free form (xtz_mode 2, z_store 2) or the precision form "
-                "(xtz_mode 1, z_store 0, latent_method 0), have no WM image and two colours");
+      set_error("bsg_batch_create: members must use a Z-free form (xtz_mode 2 or 3, z_store 2) or the precision "
+                "form with Z stored (xtz_mode 1, z_store 0, latent_method 0), have no WM image and two colours");
       return BSG_ERR_ARG;
     }
     if (c->opts.latent_method != members[0]->opts.latent_method || c->sweep != members[0]->sweep) {

@@ -2633,15 +2633,22 @@ __global__ __launch_bounds__(256) void k_latent_multi(const DevProblem *pbs, con
 }
 
 // members drawn by inversion at G = 4: item blk is block blk of the member's
-// own whole-range launch (colour tiles and pieces included)
+// own whole-range launch (colour tiles and pieces included), with the
+// member's own X'Z form (FP32 FMA, or BF16 hi + lo on the matrix cores)
 template <int P>
 __global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_latent_multi_mfm(const DevProblem *pbs,
                                                                           const WorkItem *items, uint32_t sweep)
 {
   const WorkItem it = items[blockIdx.x];
   const DevProblem pb = pbs[it.a];
-  if (vl_ctiles(pb)) vl_ctile_block<P>(pb, sweep + *pb.sweep_base, it.chain, it.blk, 3);
-  else latent_vl_mfm_body<P, false>(pb, sweep + *pb.sweep_base, it.chain, 0, pb.V, it.blk, 1 << 30, 0);
+  const uint32_t sw = sweep + *pb.sweep_base;
+  if (pb.xbx) {
+    if (vl_ctiles(pb)) vl_ctile_block<P, true>(pb, sw, it.chain, it.blk, 3);
+    else latent_vl_mfm_body<P, false, false, true>(pb, sw, it.chain, 0, pb.V, it.blk, 1 << 30, 0);
+  } else {
+    if (vl_ctiles(pb)) vl_ctile_block<P>(pb, sw, it.chain, it.blk, 3);
+    else latent_vl_mfm_body<P, false>(pb, sw, it.chain, 0, pb.V, it.blk, 1 << 30, 0);
+  }
 }
 
 // members in the precision form (Z stored, X'Z on the matrix cores): item
@@ -2728,7 +2735,10 @@ __global__ __launch_bounds__(256) void k_latent_multi_all(const DevProblem *pbs,
   const bool g4 = !BMF && pb.lat_g == 4;
 #define B_(PP)                                                                                        \
   do {                                                                                                \
-    if (g4) {                                                                                         \
+    if (g4 && pb.xbx) {                                                                               \
+      if (vl_ctiles(pb)) vl_ctile_block<PP, true>(pb, sw, it.chain, it.blk, 3);                       \
+      else latent_vl_mfm_body<PP, false, false, true>(pb, sw, it.chain, 0, pb.V, it.blk, 1 << 30, 0); \
+    } else if (g4) {                                                                                  \
       if (vl_ctiles(pb)) vl_ctile_block<PP>(pb, sw, it.chain, it.blk, 3);                             \
       else latent_vl_mfm_body<PP, false>(pb, sw, it.chain, 0, pb.V, it.blk, 1 << 30, 0);              \
     } else {                                                                                          \

@@ -271,10 +271,11 @@ typedef struct bsg_batch bsg_batch;
  * P on its own stream so that the launches of different P overlap). The
  * first is chosen for batches of fewer than 2^28 latent draws per sweep,
  * which cannot keep the chip busy between launches; bsg_batch_set_launch_mode
- * overrides the choice. Members must use the Z-free form (xtz_mode 2, z_store 2)
- * or the precision form (xtz_mode 1, z_store 0, latent_method 0: Z stored,
- * X'Z on MFMA; a batch with such a member runs one launch per phase and
- * covariate count), have no WM image and share the latent method and sweep
+ * overrides the choice. Members must use a Z-free form (xtz_mode 2 or 3,
+ * z_store 2: X'Z on FP32 FMA or as BF16 hi+lo products on MFMA inside the
+ * draws) or the precision form with Z stored (xtz_mode 1, z_store 0,
+ * latent_method 0: X'Z on MFMA from Z in HBM; a batch with such a member runs
+ * one launch per phase and covariate count), have no WM image and share the latent method and sweep
  * index; each keeps its own device state and lane split, and must outlive the
  * batch. */
 int bsg_batch_create(int n, bsg_ctx *const *members, bsg_batch **out);

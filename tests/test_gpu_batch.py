@@ -25,20 +25,24 @@ def members(seed_shift=0):
     return out
 
 
-def make(ms):
-    return [Sampler(L, d["covar"], d["data"], seed=SEED + k, nchains=ch, xtz_mode=2, z_store=2)
+def make(ms, xtz=(2,)):
+    """Z-free samplers; member k forms X'Z in mode xtz[k % len(xtz)] (2: FP32
+    FMA, 3: BF16 hi + lo on the matrix cores)."""
+    return [Sampler(L, d["covar"], d["data"], seed=SEED + k, nchains=ch, xtz_mode=xtz[k % len(xtz)], z_store=2)
             for k, (d, L, ch) in enumerate(ms)]
 
 
+@pytest.mark.parametrize("xtz", [(2,), (3,), (3, 2)])
 @pytest.mark.parametrize("mode", [1, 2])
-def test_batch_equals_members_run_alone(mode):
+def test_batch_equals_members_run_alone(mode, xtz):
     """mode 1: one launch per phase over every covariate count (each block
     switches on its member's P); mode 2: one launch per phase and P, each P on
-    its own stream."""
-    print(f"case batch of 4 analyses (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3, c5_1 N=30 P=3), mode {mode}",
-          flush=True)
+    its own stream. Members form X'Z on FP32 FMAs, as BF16 hi + lo products on
+    the matrix cores, or both kinds in one batch."""
+    print(f"case batch of 4 analyses (c1 N=50 P=3 x2, c1 N=7 P=2, c5_0 N=120 P=5 x3, c5_1 N=30 P=3), mode {mode}, "
+          f"xtz_mode {xtz}", flush=True)
     ms = members()
-    solo, grouped = make(ms), make(ms)
+    solo, grouped = make(ms, xtz), make(ms, xtz)
     B = Batch(grouped)
     B.set_launch_mode(mode)
     assert B.one_launch_per_phase == (mode == 1)

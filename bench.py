@@ -211,7 +211,8 @@ def time_mode(L, d, mode, zstore, steps, warmup, overlap=True, settle=None, prof
     events bracket each sweep's two launches."""
     from bsglmm_amd import Sampler
     S = Sampler(L, d["covar"], d["data"], seed=0x9E3779B97F4A7C15, xtz_mode=mode, z_store=zstore)
-    S.overlap = overlap
+    if overlap is not None:
+        S.overlap = overlap
     ran_overlapped = S.overlap
     if persistent:
         S.persistent = True  # bsg_sweep: one k_persist launch per call (same chain)
@@ -553,8 +554,9 @@ def main():
     if not head["overlapped"]:
         raise RuntimeError("the headline workload must run overlapped")
     (copy_gbs, copy_ms), (fma_rate, _), (trans_rate, _), (sampler_dps, _) = head["settle"]
+    # overlap None: the context's default (on up to P = 6)
     forms = {"z_free_serial": (2, 2, False), "fused_xtz_z_stored": (2, 0, True), "separate_xtz_fp32": (0, 0, True),
-             "separate_xtz_mfma_bf16x2": (1, 0, True)}
+             "separate_xtz_mfma_bf16x2": (1, 0, True), "fused_xtz_mfma_bf16x2_z_free": (3, 2, None)}
     others = {k: time_mode(L, d, m, z, a.steps, a.warmup, o) for k, (m, z, o) in forms.items()}
 
     value = a.steps / head["wall"]
@@ -590,6 +592,7 @@ def main():
     def form_line(r, z_written, fused):
         lat_b = vl_bytes if not z_written else N * V * (4 + 1 / 8) + 4 * P * V + (4 * P * V if fused else 0)
         line = {"value": a.steps / r["wall"], "unit": "sweeps/s", "ms_per_step": 1e3 * r["wall"] / a.steps,
+                "overlapped": r["overlapped"],
                 "latent_kernel_ms": r["lat_ms"], "latent_gbs": lat_b / (r["lat_ms"] / 1e3) / 1e9,
                 "kernel_ms_serial_breakdown": r["kms"],
                 "effective_sweep_hbm_frac": sweep_bytes * (a.steps / r["wall"]) / 1e9 / HBM_PEAK_GBS}
@@ -636,7 +639,7 @@ def main():
                      "limiter": limiter,
                      "note": "the kernel is bound by VALU issue (limiter), not by HBM: its algorithmic and "
                              "counted bytes are a few percent of the HBM roofline"},
-        "forms": {k: form_line(r, forms[k][1] != 2, forms[k][0] == 2) for k, r in others.items()},
+        "forms": {k: form_line(r, forms[k][1] != 2, forms[k][0] >= 2) for k, r in others.items()},
         "cpu_baseline": None,
     }
     if a.config_lines:

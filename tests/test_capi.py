@@ -63,6 +63,14 @@ def test_create_rejects_bad_arguments():
         bsglmm_amd.Sampler(L, d["covar"], d["data"], xtz_mode=7)
     with pytest.raises(BsgError):
         bsglmm_amd.Sampler(L, d["covar"], d["data"], nchains=0)
+    # the BF16 MFMA X'Z inside the draws (xtz_mode 3) keeps no Z, draws by inversion, takes no WM image
+    for kw in (dict(z_store=0), dict(z_store=1), dict(z_store=2, latent_method=1)):
+        with pytest.raises(BsgError, match="xtz_mode 3"):
+            bsglmm_amd.Sampler(L, d["covar"], d["data"], xtz_mode=3, **kw)
+    with pytest.raises(BsgError, match="xtz_mode 3"):
+        bsglmm_amd.Sampler(L, d["covar"], d["data"], wm=np.ones(L.totvox, np.float32), xtz_mode=3, z_store=2)
+    with pytest.raises(BsgError, match="z_store"):
+        bsglmm_amd.Sampler(L, d["covar"], d["data"], xtz_mode=1, z_store=2)
 
 
 def test_bench_refuses_more_than_one_gpu():

@@ -36,9 +36,10 @@ struct bsg_ctx {
   // launch left in partial sums, until a coefficient kernel (or bsg_get_xtz)
   // adds them
   int tail_pending = 0;
-  // full sweeps run overlapped (k_overlap) where supported and asked for:
-  // by default up to P = 6 (at P = 7, 8 the colour updates inside the latent
-  // kernel's register budget spill, and the serial order is faster); set_overlap
+  // full sweeps run overlapped (k_overlap) where supported and asked for (by
+  // default: at P <= 6 it shares the draws' register budget, at P = 7, 8 it
+  // runs four blocks per compute unit so that the colour updates do not
+  // spill); set_overlap
   bool overlap = true;
   bool overlap_ran = false;  // flags[2] (a Lambda wait past its bound) to check
   bsg::DevProblem pb{};
@@ -290,7 +291,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   }
   ctx->L = layout->L;
   ctx->opts = o;
-  ctx->overlap = ncovar <= 6;
+  ctx->overlap = true;
   const bsg::Layout &L = ctx->L;
   const int V = L.totvox, N = nsubs, P = ncovar, C = o.nchains;
   ctx->N = N;

@@ -2249,9 +2249,13 @@ __global__ __launch_bounds__(256) void k_coef(DevProblem pb, int colour, uint32_
 // Every variate and sum is the one the serial sweep forms.
 // ---------------------------------------------------------------------------
 // MC: several chains (whole tiles); else one chain (colour tiles and pieces),
-// compiled apart so that each keeps its own registers
+// compiled apart so that each keeps its own registers. At P = 7, 8 the
+// colour updates beside the draws (the FP32 factor, the pair terms' 28 or 36
+// FP64 sums) need more than the draws' 64-72 registers: four blocks per
+// compute unit give them 128 and no spills (at C3 the X staging in LDS
+// already holds a compute unit to about four blocks)
 template <int P, bool MC, bool BX = false>
-__global__ __launch_bounds__(256, P <= 7 ? 8 : 7) void k_overlap(DevProblem pb, int phase, uint32_t sweep, int nlat,
+__global__ __launch_bounds__(256, P <= 6 ? 8 : 4) void k_overlap(DevProblem pb, int phase, uint32_t sweep, int nlat,
                                                                int wish, uint32_t wsweep, int tail, Accum ac,
                                                                int batch)
 {

@@ -236,10 +236,11 @@ int bsg_get_lane_split(bsg_ctx *ctx, int *g); /* the split the next sweep uses *
  * whose latent kernel runs at four lanes per voxel (inversion draws, no WM
  * image; any number of chains) run as two launches per sweep, each colour's
  * coefficient update beside the other colour's latent draws. Same variates,
- * same sums, same chains as with on = 0. On by default for P <= 6; at P = 7
- * and 8 it is supported but off by default (the colour updates then spill
- * under the latent kernel's register budget and the serial order is faster).
- * bsg_get_overlap tells whether the next bsg_sweep* call runs overlapped. */
+ * same sums, same chains as with on = 0. On by default wherever supported
+ * (P <= 8; at P = 7, 8 with fewer blocks per compute unit, and where the
+ * latent launch alone fills the chip, as at N = 1,000, P = 8, the serial
+ * order can be slightly faster). bsg_get_overlap tells whether the next
+ * bsg_sweep* call runs overlapped. */
 int bsg_set_overlap(bsg_ctx *ctx, int on);
 int bsg_get_overlap(bsg_ctx *ctx, int *on);
 /* Persistent sweeps (no counterpart in the reference, mcmc.cpp:311-394 runs

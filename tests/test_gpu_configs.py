@@ -8,10 +8,13 @@ oracle's own chain-31 draws (its own Philox stream).
 
 C5: the twelve uneven analyses c5_0 ... c5_11 (ellipsoid masks of 20,031 to
 398,905 voxels, N from 40 to 1,000, P from 2 to 8), 4 chains each, in one
-batch: each member's sweeps must equal the member swept alone, bit for bit.
+batch, with X'Z as BF16 hi + lo products on the matrix cores (the
+configuration's form) and in FP32: each member's sweeps must equal the member
+swept alone, bit for bit.
 
 Batched maps: 32 chains of the C1 data in one context, run with the reference
-CPU path's semantics; their pooled posterior means must agree with a long
+CPU path's semantics, with X'Z in FP32 and in the mixed-precision form (BF16
+hi + lo products on the matrix cores); their pooled posterior means must agree with a long
 chain of the reference's own CPU path (tests/golden/ref_chain_c1.npz) within
 4 combined Monte-Carlo standard errors, up to the exceedances a Student-t
 null allows (the MCSEs are batch-means estimates, so |z| > 4 is more likely
@@ -85,14 +88,18 @@ def test_c4_32_chains_overlapped_equals_serial():
     assert not np.array_equal(A.get_spat_coef(0), A.get_spat_coef(31))
 
 
-def test_c5_twelve_analysis_batch_equals_members_alone():
-    print("case C5: c5_0..c5_11, 4 chains each, 2 batch sweeps vs each member alone", flush=True)
+# xtz_mode 3 is the configuration's form (X'Z as BF16 hi + lo products on the
+# matrix cores, FP32 accumulation, here inside the draws); 2 the FP32 one
+@pytest.mark.parametrize("xtz_mode", [3, 2])
+def test_c5_twelve_analysis_batch_equals_members_alone(xtz_mode):
+    print(f"case C5: c5_0..c5_11, 4 chains each, xtz_mode {xtz_mode}, 2 batch sweeps vs each member alone",
+          flush=True)
     solo, grouped = [], []
     for k in range(12):
         d = synth.generate(f"c5_{k}")
         L = Layout(d["mask"], *d["grid"])
         for lst in (solo, grouped):
-            lst.append(Sampler(L, d["covar"], d["data"], seed=SEED + k, nchains=4, xtz_mode=2, z_store=2))
+            lst.append(Sampler(L, d["covar"], d["data"], seed=SEED + k, nchains=4, xtz_mode=xtz_mode, z_store=2))
         del d
     Ps = sorted({S.P for S in grouped})
     assert Ps == [2, 3, 4, 5, 6, 7, 8]
@@ -117,13 +124,17 @@ def test_c5_twelve_analysis_batch_equals_members_alone():
         S.close()
 
 
-def test_batched_chains_pooled_maps_match_reference_cpu_chain():
+# the batched chains in FP32 (xtz_mode 2) and in the mixed-precision form
+# (xtz_mode 3: X'Z as BF16 hi + lo products on the matrix cores)
+@pytest.mark.parametrize("xtz_mode", [2, 3])
+def test_batched_chains_pooled_maps_match_reference_cpu_chain(xtz_mode):
     f = np.load(os.path.join(GOLD, "ref_chain_c1.npz"))
     C, burnin, draws, nb = 32, 1000, 2000, 20
-    print(f"case batched maps: c1, {C} chains x ({burnin}+{draws}) sweeps vs the reference CPU chain", flush=True)
+    print(f"case batched maps: c1, {C} chains x ({burnin}+{draws}) sweeps, xtz_mode {xtz_mode}, vs the reference "
+          f"CPU chain", flush=True)
     d = synth.generate("c1")
     L = Layout(d["mask"], *d["grid"])
-    S = Sampler(L, d["covar"], d["data"], seed=0xBA7C4, nchains=C, xtz_mode=2, z_store=2,
+    S = Sampler(L, d["covar"], d["data"], seed=0xBA7C4, nchains=C, xtz_mode=xtz_mode, z_store=2,
                 df_offset=int(f["df_offset"]), trunc_eps=float(f["trunc_eps"]))
     assert S.overlap
     S.sweep(burnin)

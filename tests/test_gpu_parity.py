@@ -284,10 +284,22 @@ def check_z_free(nsubs, ncovar, nchains, method, g=0, xtz_mode=2):
         S.set_prec(prec0, chain=c)
     S.sweep(1)
     S.synchronize()
+    T = None
+    if xtz_mode == 3:
+        # the same sweep with FP32 X'Z: the draws are the same, the X'Z sums are not
+        T = Sampler(L, d["covar"], d["data"], seed=SEED, nchains=nchains, xtz_mode=2, z_store=2)
+        T.set_sweep_index(sweep)
+        for c in range(nchains):
+            T.set_spat_coef(scs[c], chain=c)
+            T.set_prec(prec0, chain=c)
+        T.sweep(1)
+        T.synchronize()
     for c in range(nchains):
         O.set_chain(c)
         z_orc = O.update_z(scs[c], sweep, method=method)
         m_gpu = S.get_xtz(c)
+        if T is not None:
+            assert not np.array_equal(m_gpu, T.get_xtz(c)), c  # the BF16 MFMA path ran
         scale = np.abs(O.covar).astype(np.float64) @ np.abs(z_orc)
         # the GPU draws are FP32 (1e-5 of the oracle's) and summed in FP32 (xtz_mode 3: as BF16 hi + lo
         # products, each operand within 2^-18 of its value)

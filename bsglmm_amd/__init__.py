@@ -180,7 +180,13 @@ def default_opts():
 
 
 class Sampler:
-    """One sampler context: device state for nchains chains over one data set."""
+    """One sampler context: device state for nchains chains over one data set.
+
+    The options are bsg_opts' (include/bsglmm.h). X'Z forms: xtz_mode 0, an
+    FP32 FMA kernel reading Z; 1, BF16 hi + lo on the matrix cores reading Z
+    (the precision form with Z stored); 2, FP32 FMA inside the latent draws;
+    3, BF16 hi + lo on the matrix cores inside the latent draws. Modes 2 and 3
+    keep no Z with z_store=2 (mode 3 requires it)."""
 
     KERNELS = ("latent", "xtz", "coef0", "coef1", "pairs", "wishart")
 

@@ -427,7 +427,7 @@ def test_wishart_carried_by_next_latent_launch(nsubs, ncovar, nchains):
 
 
 @pytest.mark.parametrize("z_store,xtz_mode,wm,nsubs", [(2, 2, False, None), (0, 1, True, None), (2, 2, False, 150),
-                                                       (2, 2, True, 150)])
+                                                       (2, 2, True, 150), (2, 3, False, 150)])
 def test_graph_replays_equal_direct_sweeps(z_store, xtz_mode, wm, nsubs):
     """Sweeps replayed from a captured HIP graph (device sweep base advanced by
     the graph's last node) give bit-identical state to direct launches, over

@@ -285,7 +285,7 @@ def config_line(name, forms, steps, warmup):
     return out
 
 
-def chain_summaries(L, d, chains, iters, burnin, nbatches=40):
+def chain_summaries(L, d, chains, iters, burnin, nbatches=40, xtz_mode=2):
     """`chains` chains of the Z-free fused sweep in one context (every kernel
     launch covers all chains); after burn-in, every draw goes to the device
     accumulators. Returns the sampling wall time (burn-in excluded), the
@@ -295,7 +295,7 @@ def chain_summaries(L, d, chains, iters, burnin, nbatches=40):
     import numpy as np
 
     from bsglmm_amd import Sampler
-    S = Sampler(L, d["covar"], d["data"], seed=0x5EED5EED12345678, nchains=chains, xtz_mode=2, z_store=2)
+    S = Sampler(L, d["covar"], d["data"], seed=0x5EED5EED12345678, nchains=chains, xtz_mode=xtz_mode, z_store=2)
     t0 = time.perf_counter()
     S.sweep(burnin)
     S.synchronize()
@@ -353,6 +353,15 @@ def batched_vs_plain(L, d, chains, iters, burnin):
                       "rhat_max": float(rh.max()), "rhat_frac_above_1_01": float(np.mean(rh > 1.01))})
         out["maps"].append(m)
     out["ess_per_s_ratio_min_over_maps"] = float(min(ratios))
+    # the same chains in the mixed-precision form (X'Z as BF16 hi + lo products on
+    # the matrix cores inside the draws) against the same plain FP32 chain
+    mp = chain_summaries(L, d, chains, iters, burnin, xtz_mode=3)
+    mr = [(np.median(mp["ess"][:, c].sum(0)) / mp["wall"]) / (np.median(plain["ess"][0, c]) / plain["wall"])
+          for c in range(P)]
+    out["mixed_precision"] = {"xtz_mode": 3, "batched_chain_sweeps_per_s": chains * mp["draws"] / mp["wall"],
+                              "ess_per_s_ratio_by_map": [float(r) for r in mr],
+                              "ess_per_s_ratio_min_over_maps": float(min(mr)),
+                              "rhat_max": float(mp["rhat"].max()) if mp["rhat"] is not None else None}
     return out
 
 

@@ -563,7 +563,7 @@ def main():
     if not head["overlapped"]:
         raise RuntimeError("the headline workload must run overlapped")
     (copy_gbs, copy_ms), (fma_rate, _), (trans_rate, _), (sampler_dps, _) = head["settle"]
-    # overlap None: the context's default (on up to P = 6)
+    # overlap None: the context's default (on wherever supported)
     forms = {"z_free_serial": (2, 2, False), "fused_xtz_z_stored": (2, 0, True), "separate_xtz_fp32": (0, 0, True),
              "separate_xtz_mfma_bf16x2": (1, 0, True), "fused_xtz_mfma_bf16x2_z_free": (3, 2, None)}
     others = {k: time_mode(L, d, m, z, a.steps, a.warmup, o) for k, (m, z, o) in forms.items()}

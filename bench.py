@@ -353,6 +353,24 @@ def batched_vs_plain(L, d, chains, iters, burnin):
                       "rhat_max": float(rh.max()), "rhat_frac_above_1_01": float(np.mean(rh > 1.01))})
         out["maps"].append(m)
     out["ess_per_s_ratio_min_over_maps"] = float(min(ratios))
+    # the chains' sweeps as a hipGraph-captured iteration: replays of a prepared
+    # graph of 50 sweeps against the same number of direct sweeps (no accumulation)
+    from bsglmm_amd import Sampler
+    G = Sampler(L, d["covar"], d["data"], seed=0x5EED5EED12345678, nchains=chains, xtz_mode=2, z_store=2)
+    G.prepare_graph(50)
+    G.sweep(50)
+    G.synchronize()
+    t0 = time.perf_counter()
+    G.sweep_graph(500, 50)
+    G.synchronize()
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    G.sweep(500)
+    G.synchronize()
+    t_direct = time.perf_counter() - t0
+    G.close()
+    out["graph_replay"] = {"sweeps": 500, "per_graph": 50, "chain_sweeps_per_s": chains * 500 / t_graph,
+                           "direct_chain_sweeps_per_s": chains * 500 / t_direct}
     # the same chains in the mixed-precision form (X'Z as BF16 hi + lo products on
     # the matrix cores inside the draws) against the same plain FP32 chain
     mp = chain_summaries(L, d, chains, iters, burnin, xtz_mode=3)

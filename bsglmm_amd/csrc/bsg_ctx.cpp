@@ -42,6 +42,9 @@ struct bsg_ctx {
   // spill); set_overlap
   bool overlap = true;
   bool overlap_ran = false;  // flags[2] (a Lambda wait past its bound) to check
+  // pinned host copy of flags[0..3]: bsg_synchronize queues its read behind the
+  // context's work and waits once for both
+  uint32_t *flags_host = nullptr;
   bsg::DevProblem pb{};
   std::vector<void *> allocs;
   hipEvent_t ev[7] = {};
@@ -91,6 +94,8 @@ struct bsg_ctx {
     free_accum();
     for (void *p : allocs) hipFree(p);
     allocs.clear();
+    if (flags_host) hipHostFree(flags_host);
+    flags_host = nullptr;
     for (auto &e : ev) {
       if (e) hipEventDestroy(e);
       e = nullptr;
@@ -344,6 +349,7 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   // ---- device state ----
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  HIPCHK(hipHostMalloc((void **)&ctx->flags_host, 4 * sizeof(uint32_t), hipHostMallocDefault));
   for (hipEvent_t *e : {&ctx->e_c0, &ctx->e_c1})
     HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   bsg::DevProblem &pb = ctx->pb;
@@ -1357,10 +1363,13 @@ int bsg_accum_get(bsg_ctx *ctx, int chain, int *ndraws, double *mean, double *va
 int bsg_synchronize(bsg_ctx *ctx)
 {
   CTXCHK(ctx);
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // the second stream's work first; then the flags' read queued behind the
+  // main stream's work, so that one wait covers both
   HIPCHK(hipStreamSynchronize(ctx->stream2));
+  HIPCHK(hipMemcpyAsync(ctx->flags_host, ctx->pb.flags, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   uint32_t flags[4];
-  HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
+  memcpy(flags, ctx->flags_host, sizeof(flags));
   if (flags[2]) {
     // first: a wait that gave up leaves colour 0 with an incomplete precision,
     // which can also show as the failure below; both are reported here, once,

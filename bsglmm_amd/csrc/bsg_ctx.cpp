@@ -349,7 +349,11 @@ int bsg_create(const bsg_layout *layout, int nsubs, int ncovar, const float *cov
   // ---- device state ----
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
-  HIPCHK(hipHostMalloc((void **)&ctx->flags_host, 4 * sizeof(uint32_t), hipHostMallocDefault));
+  // pinned words for bsg_synchronize's read-back; without them it copies after the wait
+  if (hipHostMalloc((void **)&ctx->flags_host, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    ctx->flags_host = nullptr;
+    (void)hipGetLastError();
+  }
   for (hipEvent_t *e : {&ctx->e_c0, &ctx->e_c1})
     HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   bsg::DevProblem &pb = ctx->pb;
@@ -1366,10 +1370,15 @@ int bsg_synchronize(bsg_ctx *ctx)
   // the second stream's work first; then the flags' read queued behind the
   // main stream's work, so that one wait covers both
   HIPCHK(hipStreamSynchronize(ctx->stream2));
-  HIPCHK(hipMemcpyAsync(ctx->flags_host, ctx->pb.flags, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
   uint32_t flags[4];
-  memcpy(flags, ctx->flags_host, sizeof(flags));
+  if (ctx->flags_host) {
+    HIPCHK(hipMemcpyAsync(ctx->flags_host, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(flags, ctx->flags_host, sizeof(flags));
+  } else {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(flags, ctx->pb.flags, sizeof(flags), hipMemcpyDeviceToHost));
+  }
   if (flags[2]) {
     // first: a wait that gave up leaves colour 0 with an incomplete precision,
     // which can also show as the failure below; both are reported here, once,

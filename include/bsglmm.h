@@ -245,7 +245,7 @@ int bsg_set_overlap(bsg_ctx *ctx, int on);
 int bsg_get_overlap(bsg_ctx *ctx, int *on);
 /* Persistent sweeps (no counterpart in the reference, mcmc.cpp:311-394 runs
  * a sweep's kernels one after another): bsg_sweep's sweeps of an overlapped
- * one-chain context with P <= 6 in one launch. A fixed grid of resident
+ * one-chain context with P <= 6 and FP32 X'Z (xtz_mode 2) in one launch. A fixed grid of resident
  * blocks takes the sweeps' latent tiles, colour updates and CAR-precision
  * draws from a queue in an order that puts every item after what it reads,
  * so no launch boundary drains the chip between phases. Same variates, same
